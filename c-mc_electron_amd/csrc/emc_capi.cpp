@@ -1,0 +1,957 @@
+// C ABI of the engine (include/emc.h): context, host tables, device residency,
+// initial population and the step orchestration on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/emc.h"
+#include "band.h"
+#include "device.h"
+#include "fly_core.h"
+#include "kernels.h"
+
+using namespace emc;
+
+namespace {
+
+struct HostParticles {
+  std::vector<double> x, y, z, kx, ky, kz, energy, charge;
+  std::vector<int32_t> i, j, k, itet, isym, ptype;
+  std::vector<uint64_t> key;
+  size_t size() const { return x.size(); }
+  void resize(size_t n) {
+    for (auto* v : {&x, &y, &z, &kx, &ky, &kz, &energy, &charge}) v->resize(n);
+    for (auto* v : {&i, &j, &k, &itet, &isym, &ptype}) v->resize(n);
+    key.resize(n);
+  }
+};
+
+struct DevBuf {
+  std::vector<void*> owned;
+  hipError_t last = hipSuccess;
+  template <class T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) { last = e; return nullptr; }
+    owned.push_back(p);
+    return (T*)p;
+  }
+  template <class T>
+  T* dup(const T* h, size_t n) {
+    T* d = alloc<T>(n);
+    if (d && n) {
+      hipError_t e = hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+      if (e != hipSuccess) last = e;
+    }
+    return d;
+  }
+  template <class T>
+  T* dup(const std::vector<T>& v) { return dup(v.data(), v.size()); }
+  void free_all() {
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+  }
+};
+
+struct Gpu {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  DevBuf mem;
+  BandDev B{};
+  MeshDev M{};
+  FieldsDev F{};
+  TallyDev T{};
+  PartDev P{};
+  long long n = 0, cap = 0;
+  CloneRec* clones = nullptr;
+  unsigned long long clone_cap = 0;
+  long long* dead = nullptr;
+  unsigned long long dead_cap = 0;
+  unsigned long long* ctr = nullptr;  // [0] nclones [1] ndead [2] ticket
+  // fields / mesh work arrays
+  double *ex = nullptr, *ey = nullptr, *ez = nullptr, *rho = nullptr, *rates = nullptr;
+  unsigned long long *node_q = nullptr, *node_stat = nullptr, *cell_q = nullptr, *cell_n = nullptr;
+  unsigned long long *heat_e = nullptr, *heat_h = nullptr, *rcur = nullptr, *lcur = nullptr;
+  unsigned long long *contact = nullptr, *counters = nullptr;
+  // Poisson
+  double *diag = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *dirichlet = nullptr, *corr = nullptr;
+  double *pvol = nullptr, *pdop = nullptr, *rhs = nullptr, *pot = nullptr;
+  double *r = nullptr, *p = nullptr, *q = nullptr, *part = nullptr, *part2 = nullptr, *part3 = nullptr, *scal = nullptr;
+  uint8_t* kind = nullptr;
+  const double* cvol = nullptr;
+  const double* dx = nullptr;
+  const double* dy = nullptr;
+  const double* dz = nullptr;
+  const double* da = nullptr;
+  const int* eeff = nullptr;
+  int pcg_blocks = 0;
+  hipEvent_t ev[8] = {};
+};
+
+}  // namespace
+
+struct emc_ctx {
+  std::string err;
+  emc_opts opts{};
+  Device dev;
+  Band band;
+  std::vector<TetRec> tetrec;
+  std::vector<int> surface_dir;
+  HostParticles hp;
+  std::unique_ptr<Gpu> gpu;
+  int last_pcg_iters = 0;
+  double last_relres = 0;
+  double timing[5] = {0, 0, 0, 0, 0};
+  uint64_t next_step = 0;
+};
+
+static int fail(emc_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHECK(ctx, expr)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, EMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+// ---------------------------------------------------------------- host band view
+static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
+  const Band& b = c->band;
+  B.tr = c->tetrec.data();
+  B.tet = b.tet.data();
+  B.xkk = b.xkk.data(); B.ykk = b.ykk.data(); B.zkk = b.zkk.data(); B.eek = b.eek.data();
+  B.vgt = b.vgt.data(); B.faclow = b.faclow.data(); B.facmed = b.facmedium.data(); B.fachigh = b.fachigh.data();
+  B.matsym = &b.matsym[0][0]; B.indmat = b.indmat; B.newisym = &b.newisym[0][0];
+  B.xbz = &b.xbz[0][0]; B.ybz = &b.ybz[0][0]; B.zbz = &b.zbz[0][0];
+  B.ntlist = b.ntlist.data(); B.ptlist = b.ptlist.data(); B.tlist = b.tlist.data();
+  B.nslist = b.nslist.data(); B.pslist = b.pslist.data(); B.slist = b.slist.data();
+  B.ntlists = b.ntlists.data(); B.ptlists = b.ptlists.data(); B.tlists = b.tlists.data();
+  B.ibt = b.ibt.data();
+  B.energy = b.energy.data(); B.dostetmax = b.dostetmax.data(); B.sumscatt = b.sumscatt.data();
+  B.scatte = b.scatte.data(); B.dose = b.dose.data(); B.scattiie = b.scattiie.data();
+  B.scatth = b.scatth.data(); B.dosh = b.dosh.data(); B.scattiih = b.scattiih.data();
+  B.nt = b.nt; B.scpre = b.scpre; B.scprh = b.scprh;
+  B.nbandE = b.nband[PELEC]; B.nbandH = b.nband[PHOLE];
+  B.bandofE = b.bandof[PELEC]; B.bandofH = b.bandof[PHOLE];
+  B.dtable = b.dtable; B.emax = b.emax; B.dlist = b.dlist; B.dlists = b.dlists;
+  B.a0pi = b.a0pi; B.sieg = b.sieg; B.mell = b.mell; B.melt = b.melt; B.meld = b.meld;
+  for (int q = 0; q < 3; ++q) B.difpr[q] = b.difpr[q];
+  B.temptag = b.temptag; B.templag = b.templag; B.templog = b.templog; B.temptaf = b.temptaf;
+  B.templaf = b.templaf; B.temptof = b.temptof; B.temphop = b.temphop;
+}
+
+static void fill_mesh_dev_scalars(emc_ctx* c, MeshDev& M) {
+  const Device& d = c->dev;
+  M.cnx = d.cnx; M.cny = d.cny; M.cnz = d.cnz; M.pnx = d.pnx; M.pny = d.pny; M.pnz = d.pnz;
+  M.ncontact = (int)d.contacts.size();
+  M.flag_surface = d.cfg.flag_surface; M.flag_simp = d.cfg.flag_simp;
+  M.dt = d.cfg.dt; M.eV0 = d.phys.eV0; M.conc0 = d.phys.conc0; M.scrt0 = d.phys.scrt0;
+  M.eps_si = d.phys.eps[SILICON]; M.frickel = d.phys.frickel;
+  for (int q = 0; q < 3; ++q) M.charge_sign[q] = d.phys.charge_sign[q];
+}
+
+static void build_tetrec(emc_ctx* c) {
+  const Band& b = c->band;
+  c->tetrec.assign(b.nt, TetRec{});
+  for (int it = 0; it < b.nt; ++it) {
+    TetRec& r = c->tetrec[it];
+    for (int f = 0; f < 4; ++f)
+      for (int q = 0; q < 4; ++q) r.n[f][q] = b.dn(q, f, it);
+    for (int q = 0; q < 4; ++q) r.vt[q] = b.vt[4 * (size_t)it + q];
+    r.gam = b.gamtet[it];
+    r.e3 = b.eek[b.tet_corner(3, it)];
+    for (int f = 0; f < 4; ++f) r.neib[f] = b.neib[(size_t)f * b.nt + it];
+    r.ibt = b.ibt[it];
+  }
+}
+
+// ---------------------------------------------------------------- GPU setup
+static int gpu_setup(emc_ctx* c) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->opts.device)
+    return fail(c, EMC_ERR_NOGPU, "no HIP device available for the requested ordinal");
+  c->gpu.reset(new Gpu());
+  Gpu& g = *c->gpu;
+  g.device = c->opts.device;
+  HIPCHECK(c, hipSetDevice(g.device));
+  HIPCHECK(c, hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  for (auto& e : g.ev) HIPCHECK(c, hipEventCreate(&e));
+  const Band& b = c->band;
+  const Device& d = c->dev;
+  DevBuf& m = g.mem;
+  BandDev& B = g.B;
+  fill_band_dev_host(c, B);  // scalars; pointers replaced below
+  B.tr = m.dup(c->tetrec);
+  B.tet = m.dup(b.tet);
+  B.xkk = m.dup(b.xkk); B.ykk = m.dup(b.ykk); B.zkk = m.dup(b.zkk); B.eek = m.dup(b.eek);
+  B.vgt = m.dup(b.vgt); B.faclow = m.dup(b.faclow); B.facmed = m.dup(b.facmedium); B.fachigh = m.dup(b.fachigh);
+  B.matsym = m.dup(&b.matsym[0][0], 6 * 48); B.indmat = m.dup(b.indmat, 729);
+  B.newisym = m.dup(&b.newisym[0][0], 5 * 48);
+  B.xbz = m.dup(&b.xbz[0][0], 5 * 48); B.ybz = m.dup(&b.ybz[0][0], 5 * 48); B.zbz = m.dup(&b.zbz[0][0], 5 * 48);
+  B.ntlist = m.dup(b.ntlist); B.ptlist = m.dup(b.ptlist); B.tlist = m.dup(b.tlist);
+  B.nslist = m.dup(b.nslist); B.pslist = m.dup(b.pslist); B.slist = m.dup(b.slist);
+  B.ntlists = m.dup(b.ntlists); B.ptlists = m.dup(b.ptlists); B.tlists = m.dup(b.tlists);
+  B.ibt = m.dup(b.ibt);
+  B.energy = m.dup(b.energy); B.dostetmax = m.dup(b.dostetmax); B.sumscatt = m.dup(b.sumscatt);
+  B.scatte = m.dup(b.scatte); B.dose = m.dup(b.dose); B.scattiie = m.dup(b.scattiie);
+  B.scatth = m.dup(b.scatth); B.dosh = m.dup(b.dosh); B.scattiih = m.dup(b.scattiih);
+
+  MeshDev& M = g.M;
+  fill_mesh_dev_scalars(c, M);
+  M.lx = m.dup(d.lx); M.ly = m.dup(d.ly); M.lz = m.dup(d.lz);
+  M.dx = m.dup(d.dx); M.dy = m.dup(d.dy); M.dz = m.dup(d.dz);
+  M.motion = m.dup(d.motion); M.attached = m.dup(d.attached); M.insurf = m.dup(d.insurf);
+  M.nearest_surf = m.dup(d.nearest_surf); M.da = m.dup(d.da);
+  M.surface_dir = m.dup(c->surface_dir);
+  g.dx = M.dx; g.dy = M.dy; g.dz = M.dz; g.da = M.da;
+  g.cvol = m.dup(d.cvol);
+  g.eeff = m.dup(d.eeff_dir);
+
+  const size_t NC = d.ncell, NN = d.nnode;
+  g.ex = m.alloc<double>(NC); g.ey = m.alloc<double>(NC); g.ez = m.alloc<double>(NC);
+  g.rho = m.alloc<double>(NC); g.rates = m.alloc<double>(4 * NC);
+  g.F.ex = g.ex; g.F.ey = g.ey; g.F.ez = g.ez;
+  g.F.hx = g.ex; g.F.hy = g.ey; g.F.hz = g.ez;  // QC off: holes see the same field (src/mcmodel.cpp:2247-2262)
+  g.F.rho = g.rho;
+  g.F.e_rough = g.rates; g.F.e_sph = g.rates + NC; g.F.h_rough = g.rates + 2 * NC; g.F.h_sph = g.rates + 3 * NC;
+  g.node_q = m.alloc<unsigned long long>(NN);
+  g.node_stat = m.alloc<unsigned long long>(10 * NN);
+  g.cell_q = m.alloc<unsigned long long>(3 * NC);
+  g.cell_n = m.alloc<unsigned long long>(NC);
+  g.heat_e = m.alloc<unsigned long long>(NN);
+  g.heat_h = m.alloc<unsigned long long>(NN);
+  g.rcur = m.alloc<unsigned long long>(d.pny);
+  g.lcur = m.alloc<unsigned long long>(d.pny);
+  g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
+  g.counters = m.alloc<unsigned long long>(CNT_N);
+  g.ctr = m.alloc<unsigned long long>(4);
+  g.T.contact = g.contact; g.T.counters = g.counters; g.T.rcur = g.rcur; g.T.lcur = g.lcur;
+  g.T.heat_e = g.heat_e; g.T.heat_h = g.heat_h;
+
+  g.diag = m.dup(d.diag); g.cx = m.dup(d.cx); g.cy = m.dup(d.cy); g.cz = m.dup(d.cz);
+  g.dirichlet = m.dup(d.dirichlet_val); g.corr = m.dup(d.rhs_corr); g.kind = m.dup(d.node_kind);
+  g.pvol = m.dup(d.pvol); g.pdop = m.dup(d.pdop);
+  g.rhs = m.alloc<double>(NN); g.pot = m.alloc<double>(NN);
+  g.r = m.alloc<double>(NN); g.p = m.alloc<double>(NN); g.q = m.alloc<double>(NN);
+  g.pcg_blocks = emc_pcg_blocks(NN);
+  g.part = m.alloc<double>(3 * g.pcg_blocks); g.part2 = m.alloc<double>(3 * g.pcg_blocks);
+  g.part3 = m.alloc<double>(3 * g.pcg_blocks); g.scal = m.alloc<double>(8);
+  if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
+  HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
+  HIPCHECK(c, hipMemset(g.ex, 0, NC * sizeof(double)));
+  HIPCHECK(c, hipMemset(g.ey, 0, NC * sizeof(double)));
+  HIPCHECK(c, hipMemset(g.ez, 0, NC * sizeof(double)));
+  HIPCHECK(c, hipMemset(g.rates, 0, 4 * NC * sizeof(double)));
+  std::vector<double> rho0(NC);
+  for (size_t q = 0; q < NC; ++q) rho0[q] = 0.5 * d.da[q];
+  HIPCHECK(c, hipMemcpy(g.rho, rho0.data(), NC * sizeof(double), hipMemcpyHostToDevice));
+  for (auto* pz : {g.node_stat}) HIPCHECK(c, hipMemset(pz, 0, 10 * NN * 8));
+  for (auto* pz : {g.heat_e, g.heat_h}) HIPCHECK(c, hipMemset(pz, 0, NN * 8));
+  for (auto* pz : {g.rcur, g.lcur}) HIPCHECK(c, hipMemset(pz, 0, d.pny * 8));
+  HIPCHECK(c, hipDeviceSynchronize());
+  return EMC_OK;
+}
+
+static int gpu_alloc_particles(emc_ctx* c, long long cap) {
+  Gpu& g = *c->gpu;
+  if (g.cap >= cap) return EMC_OK;
+  if (g.cap > 0) return fail(c, EMC_ERR_CAPACITY, "particle arrays are already allocated with a smaller capacity");
+  DevBuf& m = g.mem;
+  g.cap = cap;
+  g.P.x = m.alloc<double>(cap); g.P.y = m.alloc<double>(cap); g.P.z = m.alloc<double>(cap);
+  g.P.kx = m.alloc<double>(cap); g.P.ky = m.alloc<double>(cap); g.P.kz = m.alloc<double>(cap);
+  g.P.energy = m.alloc<double>(cap); g.P.charge = m.alloc<double>(cap);
+  g.P.pk = m.alloc<uint32_t>(cap); g.P.itet = m.alloc<int32_t>(cap); g.P.isym = m.alloc<int32_t>(cap);
+  g.P.key = m.alloc<uint64_t>(cap);
+  g.clone_cap = (unsigned long long)std::max<long long>(4096, cap / 16);
+  g.clones = m.alloc<CloneRec>(g.clone_cap);
+  g.dead_cap = g.clone_cap;
+  g.dead = m.alloc<long long>(g.dead_cap);
+  if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("particle allocation: ") + hipGetErrorString(m.last));
+  return EMC_OK;
+}
+
+// ---------------------------------------------------------------- initial population
+// init_particle_data / compute_par_num / select_kstate (src/particle_init.cpp:17-219,
+// src/mcmodel.cpp:3006-3057), with the per-particle Philox stream of the build
+// (draw stream step = 2^64-1 reserved for initialisation).
+static void sample_population(emc_ctx* c) {
+  const Device& d = c->dev;
+  const Band& b = c->band;
+  struct Spec { int i, j, k, type; double q; };
+  std::vector<Spec> spec;
+  std::vector<int> ne(d.ncell, 0), nh(d.ncell, 0);
+  for (int i = 0; i < d.cnx; ++i)
+    for (int k = 0; k < d.cnz; ++k)
+      for (int j = 0; j < d.cny; ++j) {
+        size_t cc = d.cell(i, j, k);
+        if (d.material[cc] != SILICON) continue;
+        ne[cc] = (int)(1e-10 + d.init_e_charge[cc] / d.sum_charge[0] * d.cfg.electron_number) + 1;
+        nh[cc] = (int)(1e-10 + d.init_h_charge[cc] / d.sum_charge[1] * d.cfg.hole_number) + 1;
+      }
+  for (int t = 0; t < 2; ++t)
+    for (int i = 0; i < d.cnx; ++i)
+      for (int k = 0; k < d.cnz; ++k)
+        for (int j = 0; j < d.cny; ++j) {
+          size_t cc = d.cell(i, j, k);
+          if (d.material[cc] != SILICON) continue;
+          const int pn = t == 0 ? ne[cc] : nh[cc];
+          const double q = (t == 0 ? d.init_e_charge[cc] : d.init_h_charge[cc]) / pn;
+          for (int a = 0; a < pn; ++a) spec.push_back(Spec{i, j, k, t, q});
+        }
+  HostParticles& hp = c->hp;
+  const size_t n = spec.size();
+  hp.resize(n);
+  BandDev B{};
+  fill_band_dev_host(c, B);
+  MeshDev M{};
+  fill_mesh_dev_scalars(c, M);
+  FieldsDev F{};
+  TallyDev T{};
+  const double tmp_const = 1 - std::exp(-b.emax);
+  const uint64_t seed = c->opts.seed;
+  auto work = [&](size_t lo, size_t hi) {
+    Flight f(B, M, F, T);
+    for (size_t s = lo; s < hi; ++s) {
+      const Spec& sp = spec[s];
+      const uint64_t key = init_key(seed, s);
+      f.rng.init(key, ~0ull);
+      double lam = f.Random();
+      hp.x[s] = lam * d.lx[sp.i] + (1 - lam) * d.lx[sp.i + 1];
+      lam = f.Random();
+      hp.y[s] = lam * d.ly[sp.j] + (1 - lam) * d.ly[sp.j + 1];
+      lam = f.Random();
+      hp.z[s] = lam * d.lz[sp.k] + (1 - lam) * d.lz[sp.k + 1];
+      f.par_type = sp.type;
+      f.itet = 0;
+      for (;;) {
+        f.energy = -dm_log(1 - f.Random() * tmp_const);
+        const double ddos = b.caldossum(f.energy, sp.type);
+        double rdos = f.Random() * b.DOSMAX[sp.type];
+        if (rdos > ddos) continue;
+        int ib = b.bandof[sp.type];
+        for (; ib < b.bandof[sp.type] + b.nband[sp.type]; ++ib) {
+          rdos -= b.caldos(f.energy, ib);
+          if (rdos < 0) break;
+        }
+        if (ib >= b.bandof[sp.type] + b.nband[sp.type]) ib = b.bandof[sp.type] + b.nband[sp.type] - 1;
+        f.iband = ib;
+        f.isym = ((int)(f.Random() * 48)) % 48;
+        f.get_state();
+        if (f.Flag_SelfScatter) continue;
+        break;
+      }
+      hp.kx[s] = f.kx; hp.ky[s] = f.ky; hp.kz[s] = f.kz; hp.energy[s] = f.energy;
+      hp.charge[s] = sp.q;
+      hp.i[s] = sp.i; hp.j[s] = sp.j; hp.k[s] = sp.k;
+      hp.itet[s] = f.itet; hp.isym[s] = f.isym; hp.ptype[s] = sp.type; hp.key[s] = key;
+    }
+  };
+  unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 20000) nth = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t) th.emplace_back(work, n * t / nth, n * (t + 1) / nth);
+  for (auto& t : th) t.join();
+}
+
+static int gpu_upload_host(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  const HostParticles& hp = c->hp;
+  const long long n = (long long)hp.size();
+  const long long want = (long long)std::ceil(n * std::max(1.0, c->opts.capacity_factor)) + 1024;
+  int st = gpu_alloc_particles(c, want);
+  if (st) return st;
+  if (n > g.cap) return fail(c, EMC_ERR_CAPACITY, "particle count exceeds the device capacity");
+  std::vector<uint32_t> pk(n);
+  for (long long s = 0; s < n; ++s)
+    pk[s] = hp.i[s] < 0 ? PK_DEAD : pk_make(hp.ptype[s], hp.i[s], hp.j[s], hp.k[s]);
+  auto up = [&](void* dptr, const void* h, size_t bytes) { return hipMemcpy(dptr, h, bytes, hipMemcpyHostToDevice); };
+  HIPCHECK(c, up(g.P.x, hp.x.data(), n * 8)); HIPCHECK(c, up(g.P.y, hp.y.data(), n * 8));
+  HIPCHECK(c, up(g.P.z, hp.z.data(), n * 8)); HIPCHECK(c, up(g.P.kx, hp.kx.data(), n * 8));
+  HIPCHECK(c, up(g.P.ky, hp.ky.data(), n * 8)); HIPCHECK(c, up(g.P.kz, hp.kz.data(), n * 8));
+  HIPCHECK(c, up(g.P.energy, hp.energy.data(), n * 8)); HIPCHECK(c, up(g.P.charge, hp.charge.data(), n * 8));
+  HIPCHECK(c, up(g.P.pk, pk.data(), n * 4)); HIPCHECK(c, up(g.P.itet, hp.itet.data(), n * 4));
+  HIPCHECK(c, up(g.P.isym, hp.isym.data(), n * 4)); HIPCHECK(c, up(g.P.key, hp.key.data(), n * 8));
+  g.n = n;
+  return EMC_OK;
+}
+
+// ---------------------------------------------------------------- public API
+extern "C" {
+
+void emc_default_opts(emc_opts* o) {
+  if (!o) return;
+  o->table_dir = nullptr;
+  o->device = 0;
+  o->seed = 1;
+  o->capacity_factor = 1.25;
+  o->poisson_rtol = 1e-10;
+  o->poisson_maxit = 20000;
+}
+
+emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
+  emc_ctx* c = new emc_ctx();
+  auto done = [&](int st) -> emc_ctx* {
+    if (status) *status = st;
+    return c;
+  };
+  if (!input_txt) return done(fail(c, EMC_ERR_ARG, "input path is NULL"));
+  if (o) c->opts = *o;
+  else emc_default_opts(&c->opts);
+  std::string err = device_parse(c->dev, input_txt);
+  if (!err.empty()) return done(fail(c, err.find("cannot open") != std::string::npos ? EMC_ERR_IO : EMC_ERR_DECK, err));
+  std::string tdir = c->opts.table_dir ? c->opts.table_dir : "";
+  if (tdir.empty()) return done(fail(c, EMC_ERR_ARG, "table_dir is required"));
+  err = band_build(c->band, c->dev.phys, tdir);
+  if (!err.empty()) return done(fail(c, EMC_ERR_IO, err));
+  err = device_finish(c->dev, c->band);
+  if (!err.empty()) return done(fail(c, EMC_ERR_DECK, err));
+  c->surface_dir.assign(std::max(1, c->dev.nsurf), 0);
+  for (int s = 0; s < c->dev.nsurf; ++s) c->surface_dir[s] = c->dev.surf_dir[s];
+  build_tetrec(c);
+  if (c->opts.device >= 0) {
+    int st = gpu_setup(c);
+    if (st) return done(st);
+  }
+  return done(EMC_OK);
+}
+
+void emc_destroy(emc_ctx* c) {
+  if (!c) return;
+  if (c->gpu) {
+    (void)hipSetDevice(c->gpu->device);
+    (void)hipDeviceSynchronize();
+    for (auto e : c->gpu->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (c->gpu->stream) (void)hipStreamDestroy(c->gpu->stream);
+    c->gpu->mem.free_all();
+  }
+  delete c;
+}
+
+const char* emc_last_error(emc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
+  if (!c || !v) return EMC_ERR_ARG;
+  const Band& b = c->band;
+  std::memset(v, 0, sizeof(*v));
+  v->nk = b.nk; v->nt = b.nt; v->nbt = b.nbt;
+  for (int q = 0; q < 3; ++q) { v->nband[q] = b.nband[q]; v->bandof[q] = b.bandof[q]; v->difpr[q] = b.difpr[q]; v->DOSMAX[q] = b.DOSMAX[q]; }
+  v->scpre = b.scpre; v->scprh = b.scprh;
+  v->dtable = b.dtable; v->emax = b.emax; v->dlist = b.dlist; v->dlists = b.dlists; v->a0pi = b.a0pi; v->sieg = b.sieg;
+  v->mell = b.mell; v->melt = b.melt; v->meld = b.meld;
+  v->temptag = b.temptag; v->templag = b.templag; v->templog = b.templog; v->temptaf = b.temptaf;
+  v->templaf = b.templaf; v->temptof = b.temptof; v->temphop = b.temphop; v->Ni = b.Ni;
+  v->xkk = b.xkk.data(); v->ykk = b.ykk.data(); v->zkk = b.zkk.data(); v->eek = b.eek.data();
+  v->tet = b.tet.data(); v->neib = b.neib.data(); v->ibt = b.ibt.data(); v->dnt = b.dnt.data(); v->vt = b.vt.data();
+  v->vgt = b.vgt.data(); v->gamtet = b.gamtet.data(); v->faclow = b.faclow.data(); v->facmedium = b.facmedium.data();
+  v->fachigh = b.fachigh.data(); v->maxaovtet = b.maxaovtet.data();
+  v->xkct = b.xkct.data(); v->ykct = b.ykct.data(); v->zkct = b.zkct.data();
+  v->matsym = &b.matsym[0][0]; v->indmat = b.indmat; v->newisym = &b.newisym[0][0];
+  v->xbz = &b.xbz[0][0]; v->ybz = &b.ybz[0][0]; v->zbz = &b.zbz[0][0];
+  v->ntlist = b.ntlist.data(); v->ptlist = b.ptlist.data(); v->tlist = b.tlist.data();
+  v->nslist = b.nslist.data(); v->pslist = b.pslist.data(); v->slist = b.slist.data();
+  v->ntlists = b.ntlists.data(); v->ptlists = b.ptlists.data(); v->tlists = b.tlists.data();
+  v->tlist_len = (int)b.tlist.size(); v->slist_len = (int)b.slist.size(); v->tlists_len = (int)b.tlists.size();
+  v->energy = b.energy.data(); v->dos = b.dos.data(); v->sumdos = b.sumdos.data(); v->dostetmax = b.dostetmax.data();
+  v->sumscatt = b.sumscatt.data(); v->scatte = b.scatte.data(); v->dose = b.dose.data(); v->scattiie = b.scattiie.data();
+  v->scatth = b.scatth.data(); v->dosh = b.dosh.data(); v->scattiih = b.scattiih.data();
+  return EMC_OK;
+}
+
+int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
+  if (!c || !v) return EMC_ERR_ARG;
+  const Device& d = c->dev;
+  const Phys& p = d.phys;
+  std::memset(v, 0, sizeof(*v));
+  v->cnx = d.cnx; v->cny = d.cny; v->cnz = d.cnz; v->pnx = d.pnx; v->pny = d.pny; v->pnz = d.pnz;
+  v->ncell = (int64_t)d.ncell; v->nnode = (int64_t)d.nnode;
+  v->ncontact = (int)d.contacts.size(); v->nsurf = d.nsurf;
+  v->dt = d.cfg.dt;
+  v->spr0 = p.spr0; v->time0 = p.time0; v->eV0 = p.eV0; v->pot0 = p.pot0; v->conc0 = p.conc0; v->scrt0 = p.scrt0;
+  v->field0 = p.field0; v->curr0 = p.curr0; v->velo0 = p.velo0;
+  v->eps_si = p.eps[SILICON]; v->eps_ox = p.eps[OXIDE]; v->frickel = p.frickel; v->Tn = p.Tn; v->T0 = p.T0;
+  v->em0 = p.em0; v->hq0 = p.hq0; v->ec0 = p.ec0; v->sirho = p.sirho;
+  v->surf_delta = p.SurfSc_delta; v->surf_ail = p.SurfSc_ail; v->surf_XIph = p.SurfSc_XIph;
+  v->surf_theta = p.SurfSc_theta; v->surf_gama = p.SurfSc_gama; v->surf_Nbmod = p.SurfSc_Nbmod;
+  v->surf_Npf = p.SurfSc_Npf; v->surf_Pft = p.SurfSc_Pft; v->surf_Pfn = p.SurfSc_Pfn; v->surf_Kpha = p.SurfSc_Kpha;
+  for (int q = 0; q < 3; ++q) { v->mass_eff[q] = p.mass_eff[q]; v->charge_sign[q] = p.charge_sign[q]; }
+  v->flag_surface = d.cfg.flag_surface; v->flag_rough = d.cfg.flag_rough; v->flag_sphonon = d.cfg.flag_sphonon;
+  v->flag_simp = d.cfg.flag_simp; v->flag_heat = d.cfg.flag_heat;
+  v->lx = d.lx.data(); v->ly = d.ly.data(); v->lz = d.lz.data(); v->dx = d.dx.data(); v->dy = d.dy.data(); v->dz = d.dz.data();
+  v->material = d.material.data(); v->attached = d.attached.data(); v->nearest_surf = d.nearest_surf.data();
+  v->eeff_dir = d.eeff_dir.data(); v->motion = d.motion.data(); v->insurf = d.insurf.data();
+  v->cvol = d.cvol.data(); v->da = d.da.data(); v->surface_dir = c->surface_dir.data();
+  v->pvol = d.pvol.data(); v->pdop = d.pdop.data(); v->vadd = d.vadd.data();
+  v->node_kind = d.node_kind.data(); v->dirichlet = d.dirichlet_val.data();
+  v->diag = d.diag.data(); v->cx = d.cx.data(); v->cy = d.cy.data(); v->cz = d.cz.data();
+  v->rhs_corr = d.rhs_corr.data(); v->stencil = d.stencil.data();
+  return EMC_OK;
+}
+
+int emc_particles_init(emc_ctx* c, int64_t* n_out) {
+  if (!c) return EMC_ERR_ARG;
+  sample_population(c);
+  if (n_out) *n_out = (int64_t)c->hp.size();
+  if (c->gpu) return gpu_upload_host(c);
+  return EMC_OK;
+}
+
+int emc_particles_count(emc_ctx* c, int64_t* n_out) {
+  if (!c || !n_out) return EMC_ERR_ARG;
+  *n_out = c->gpu ? c->gpu->n : (int64_t)c->hp.size();
+  return EMC_OK;
+}
+
+int emc_particles_upload(emc_ctx* c, const emc_particles* h, int64_t n) {
+  if (!c || !h || n < 0) return EMC_ERR_ARG;
+  HostParticles& hp = c->hp;
+  hp.resize((size_t)n);
+  for (int64_t s = 0; s < n; ++s) {
+    hp.x[s] = h->x[s]; hp.y[s] = h->y[s]; hp.z[s] = h->z[s]; hp.kx[s] = h->kx[s]; hp.ky[s] = h->ky[s];
+    hp.kz[s] = h->kz[s]; hp.energy[s] = h->energy[s]; hp.charge[s] = h->charge[s];
+    hp.i[s] = h->i[s]; hp.j[s] = h->j[s]; hp.k[s] = h->k[s]; hp.itet[s] = h->itet[s]; hp.isym[s] = h->isym[s];
+    hp.ptype[s] = h->ptype[s]; hp.key[s] = h->key[s];
+    const Device& d = c->dev;
+    if (hp.i[s] >= 0 && (hp.i[s] >= d.cnx || hp.j[s] < 0 || hp.j[s] >= d.cny || hp.k[s] < 0 || hp.k[s] >= d.cnz ||
+                         hp.itet[s] < 0 || hp.itet[s] >= c->band.nt || hp.isym[s] < 0 || hp.isym[s] >= 48 ||
+                         hp.ptype[s] < 0 || hp.ptype[s] > 1))
+      return fail(c, EMC_ERR_ARG, "particle " + std::to_string(s) + " has an out-of-range index");
+  }
+  if (c->gpu) return gpu_upload_host(c);
+  return EMC_OK;
+}
+
+int emc_particles_download(emc_ctx* c, emc_particles* h, int64_t cap, int64_t* n_out) {
+  if (!c || !h) return EMC_ERR_ARG;
+  if (!c->gpu) {
+    const HostParticles& hp = c->hp;
+    const int64_t n = (int64_t)hp.size();
+    if (n_out) *n_out = n;
+    if (n > cap) return fail(c, EMC_ERR_CAPACITY, "host buffer too small");
+    for (int64_t s = 0; s < n; ++s) {
+      h->x[s] = hp.x[s]; h->y[s] = hp.y[s]; h->z[s] = hp.z[s]; h->kx[s] = hp.kx[s]; h->ky[s] = hp.ky[s];
+      h->kz[s] = hp.kz[s]; h->energy[s] = hp.energy[s]; h->charge[s] = hp.charge[s];
+      h->i[s] = hp.i[s]; h->j[s] = hp.j[s]; h->k[s] = hp.k[s]; h->itet[s] = hp.itet[s]; h->isym[s] = hp.isym[s];
+      h->ptype[s] = hp.ptype[s]; h->key[s] = hp.key[s];
+    }
+    return EMC_OK;
+  }
+  Gpu& g = *c->gpu;
+  const long long n = g.n;
+  if (n_out) *n_out = n;
+  if (n > cap) return fail(c, EMC_ERR_CAPACITY, "host buffer too small");
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  std::vector<uint32_t> pk(n);
+  auto dn = [&](void* hptr, const void* dptr, size_t bytes) { return hipMemcpy(hptr, dptr, bytes, hipMemcpyDeviceToHost); };
+  HIPCHECK(c, dn(h->x, g.P.x, n * 8)); HIPCHECK(c, dn(h->y, g.P.y, n * 8)); HIPCHECK(c, dn(h->z, g.P.z, n * 8));
+  HIPCHECK(c, dn(h->kx, g.P.kx, n * 8)); HIPCHECK(c, dn(h->ky, g.P.ky, n * 8)); HIPCHECK(c, dn(h->kz, g.P.kz, n * 8));
+  HIPCHECK(c, dn(h->energy, g.P.energy, n * 8)); HIPCHECK(c, dn(h->charge, g.P.charge, n * 8));
+  HIPCHECK(c, dn(pk.data(), g.P.pk, n * 4)); HIPCHECK(c, dn(h->itet, g.P.itet, n * 4));
+  HIPCHECK(c, dn(h->isym, g.P.isym, n * 4)); HIPCHECK(c, dn(h->key, g.P.key, n * 8));
+  for (long long s = 0; s < n; ++s) {
+    if (pk[s] == PK_DEAD) { h->i[s] = -1; h->j[s] = -1; h->k[s] = -1; h->ptype[s] = -1; continue; }
+    h->i[s] = pk_i(pk[s]); h->j[s] = pk_j(pk[s]); h->k[s] = pk_k(pk[s]); h->ptype[s] = pk_type(pk[s]);
+  }
+  return EMC_OK;
+}
+
+int emc_fields_set(emc_ctx* c, const emc_cell_fields* f) {
+  if (!c || !f) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NC = c->dev.ncell;
+  if (f->hx != f->ex || f->hy != f->ey || f->hz != f->ez) {
+    // distinct hole fields: give them their own storage once
+    if (g.F.hx == g.ex) {
+      g.F.hx = g.mem.alloc<double>(NC); g.F.hy = g.mem.alloc<double>(NC); g.F.hz = g.mem.alloc<double>(NC);
+      if (g.mem.last != hipSuccess) return fail(c, EMC_ERR_HIP, "allocation of hole fields");
+    }
+    HIPCHECK(c, hipMemcpy((void*)g.F.hx, f->hx, NC * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy((void*)g.F.hy, f->hy, NC * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy((void*)g.F.hz, f->hz, NC * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHECK(c, hipMemcpy(g.ex, f->ex, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.ey, f->ey, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.ez, f->ez, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rho, f->rho, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rates, f->e_rough, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rates + NC, f->e_sph, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rates + 2 * NC, f->h_rough, NC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rates + 3 * NC, f->h_sph, NC * 8, hipMemcpyHostToDevice));
+  return EMC_OK;
+}
+
+int emc_fields_get(emc_ctx* c, emc_cell_fields* f) {
+  if (!c || !f) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NC = c->dev.ncell;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(f->ex, g.ex, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->ey, g.ey, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->ez, g.ez, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->hx, g.F.hx, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->hy, g.F.hy, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->hz, g.F.hz, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->rho, g.rho, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->e_rough, g.rates, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->e_sph, g.rates + NC, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->h_rough, g.rates + 2 * NC, NC * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(f->h_sph, g.rates + 3 * NC, NC * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- step pieces
+static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
+  Gpu& g = *c->gpu;
+  FlyArgs a{};
+  a.B = g.B; a.M = g.M; a.F = g.F; a.T = g.T; a.P = g.P;
+  if (!flag_heat) { a.T.heat_e = nullptr; a.T.heat_h = nullptr; }
+  a.n = g.n; a.cap = g.cap; a.step = step; a.flag_heat = flag_heat;
+  a.clones = g.clones; a.nclones = g.ctr; a.clone_cap = g.clone_cap;
+  a.dead = g.dead; a.ndead = g.ctr + 1; a.dead_cap = g.dead_cap;
+  return a;
+}
+
+// One flight step on the stream: fly, clone passes, merge.  Synchronises to size
+// the clone passes; updates g.n.
+static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, emc_launch_zero(g.ctr, 4, g.stream));
+  FlyArgs a = fly_args(c, step, flag_heat);
+  HIPCHECK(c, emc_launch_fly(a, g.stream));
+  unsigned long long h[4];
+  HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  unsigned long long begin = 0, end = std::min(h[0], g.clone_cap);
+  int passes = 0;
+  while (end > begin) {
+    HIPCHECK(c, emc_launch_fly_clones(a, begin, end, g.stream));
+    ++passes;
+    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    begin = end;
+    end = std::min(h[0], g.clone_cap);
+    if (passes > 64) return fail(c, EMC_ERR_STATE, "clone passes do not terminate");
+  }
+  const unsigned long long ndead = h[1];
+  if (end > 0) {
+    HIPCHECK(c, emc_launch_merge(a, end, ndead, g.ctr + 2, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    const unsigned long long surv = h[2];
+    const unsigned long long used = std::min(ndead, g.dead_cap);
+    if (surv > used) g.n = std::min<long long>(g.cap, g.n + (long long)(surv - used));
+  }
+  return passes;
+}
+
+static void read_tallies(emc_ctx* c, emc_fly_tallies* out, int passes) {
+  Gpu& g = *c->gpu;
+  unsigned long long ct[EMC_MAX_CONTACT * EMC_CT_NCOL], cn[CNT_N];
+  (void)hipMemcpy(ct, g.contact, sizeof(ct), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(cn, g.counters, sizeof(cn), hipMemcpyDeviceToHost);
+  if (!out) return;
+  for (int i = 0; i < EMC_MAX_CONTACT; ++i)
+    for (int j = 0; j < EMC_CT_NCOL; ++j) out->contact[i][j] += (int64_t)ct[i * EMC_CT_NCOL + j];
+  for (int r = 0; r < EMC_NREASON; ++r) out->drops[r] += (int64_t)cn[CNT_DROP0 + r];
+  out->n_catch += (int64_t)cn[CNT_CATCH];
+  out->n_gen += (int64_t)cn[CNT_GEN];
+  out->n_phonon += (int64_t)cn[CNT_PHONON];
+  out->n_impurity += (int64_t)cn[CNT_IMPURITY];
+  out->n_events += (int64_t)cn[CNT_EVENTS];
+  out->n_clone_passes += passes;
+}
+
+static int zero_tallies(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, emc_launch_zero(g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.counters, CNT_N, g.stream));
+  return EMC_OK;
+}
+
+static DepositArgs deposit_args(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  DepositArgs a{};
+  a.B = g.B; a.M = g.M; a.P = g.P; a.n = g.n;
+  a.nnode = c->dev.nnode; a.ncell = c->dev.ncell;
+  a.node_q = g.node_q; a.node_stat = g.node_stat; a.cell_q = g.cell_q; a.cell_n = g.cell_n;
+  return a;
+}
+
+static FieldArgs field_args(emc_ctx* c, const double* pot, bool rho_from_cells) {
+  Gpu& g = *c->gpu;
+  // the computed field serves both species (no quantum correction)
+  g.F.hx = g.ex; g.F.hy = g.ey; g.F.hz = g.ez;
+  const Device& d = c->dev;
+  const Phys& p = d.phys;
+  FieldArgs a{};
+  a.ncell = d.ncell; a.cnx = d.cnx; a.cny = d.cny; a.cnz = d.cnz; a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz;
+  a.pot = pot; a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.cvol = g.cvol; a.da = g.da; a.eeff_dir = g.eeff;
+  a.cell_q = rho_from_cells ? g.cell_q : nullptr;
+  a.ex = g.ex; a.ey = g.ey; a.ez = g.ez; a.rho = g.rho;
+  a.rates = d.cfg.flag_surface ? g.rates : nullptr;
+  a.flag_rough = d.cfg.flag_rough; a.flag_sphonon = d.cfg.flag_sphonon;
+  a.eps_si = p.eps[SILICON]; a.eps_ox = p.eps[OXIDE]; a.Tn = p.Tn; a.T0 = p.T0; a.field0 = p.field0;
+  a.em0 = p.em0; a.ec0 = p.ec0; a.hq0 = p.hq0; a.conc0 = p.conc0; a.scrt0 = p.scrt0; a.sirho = p.sirho;
+  a.surf_delta = p.SurfSc_delta; a.surf_ail = p.SurfSc_ail; a.surf_XIph = p.SurfSc_XIph;
+  a.surf_theta = p.SurfSc_theta; a.surf_gama = p.SurfSc_gama; a.surf_Nbmod = p.SurfSc_Nbmod;
+  a.surf_Npf = p.SurfSc_Npf; a.surf_Pft = p.SurfSc_Pft; a.surf_Pfn = p.SurfSc_Pfn; a.surf_Kpha = p.SurfSc_Kpha;
+  a.mass_eff[0] = p.mass_eff[0]; a.mass_eff[1] = p.mass_eff[1];
+  return a;
+}
+
+static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x) {
+  Gpu& g = *c->gpu;
+  const Device& d = c->dev;
+  PcgArgs a{};
+  a.n = d.nnode; a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz; a.nblocks = g.pcg_blocks;
+  a.diag = g.diag; a.cx = g.cx; a.cy = g.cy; a.cz = g.cz; a.b = b; a.kind = g.kind;
+  a.x = x; a.r = g.r; a.p = g.p; a.q = g.q;
+  a.part = g.part; a.part2 = g.part2; a.part3 = g.part3; a.scal = g.scal;
+  return a;
+}
+
+// Jacobi-PCG; convergence read back every `chk` iterations.
+static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres) {
+  Gpu& g = *c->gpu;
+  PcgArgs a = pcg_args(c, b, x);
+  HIPCHECK(c, emc_launch_pcg_start(a, g.stream));
+  double s[3];
+  HIPCHECK(c, hipMemcpyAsync(s, g.scal, sizeof(s), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  const double bn = std::sqrt(s[2]) > 0 ? std::sqrt(s[2]) : 1.0;
+  double rel = std::sqrt(s[0]) / bn;
+  int it = 0;
+  const int chk = 16;
+  while (rel > rtol && it < maxit) {
+    for (int q = 0; q < chk && it < maxit; ++q, ++it) HIPCHECK(c, emc_launch_pcg_iter(a, it, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(s, g.scal, sizeof(double) * 2, hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    rel = std::sqrt(std::max(0.0, s[0])) / bn;
+    if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
+  }
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  c->last_pcg_iters = it;
+  c->last_relres = rel;
+  if (rel > rtol) return fail(c, EMC_ERR_SOLVER, "PCG did not reach the tolerance");
+  return EMC_OK;
+}
+
+extern "C" {
+
+int emc_fly_step(emc_ctx* c, uint64_t step, int flag_heat, emc_fly_tallies* out) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the flight runs on the GPU only");
+  if (c->gpu->cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  if (out) std::memset(out, 0, sizeof(*out));
+  int st = zero_tallies(c);
+  if (st) return st;
+  int passes = run_fly(c, step, flag_heat);
+  if (passes < 0) return passes;
+  read_tallies(c, out, passes);
+  HIPCHECK(c, hipGetLastError());
+  return EMC_OK;
+}
+
+int emc_heat_get(emc_ctx* c, int64_t* heat_e, int64_t* heat_h) {
+  if (!c || !c->gpu) return EMC_ERR_ARG;
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  if (heat_e) HIPCHECK(c, hipMemcpy(heat_e, g.heat_e, c->dev.nnode * 8, hipMemcpyDeviceToHost));
+  if (heat_h) HIPCHECK(c, hipMemcpy(heat_h, g.heat_h, c->dev.nnode * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+int emc_currents_get(emc_ctx* c, int64_t* rcur, int64_t* lcur) {
+  if (!c || !c->gpu) return EMC_ERR_ARG;
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  if (rcur) HIPCHECK(c, hipMemcpy(rcur, g.rcur, c->dev.pny * 8, hipMemcpyDeviceToHost));
+  if (lcur) HIPCHECK(c, hipMemcpy(lcur, g.lcur, c->dev.pny * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+int emc_tallies_reset(emc_ctx* c) {
+  if (!c || !c->gpu) return EMC_ERR_ARG;
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  HIPCHECK(c, emc_launch_zero(g.heat_e, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.heat_h, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.rcur, c->dev.pny, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.lcur, c->dev.pny, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.node_stat, 10 * NN, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  return EMC_OK;
+}
+
+int emc_deposit(emc_ctx* c, int64_t* node_q, int64_t* node_stat, int64_t* cell_q, int64_t* cell_n) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode, NC = c->dev.ncell;
+  HIPCHECK(c, emc_launch_zero(g.node_q, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.node_stat, 10 * NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.cell_q, 3 * NC, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.cell_n, NC, g.stream));
+  HIPCHECK(c, emc_launch_deposit(deposit_args(c), g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  if (node_q) HIPCHECK(c, hipMemcpy(node_q, g.node_q, NN * 8, hipMemcpyDeviceToHost));
+  if (node_stat) HIPCHECK(c, hipMemcpy(node_stat, g.node_stat, 10 * NN * 8, hipMemcpyDeviceToHost));
+  if (cell_q) HIPCHECK(c, hipMemcpy(cell_q, g.cell_q, 3 * NC * 8, hipMemcpyDeviceToHost));
+  if (cell_n) HIPCHECK(c, hipMemcpy(cell_n, g.cell_n, NC * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+int emc_rhs_from_charge(emc_ctx* c, const int64_t* node_q, double* rhs_elim) {
+  if (!c || !node_q || !rhs_elim) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  HIPCHECK(c, hipMemcpy(g.node_q, node_q, NN * 8, hipMemcpyHostToDevice));
+  RhsArgs a{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
+  HIPCHECK(c, emc_launch_rhs(a, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(rhs_elim, g.rhs, NN * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+int emc_poisson_solve(emc_ctx* c, const double* rhs, double* sol, double rtol, int maxit, int* iters,
+                      double* relres) {
+  if (!c || !rhs || !sol) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  HIPCHECK(c, hipMemcpy(g.rhs, rhs, NN * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.pot, sol, NN * 8, hipMemcpyHostToDevice));
+  int st = run_pcg(c, g.rhs, g.pot, rtol > 0 ? rtol : c->opts.poisson_rtol, maxit > 0 ? maxit : c->opts.poisson_maxit,
+                   iters, relres);
+  HIPCHECK(c, hipMemcpy(sol, g.pot, NN * 8, hipMemcpyDeviceToHost));
+  return st;
+}
+
+int emc_compute_field(emc_ctx* c, const double* pot, const double* cell_rho) {
+  if (!c || !pot || !cell_rho) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, hipMemcpy(g.pot, pot, c->dev.nnode * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.rho, cell_rho, c->dev.ncell * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, emc_launch_field(field_args(c, g.pot, false), g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  return EMC_OK;
+}
+
+int emc_potential_get(emc_ctx* c, double* pot) {
+  if (!c || !pot || !c->gpu) return EMC_ERR_ARG;
+  HIPCHECK(c, hipStreamSynchronize(c->gpu->stream));
+  HIPCHECK(c, hipMemcpy(pot, c->gpu->pot, c->dev.nnode * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
+// density() + update_particle for nsteps (src/mcmodel.cpp:424-471): deposit the
+// current population, solve Poisson (warm start), fields and surface rates,
+// then fly.  Window statistics (node_stat) accumulate across calls.
+int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
+  if (!c || nsteps < 0) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the step runs on the GPU only");
+  Gpu& g = *c->gpu;
+  if (g.cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  const size_t NN = c->dev.nnode, NC = c->dev.ncell;
+  double tsum[5] = {0, 0, 0, 0, 0};
+  for (int s = 0; s < nsteps; ++s) {
+    const uint64_t step = c->next_step++;
+    HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
+    HIPCHECK(c, emc_launch_zero(g.node_q, NN, g.stream));
+    HIPCHECK(c, emc_launch_zero(g.cell_q, 3 * NC, g.stream));
+    HIPCHECK(c, emc_launch_zero(g.cell_n, NC, g.stream));
+    DepositArgs da = deposit_args(c);
+    HIPCHECK(c, emc_launch_deposit(da, g.stream));
+    RhsArgs ra{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
+    HIPCHECK(c, emc_launch_rhs(ra, g.stream));
+    HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
+    int st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr);
+    if (st) return st;
+    HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
+    HIPCHECK(c, emc_launch_field(field_args(c, g.pot, true), g.stream));
+    HIPCHECK(c, hipEventRecord(g.ev[3], g.stream));
+    st = zero_tallies(c);
+    if (st) return st;
+    int passes = run_fly(c, step, c->dev.cfg.flag_heat ? 1 : 0);
+    if (passes < 0) return passes;
+    HIPCHECK(c, hipEventRecord(g.ev[4], g.stream));
+    HIPCHECK(c, hipEventSynchronize(g.ev[4]));
+    float ms;
+    for (int q = 0; q < 4; ++q) {
+      HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[q], g.ev[q + 1]));
+      tsum[q] += ms;
+    }
+    read_tallies(c, acc, passes);
+  }
+  for (int q = 0; q < 5; ++q) c->timing[q] = nsteps ? tsum[q] / nsteps : 0;
+  return EMC_OK;
+}
+
+int emc_step_timing(emc_ctx* c, double* ms5, int* pcg_iters) {
+  if (!c) return EMC_ERR_ARG;
+  if (ms5)
+    for (int q = 0; q < 5; ++q) ms5[q] = c->timing[q];
+  if (pcg_iters) *pcg_iters = c->last_pcg_iters;
+  return EMC_OK;
+}
+
+int emc_run_file(const char* input_txt, const emc_opts* o) {
+  int st = 0;
+  emc_ctx* c = emc_create(input_txt, o, &st);
+  if (st) {
+    std::fprintf(stderr, "emc_create: %s\n", emc_last_error(c));
+    emc_destroy(c);
+    return st;
+  }
+  int64_t n = 0;
+  st = emc_particles_init(c, &n);
+  if (st) { std::fprintf(stderr, "init: %s\n", emc_last_error(c)); emc_destroy(c); return st; }
+  const int total = c->dev.cfg.total_step, stat = c->dev.cfg.stat_step;
+  emc_fly_tallies acc;
+  std::memset(&acc, 0, sizeof(acc));
+  FILE* fcur = std::fopen("data/current", "w");
+  for (int s = 0; s < total; ++s) {
+    st = emc_step(c, 1, &acc);
+    if (st) { std::fprintf(stderr, "step %d: %s\n", s, emc_last_error(c)); break; }
+    if (s % stat == stat - 1) {
+      // current_scatter_info (src/mcmodel.cpp:756-815): Current = (Gen - Catch)/(dt*stat_step)*curr0*1e-2
+      if (fcur) {
+        std::fprintf(fcur, "step = %d\n", s);
+        for (size_t q = 0; q < c->dev.contacts.size(); ++q) {
+          const double qg = acc.contact[q][EMC_CT_QGEN] / EMC_FX_CONTACT, qc = acc.contact[q][EMC_CT_QCATCH] / EMC_FX_CONTACT;
+          const double cur = (qg - qc) / (c->dev.cfg.dt * stat) * c->dev.phys.curr0 * 1e-2;
+          std::fprintf(fcur, "%zu %g %lld %g %lld %g %g %g %g\n", q, qg, (long long)acc.contact[q][EMC_CT_NGEN], qc,
+                       (long long)acc.contact[q][EMC_CT_NCATCH], acc.contact[q][EMC_CT_EGEN] / EMC_FX_CONTACT, 0.0, 0.0, cur);
+        }
+        std::fflush(fcur);
+      }
+      std::memset(&acc, 0, sizeof(acc));
+    }
+  }
+  if (fcur) std::fclose(fcur);
+  emc_destroy(c);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,845 @@
+// Per-particle full-band flight: update_particle / particle_fly and every
+// callee (src/mcmodel.cpp:1874-2161, src/TetHit.cpp, src/CellHit.cpp,
+// src/ParticleUtils.cpp:48-748, src/ElectronPhononScatter.cpp:5-497,
+// src/HolePhononScatter.cpp, src/ElectronImpurityScatter.cpp,
+// src/ParSurfaceScatter.cpp, src/heat.cpp:169-196).
+//
+// Written for one lane per particle: the state lives in registers, every table
+// read is a gather from HBM/L2/MALL, tallies are int64 atomics.  The same code
+// compiles for the host (initial sampling only uses get_state there).
+//
+// Semantics the reference leaves undefined or serial-order dependent, fixed here:
+//  * Random(): per particle Philox stream, counter (draw block, step); see devmath.h.
+//  * surface-phonon drand = Random()*CALSCATTSUM(E, band) (src/ParSurfaceScatter.cpp:72,124
+//    read an uninitialised drand).
+//  * SurfScGamma starts at 0 for each flight (a carried member in the reference).
+//  * sign(0) = +1; pow(x,2) = x*x; pow(x,-2) = 1/(x*x).
+//  * GENERATE clones fly the rest of the step in which they were made, with key
+//    clone_key(parent, step, ordinal); HitCell across the outer mesh face drops
+//    the particle (reason OFF_MESH) instead of re-flying it on a neighbour rank.
+//  * the event loop stops after 65536 iterations (reason LOOPS).
+#pragma once
+#include <stdint.h>
+
+#include "devmath.h"
+#include "emc_const.h"
+#include "tables.h"
+
+namespace emc {
+
+constexpr int FLY_LOOP_CAP = 65536;
+
+EMC_HD inline void tally_add(unsigned long long* p, long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd(p, (unsigned long long)v);
+#else
+  *p += (unsigned long long)v;
+#endif
+}
+
+EMC_HD inline long long fxd(double v, double scale) { return (long long)rint(v * scale); }
+
+struct Flight {
+  const BandDev& B;
+  const MeshDev& M;
+  const FieldsDev& F;
+  const TallyDev& T;
+  CloneRec* clones;
+  unsigned long long* nclones;
+  unsigned long long clone_cap;
+
+  // particle
+  double x, y, z, kx, ky, kz, energy, charge, left_time;
+  int icell, jcell, kcell, itet, isym, par_type, iband;
+  uint64_t key, step;
+  // flight
+  double vx, vy, vz, Ex, Ey, Ez, Rho, DA;
+  double ImpScGamma, SurfScGamma, SurfRoughnessScRate, SurfPhononScRate;
+  int itetdir, idir, Flag_Catch, Flag_SelfScatter, drop_reason, nclone, flag_heat;
+  long long n_events, n_phonon, n_impurity;
+  Rng rng;
+
+  EMC_HD Flight(const BandDev& b, const MeshDev& m, const FieldsDev& f, const TallyDev& t)
+      : B(b), M(m), F(f), T(t), clones(nullptr), nclones(nullptr), clone_cap(0) {}
+
+  EMC_HD double Random() { return rng.next(); }
+  EMC_HD int ms(int r, int s) const { return B.matsym[r * 48 + s]; }
+  EMC_HD int indmat(int sx, int sy, int sz, int p0, int p1, int p2) const {
+    return B.indmat[((((sx + 1) * 3 + (sy + 1)) * 3 + (sz + 1)) * 3 + p0) * 9 + p1 * 3 + p2];
+  }
+  EMC_HD size_t cell(int i, int j, int k) const { return ((size_t)j * M.cnx + i) * M.cnz + k; }
+  EMC_HD size_t node(int i, int j, int k) const { return ((size_t)j * M.pnx + i) * M.pnz + k; }
+  EMC_HD int corner(int c, int it) const { return B.tet[(size_t)c * B.nt + it]; }
+  EMC_HD void count(int slot) { tally_add(&T.counters[slot], 1); }
+
+  // in_wedge / OutWedge (src/ParticleUtils.cpp:293-305, src/Band.cpp:555-570)
+  EMC_HD void in_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
+    double v[3];
+    v[ms(0, isym) - 1] = xi * ms(3, isym);
+    v[ms(1, isym) - 1] = yi * ms(4, isym);
+    v[ms(2, isym) - 1] = zi * ms(5, isym);
+    xo = v[0]; yo = v[1]; zo = v[2];
+  }
+  EMC_HD void out_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
+    const double v[3] = {xi, yi, zi};
+    xo = v[ms(0, isym) - 1] * ms(3, isym);
+    yo = v[ms(1, isym) - 1] * ms(4, isym);
+    zo = v[ms(2, isym) - 1] * ms(5, isym);
+  }
+  EMC_HD void GetV() {
+    const TetRec& r = B.tr[itet];
+    out_wedge(vx, vy, vz, r.vt[0], r.vt[1], r.vt[2]);
+  }
+  EMC_HD void GetEnergy() {
+    double dx, dy, dz;
+    in_wedge(dx, dy, dz, kx, ky, kz);
+    const int c0 = corner(0, itet);
+    dx -= B.xkk[c0]; dy -= B.ykk[c0]; dz -= B.zkk[c0];
+    const TetRec& r = B.tr[itet];
+    energy = r.vt[3] + r.vt[0] * dx + r.vt[1] * dy + r.vt[2] * dz;
+  }
+
+  // ---- Band interpolators and areas (src/Band.cpp:194-352, 665-703, 1999-2321)
+  EMC_HD static double fii(double x1, double y1, double z1, double x2, double y2, double z2, double x3,
+                           double y3, double z3) {
+    const double sbx = (y2 - y1) * (z3 - z1) - (y3 - y1) * (z2 - z1);
+    const double sby = (x3 - x1) * (z2 - z1) - (x2 - x1) * (z3 - z1);
+    const double sbz = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
+    return 0.50 * sqrt(sbx * sbx + sby * sby + sbz * sbz);
+  }
+  EMC_HD double surf(double e, int it) const {
+    const int c0 = corner(0, it), c1 = corner(1, it), c2 = corner(2, it), c3 = corner(3, it);
+    const double *X = B.xkk, *Y = B.ykk, *Z = B.zkk;
+    const double e0 = B.eek[c0], e1 = B.eek[c1], e2 = B.eek[c2], e3 = B.eek[c3];
+    double r;
+    if (!(e > e0 && e < e3)) return 0.0;
+    if (e <= e1) {
+      r = (e - e0) / (e1 - e0);
+      const double ax = r * (X[c1] - X[c0]), ay = r * (Y[c1] - Y[c0]), az = r * (Z[c1] - Z[c0]);
+      r = (e - e0) / (e2 - e0);
+      const double bx = r * (X[c2] - X[c0]), by = r * (Y[c2] - Y[c0]), bz = r * (Z[c2] - Z[c0]);
+      r = (e - e0) / (e3 - e0);
+      const double cx = r * (X[c3] - X[c0]), cy = r * (Y[c3] - Y[c0]), cz = r * (Z[c3] - Z[c0]);
+      return fii(ax, ay, az, bx, by, bz, cx, cy, cz);
+    }
+    if (e > e1 && e < e2) {
+      r = (e - e0) / (e2 - e0);
+      const double ax = r * (X[c2] - X[c0]) + X[c0], ay = r * (Y[c2] - Y[c0]) + Y[c0], az = r * (Z[c2] - Z[c0]) + Z[c0];
+      r = (e - e1) / (e2 - e1);
+      const double bx = r * (X[c2] - X[c1]) + X[c1], by = r * (Y[c2] - Y[c1]) + Y[c1], bz = r * (Z[c2] - Z[c1]) + Z[c1];
+      r = (e - e0) / (e3 - e0);
+      const double cx = r * (X[c3] - X[c0]) + X[c0], cy = r * (Y[c3] - Y[c0]) + Y[c0], cz = r * (Z[c3] - Z[c0]) + Z[c0];
+      r = (e - e1) / (e3 - e1);
+      const double dx = r * (X[c3] - X[c1]) + X[c1], dy = r * (Y[c3] - Y[c1]) + Y[c1], dz = r * (Z[c3] - Z[c1]) + Z[c1];
+      const double a1 = fii(ax, ay, az, bx, by, bz, cx, cy, cz);
+      const double a2 = fii(dx, dy, dz, bx, by, bz, cx, cy, cz);
+      const double a3 = fii(ax, ay, az, dx, dy, dz, cx, cy, cz);
+      const double a4 = fii(ax, ay, az, bx, by, bz, dx, dy, dz);
+      return 0.50 * (a1 + a2 + a3 + a4);
+    }
+    if (e > e2 && e < e3) {
+      r = (e - e3) / (e0 - e3);
+      const double ax = r * (X[c0] - X[c3]), ay = r * (Y[c0] - Y[c3]), az = r * (Z[c0] - Z[c3]);
+      r = (e - e3) / (e1 - e3);
+      const double bx = r * (X[c1] - X[c3]), by = r * (Y[c1] - Y[c3]), bz = r * (Z[c1] - Z[c3]);
+      r = (e - e3) / (e2 - e3);
+      const double cx = r * (X[c2] - X[c3]), cy = r * (Y[c2] - Y[c3]), cz = r * (Z[c2] - Z[c3]);
+      return fii(ax, ay, az, bx, by, bz, cx, cy, cz);
+    }
+    return 0.0;
+  }
+  EMC_HD double fastsurf(double e, int it) {
+    const double e0 = B.eek[corner(0, it)], e1 = B.eek[corner(1, it)], e2 = B.eek[corner(2, it)],
+                 e3 = B.eek[corner(3, it)];
+    double r = 0.0;
+    if (e > e0 && e < e3) {
+      if (e <= e1)
+        r = B.faclow[it] * (e - e0) * (e - e0);
+      else if (e < e2) {
+        if (e1 - e0 > 1e-10)
+          r = B.faclow[it] * (e - e0) * (e - e0) - B.facmed[it] * (e - e1) * (e - e1);
+        else
+          r = surf(e, it);
+      } else
+        r = B.fachigh[it] * (e - e3) * (e - e3);
+    }
+    if (r < 0.0) {
+      if (T.counters) count(CNT_DROP0 + 6);
+      r = 0.0;
+    }
+    return r;
+  }
+  EMC_HD double calscattsum(double e, int ib) const {
+    if (e <= 0.0) return B.sumscatt[ib];
+    if (e >= B.emax) return B.sumscatt[(size_t)MTAB * MNB + ib];
+    const int itab = (int)(e / B.dtable);
+    const double intp = (e - B.energy[itab]) / (B.energy[itab + 1] - B.energy[itab]);
+    return B.sumscatt[(size_t)itab * MNB + ib] +
+           intp * (B.sumscatt[(size_t)(itab + 1) * MNB + ib] - B.sumscatt[(size_t)itab * MNB + ib]);
+  }
+  EMC_HD double caldostetmax(double e, int ib) const {
+    const double* d = &B.dostetmax[(size_t)ib * NTAB1];
+    if (e <= 0.0) return d[0];
+    if (e >= B.emax) return d[MTAB];
+    const int itab = (int)(e / B.dtable);
+    return d[itab] > d[itab + 1] ? d[itab] : d[itab + 1];
+  }
+  EMC_HD void confbz(double& a, double& b, double& c) const {
+    const double q = B.a0pi;
+    for (int guard = 0; guard < 1024; ++guard) {  // a finite k folds in a few passes
+      const bool out = fabs(a) > q || fabs(b) > q || fabs(c) > q || fabs(a) + fabs(b) + fabs(c) > 1.50 * q;
+      if (!out) return;
+      a = a > 0.0 ? a - q : a + q;
+      b = b > 0.0 ? b - q : b + q;
+      c = c > 0.0 ? c - q : c + q;
+    }
+  }
+
+  // ---- tetrahedron search (src/ParticleUtils.cpp:331-625)
+  EMC_HD bool contains(int it, double a, double b, double c) const {
+    const TetRec& r = B.tr[it];
+    const double d1 = r.n[0][0] * a + r.n[0][1] * b + r.n[0][2] * c - r.n[0][3];
+    const double d2 = r.n[1][0] * a + r.n[1][1] * b + r.n[1][2] * c - r.n[1][3];
+    const double d3 = r.n[2][0] * a + r.n[2][1] * b + r.n[2][2] * c - r.n[2][3];
+    const double d4 = r.n[3][0] * a + r.n[3][1] * b + r.n[3][2] * c - r.n[3][3];
+    return (d1 >= -MY_ZERO) && (d2 >= -MY_ZERO) && (d3 >= -MY_ZERO) && (d4 >= -MY_ZERO);
+  }
+  EMC_HD bool scan(const int* list, int p0, int nn, double a, double b, double c) {
+    for (int itl = p0; itl <= p0 + nn - 1; ++itl) {
+      const int it = list[itl];
+      if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return true; }
+    }
+    return false;
+  }
+  EMC_HD void Getisym() {
+    int pos[3] = {1, 1, 1}, mink, maxk;
+    double tmin, tmax;
+    if (fabs(kx) > fabs(ky)) { tmin = fabs(ky); tmax = fabs(kx); mink = 1; maxk = 0; }
+    else { tmin = fabs(kx); tmax = fabs(ky); mink = 0; maxk = 1; }
+    if (fabs(kz) < tmin) mink = 2;
+    else if (fabs(kz) > tmax) maxk = 2;
+    pos[mink] = 2;
+    pos[maxk] = 0;
+    isym = indmat(kx < 0 ? -1 : 1, ky < 0 ? -1 : 1, kz < 0 ? -1 : 1, pos[0], pos[1], pos[2]);
+    double a, b, c;
+    in_wedge(a, b, c, kx, ky, kz);
+    if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
+    int ie = (int)(energy * B.dlists);
+    if (ie < 0) ie = 0;
+    if (ie < MWLES && scan(B.tlists, B.ptlists[ie * MNB + iband], B.ntlists[ie * MNB + iband], a, b, c)) return;
+    ie = (int)(energy * B.dlist);
+    if (ie < 0) ie = 0;
+    if (ie >= MWLE) ie = MWLE - 1;
+    if (scan(B.tlist, B.ptlist[ie * MNB + iband], B.ntlist[ie * MNB + iband], a, b, c)) return;
+    if (ie + 1 < MWLE && scan(B.tlist, B.ptlist[(ie + 1) * MNB + iband], B.ntlist[(ie + 1) * MNB + iband], a, b, c))
+      return;
+    if (ie - 1 >= 0 && scan(B.tlist, B.ptlist[(ie - 1) * MNB + iband], B.ntlist[(ie - 1) * MNB + iband], a, b, c))
+      return;
+    for (int it = 0; it < B.nt; ++it)
+      if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return; }
+    Flag_Catch = 1;
+    drop_reason = EMC_DROP_NO_TET_;
+  }
+  static constexpr int EMC_DROP_NO_TET_ = 2;
+  EMC_HD void GetisymBZ() {
+    double a, b, c;
+    in_wedge(a, b, c, kx, ky, kz);
+    if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
+    int ie = (int)(energy * B.dlist);
+    if (ie < 0) ie = 0;
+    if (ie >= MWLE) ie = MWLE - 1;
+    if (scan(B.slist, B.pslist[ie * MNB + iband], B.nslist[ie * MNB + iband], a, b, c)) return;
+    Getisym();
+  }
+
+  // ---- final state on the iso-energy surface (src/ParticleUtils.cpp:48-284)
+  EMC_HD void getK() {
+    const int it = itet;
+    const int c0 = corner(0, it), c1 = corner(1, it), c2 = corner(2, it), c3 = corner(3, it);
+    const double *X = B.xkk, *Y = B.ykk, *Z = B.zkk, *E = B.eek;
+    const double rv1 = 1.0 - sqrt(Random());
+    const double rv2 = Random() * (1.0 - rv1);
+    if ((energy < E[c0]) || (energy > E[c3])) energy = E[c0] + (E[c3] - E[c0]) * Random();
+    const double e = energy;
+    double r, xkl, ykl, zkl;
+    if (e <= E[c1]) {
+      r = (e - E[c0]) / (E[c1] - E[c0]);
+      const double ax = r * (X[c1] - X[c0]), ay = r * (Y[c1] - Y[c0]), az = r * (Z[c1] - Z[c0]);
+      r = (e - E[c0]) / (E[c2] - E[c0]);
+      const double bx = r * (X[c2] - X[c0]), by = r * (Y[c2] - Y[c0]), bz = r * (Z[c2] - Z[c0]);
+      r = (e - E[c0]) / (E[c3] - E[c0]);
+      const double cx = r * (X[c3] - X[c0]), cy = r * (Y[c3] - Y[c0]), cz = r * (Z[c3] - Z[c0]);
+      xkl = rv1 * (bx - ax) + rv2 * (cx - ax) + ax + X[c0];
+      ykl = rv1 * (by - ay) + rv2 * (cy - ay) + ay + Y[c0];
+      zkl = rv1 * (bz - az) + rv2 * (cz - az) + az + Z[c0];
+    } else if (e < E[c2]) {
+      r = (e - E[c0]) / (E[c2] - E[c0]);
+      const double ax = r * (X[c2] - X[c0]) + X[c0], ay = r * (Y[c2] - Y[c0]) + Y[c0], az = r * (Z[c2] - Z[c0]) + Z[c0];
+      r = (e - E[c1]) / (E[c2] - E[c1]);
+      const double bx = r * (X[c2] - X[c1]) + X[c1], by = r * (Y[c2] - Y[c1]) + Y[c1], bz = r * (Z[c2] - Z[c1]) + Z[c1];
+      r = (e - E[c0]) / (E[c3] - E[c0]);
+      const double cx = r * (X[c3] - X[c0]) + X[c0], cy = r * (Y[c3] - Y[c0]) + Y[c0], cz = r * (Z[c3] - Z[c0]) + Z[c0];
+      r = (e - E[c1]) / (E[c3] - E[c1]);
+      const double dx = r * (X[c3] - X[c1]) + X[c1], dy = r * (Y[c3] - Y[c1]) + Y[c1], dz = r * (Z[c3] - Z[c1]) + Z[c1];
+      const double a1 = fii(ax, ay, az, bx, by, bz, cx, cy, cz);
+      const double a2 = fii(dx, dy, dz, bx, by, bz, cx, cy, cz);
+      const double a3 = fii(ax, ay, az, dx, dy, dz, cx, cy, cz);
+      const double a4 = fii(ax, ay, az, bx, by, bz, dx, dy, dz);
+      const double ar = Random() * (a1 + a2 + a3 + a4);
+      double px, py, pz, qx, qy, qz, ox, oy, oz;  // triangle (origin o, edges p-o, q-o)
+      if (ar < a1) { ox = ax; oy = ay; oz = az; px = bx; py = by; pz = bz; qx = cx; qy = cy; qz = cz; }
+      else if (ar < (a1 + a2)) { ox = dx; oy = dy; oz = dz; px = bx; py = by; pz = bz; qx = cx; qy = cy; qz = cz; }
+      else if (ar < (a1 + a2 + a3)) { ox = ax; oy = ay; oz = az; px = dx; py = dy; pz = dz; qx = cx; qy = cy; qz = cz; }
+      else { ox = ax; oy = ay; oz = az; px = bx; py = by; pz = bz; qx = dx; qy = dy; qz = dz; }
+      xkl = rv1 * (px - ox) + rv2 * (qx - ox) + ox;
+      ykl = rv1 * (py - oy) + rv2 * (qy - oy) + oy;
+      zkl = rv1 * (pz - oz) + rv2 * (qz - oz) + oz;
+    } else {
+      r = (e - E[c3]) / (E[c0] - E[c3]);
+      const double ax = r * (X[c0] - X[c3]), ay = r * (Y[c0] - Y[c3]), az = r * (Z[c0] - Z[c3]);
+      r = (e - E[c3]) / (E[c1] - E[c3]);
+      const double bx = r * (X[c1] - X[c3]), by = r * (Y[c1] - Y[c3]), bz = r * (Z[c1] - Z[c3]);
+      r = (e - E[c3]) / (E[c2] - E[c3]);
+      const double cx = r * (X[c2] - X[c3]), cy = r * (Y[c2] - Y[c3]), cz = r * (Z[c2] - Z[c3]);
+      xkl = rv1 * (bx - ax) + rv2 * (cx - ax) + ax + X[c3];
+      ykl = rv1 * (by - ay) + rv2 * (cy - ay) + ay + Y[c3];
+      zkl = rv1 * (bz - az) + rv2 * (cz - az) + az + Z[c3];
+    }
+    out_wedge(kx, ky, kz, xkl, ykl, zkl);
+  }
+
+  EMC_HD void get_state() {
+    const double e = energy;
+    if (e <= 0.0) { Flag_SelfScatter = 1; return; }
+    const double maxdostet = caldostetmax(e, iband);
+    int itab = (int)(e * B.dlists);
+    const bool smallfl = itab < MWLES;
+    if (!smallfl) {
+      itab = (int)(e * B.dlist);
+      if (itab >= MWLE) itab = MWLE - 1;
+    }
+    const int nted = smallfl ? B.ntlists[itab * MNB + iband] : B.ntlist[itab * MNB + iband];
+    const int base = smallfl ? B.ptlists[itab * MNB + iband] : B.ptlist[itab * MNB + iband];
+    const int* list = smallfl ? B.tlists : B.tlist;
+    if (nted == 0) { Flag_SelfScatter = 1; return; }
+    int count2 = 0;
+    for (;;) {
+      itet = list[(int)(nted * Random()) + base];
+      count2++;
+      if ((count2 < 10000) && !((B.tr[itet].vt[3] < e) && (B.tr[itet].e3 > e))) continue;
+      const double dostet = fastsurf(e, itet) / B.vgt[itet];
+      if (((Random() * maxdostet) > dostet) && (count2 < 10000)) continue;
+      break;
+    }
+    if (count2 >= 10000) Flag_SelfScatter = 1;
+    else { Flag_SelfScatter = 0; getK(); }
+  }
+
+  // TALIND (src/ParticleUtils.cpp:626-748)
+  EMC_HD void TALIND(int ityp) {
+    int ital = 1;
+    const double ax = fabs(kx), ay = fabs(ky), az = fabs(kz);
+    if ((ax >= ay) && (ax >= az)) {
+      if (ityp == 3) { const int t = (int)(Random() * 4.0 + 1.0); ital = t <= 2 ? t + 2 : t + 2; }
+      else ital = (kx >= 0.0) == (ityp == 1) ? 1 : 2;
+    }
+    if ((ay >= az) && (ay >= ax)) {
+      if (ityp == 3) { const int t = (int)(Random() * 4.0 + 1.0); ital = t <= 2 ? t : t + 2; }
+      else ital = (ky >= 0.0) == (ityp == 1) ? 3 : 4;
+    }
+    if ((az > ay) && (az > ax)) {
+      if (ityp == 3) ital = (int)(Random() * 4.0 + 1.0);
+      else ital = (kz >= 0.0) == (ityp == 1) ? 5 : 6;
+    }
+    isym = (int)(Random() * 8.0) + 8 * (ital - 1);
+  }
+
+  // heat_to_point (src/heat.cpp:169-196)
+  EMC_HD void heat_to_point(double e, unsigned long long* heat) {
+    if (!heat) return;
+    const int i = icell, j = jcell, k = kcell;
+    const double rx = (x - M.lx[i]) / M.dx[i];
+    const double ry = (y - M.ly[j]) / M.dy[j];
+    const double rz = (z - M.lz[k]) / M.dz[k];
+    tally_add(&heat[node(i, j, k)], fxd(e * (1 - ry) * (1 - rx) * (1 - rz), EMC_FX_HEAT_));
+    tally_add(&heat[node(i + 1, j, k)], fxd(e * (1 - ry) * rx * (1 - rz), EMC_FX_HEAT_));
+    tally_add(&heat[node(i, j, k + 1)], fxd(e * (1 - ry) * (1 - rx) * rz, EMC_FX_HEAT_));
+    tally_add(&heat[node(i + 1, j, k + 1)], fxd(e * (1 - ry) * rx * rz, EMC_FX_HEAT_));
+    tally_add(&heat[node(i, j + 1, k)], fxd(e * ry * (1 - rx) * (1 - rz), EMC_FX_HEAT_));
+    tally_add(&heat[node(i + 1, j + 1, k)], fxd(e * ry * rx * (1 - rz), EMC_FX_HEAT_));
+    tally_add(&heat[node(i, j + 1, k + 1)], fxd(e * ry * (1 - rx) * rz, EMC_FX_HEAT_));
+    tally_add(&heat[node(i + 1, j + 1, k + 1)], fxd(e * ry * rx * rz, EMC_FX_HEAT_));
+  }
+  static constexpr double EMC_FX_HEAT_ = 1099511627776.0;     // 2^40
+  static constexpr double EMC_FX_CONTACT_ = 1099511627776.0;  // 2^40
+
+  // ---- phonon scattering (src/ElectronPhononScatter.cpp:5-497, src/HolePhononScatter.cpp)
+  struct Saved {
+    double e, kx, ky, kz;
+    int iband, itet, isym;
+  };
+  EMC_HD Saved save() const { return Saved{energy, kx, ky, kz, iband, itet, isym}; }
+  EMC_HD void restore(const Saved& s) {
+    energy = s.e; kx = s.kx; ky = s.ky; kz = s.kz; iband = s.iband; itet = s.itet; isym = s.isym;
+  }
+  EMC_HD void EPSCAT(int iscat) {
+    const Saved s0 = save();
+    iband = iscat / B.scpre + B.bandofE;
+    const int p = iscat % B.scpre;
+    if (p == 0) TALIND(1);
+    else if (p <= 12) {
+      const double temps[6] = {B.temptag, B.templag, B.templog, B.temptaf, B.templaf, B.temptof};
+      const double hw = temps[(p - 1) >> 1];
+      energy = (p & 1) ? energy + hw : energy - hw;
+      TALIND(p <= 6 ? 2 : 3);
+    }
+    get_state();
+    if (Flag_SelfScatter) restore(s0);
+    if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_e);
+  }
+  EMC_HD void SCATII(int iscat, int nproc) {
+    const Saved s0 = save();
+    iband = iscat / nproc + (par_type == PELEC ? B.bandofE : B.bandofH);
+    energy = (energy - B.sieg) / 3.0;
+    isym = ((int)(48 * Random()) % 48);
+    get_state();
+    if (Flag_SelfScatter) restore(s0);
+  }
+  EMC_HD void HPSCAT(int iscat) {
+    const Saved s0 = save();
+    iband = iscat / B.scprh + B.bandofH;
+    const int p = iscat % B.scprh;
+    if (p == 1) energy = energy + B.temphop;
+    else if (p == 2) energy = energy - B.temphop;
+    isym = ((int)(48 * Random()) % 48);
+    get_state();
+    if (Flag_SelfScatter) restore(s0);
+    if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_h);
+  }
+  // rate of process iscat (CALSCATTE / CALSCATTH element, src/Band.cpp:2098-2231)
+  EMC_HD double process_rate(bool elec, int iscat, int itab, double intp, int where) const {
+    const int np = elec ? B.scpre : B.scprh;
+    const int is = iscat % np, f = iscat / np;
+    const int ini = iband - (elec ? B.bandofE : B.bandofH);
+    const int nb = elec ? 4 : 3;
+    const double* dos = elec ? B.dose : B.dosh;
+    const double* sc = elec ? B.scatte : B.scatth;
+    const double* ii = elec ? B.scattiie : B.scattiih;
+    const double* drow;
+    if (is < np - 1) {
+      const double k = sc[((size_t)is * nb + f) * nb + ini];
+      drow = &dos[((size_t)is * nb + f) * NTAB1];
+      if (where < 0) return k * drow[0];
+      if (where > 0) return k * drow[MTAB];
+      return k * (drow[itab] + intp * (drow[itab + 1] - drow[itab]));
+    }
+    drow = &ii[((size_t)f * nb + ini) * NTAB1];
+    if (where < 0) return drow[0];
+    if (where > 0) return drow[MTAB];
+    return drow[itab] + intp * (drow[itab + 1] - drow[itab]);
+  }
+  EMC_HD void select_process(double drand) {
+    const bool elec = par_type == PELEC;
+    const int np = elec ? B.scpre : B.scprh;
+    const int nmax = np * (elec ? B.nbandE : B.nbandH);
+    const double e = energy;
+    int where = 0, itab = 0;
+    double intp = 0.0;
+    if (e <= 0.0) where = -1;
+    else if (e >= B.emax) where = 1;
+    else {
+      itab = (int)(e / B.dtable);
+      intp = (e - B.energy[itab]) / (B.energy[itab + 1] - B.energy[itab]);
+    }
+    int iscat = -1;
+    double dscat = 0.0;
+    while (drand > dscat) {
+      iscat = iscat + 1;
+      if (iscat >= nmax) { Flag_SelfScatter = 1; break; }
+      dscat = dscat + process_rate(elec, iscat, itab, intp, where);
+    }
+    if (Flag_SelfScatter) return;
+    if (((iscat % np) + 1) == np) SCATII(iscat, np);
+    else if (elec) EPSCAT(iscat);
+    else HPSCAT(iscat);
+  }
+  EMC_HD void PhononScatter() {
+    const double scasum = calscattsum(energy, iband);
+    const double g = B.tr[itet].gam;
+    const double drand = scasum > g ? Random() * scasum : Random() * g;
+    if (scasum < drand) { Flag_SelfScatter = 1; return; }
+    Flag_SelfScatter = 0;
+    select_process(drand);
+  }
+
+  // ---- ionised impurities (src/ElectronImpurityScatter.cpp:6-306)
+  EMC_HD double GetImpScRate(double eel) const {
+    if ((eel * M.eV0 < 0.120) && (DA * M.conc0 >= 1e22)) {
+      const double fr = M.frickel;
+      const double eebeta = fr * 4 * PI * Rho / (2.0 * B.meld * M.eps_si);
+      if (M.flag_surface && M.flag_simp && M.insurf[cell(icell, jcell, kcell)]) return 0.0;
+      const double fr2 = (fr * Rho) * (fr * Rho);
+      return DA * B.meld * sqrt(2.0 * B.meld * eel) / (PI * fr2 * (1.0 + 4.0 * eel * B.melt / (eebeta * B.meld)));
+    }
+    return 0.0;
+  }
+  EMC_HD double GetImpScGamma() const {
+    const double e0 = B.tr[itet].vt[3], e3 = B.tr[itet].e3;
+    const double eemax = M.frickel * 4 * PI * Rho / (8.0 * B.melt * M.eps_si);
+    double gammax;
+    if (eemax < e0) gammax = GetImpScRate(e0);
+    else if (eemax > e3) gammax = GetImpScRate(e3);
+    else gammax = GetImpScRate(eemax);
+    const double lo = 1.0 / M.scrt0;
+    return gammax > lo ? gammax : lo;
+  }
+  EMC_HD void EleImpScUpdateK() {
+    double xkl, ykl, zkl;
+    in_wedge(xkl, ykl, zkl, kx, ky, kz);
+    xkl = xkl - 0.85 * B.a0pi;
+    xkl = xkl * sqrt(B.meld / B.mell);
+    ykl = ykl * sqrt(B.meld / B.melt);
+    zkl = zkl * sqrt(B.meld / B.melt);
+    const double betaq = M.frickel * 4 * PI * Rho / M.eps_si;
+    const double eebeta = betaq / (2.0 * B.meld);
+    const double alfa = eebeta * B.meld / (2.0 * energy * B.melt);
+    double pr = Random() / (1.0 + 0.5 * alfa);
+    const double costr = 1.0 - alfa * pr / (1.0 - pr);
+    const double sintr = sqrt(1.0 - costr * costr);
+    pr = 2.0 * PI * Random();
+    const double cospr = dm_cos(pr), sinpr = dm_sin(pr);
+    const double ca = sqrt(xkl * xkl + ykl * ykl);
+    const double cb = sqrt(ca * ca + zkl * zkl);
+    const double xkll = cb * sintr * cospr, ykll = cb * sintr * sinpr, zkll = cb * costr;
+    const double cosphi = xkl / ca, sinphi = ykl / ca, costheta = zkl / cb, sintheta = ca / cb;
+    const double xklll = cosphi * costheta * xkll - sinphi * ykll + cosphi * sintheta * zkll;
+    const double yklll = sinphi * costheta * xkll + cosphi * ykll + sinphi * sintheta * zkll;
+    const double zklll = -sintheta * xkll + costheta * zkll;
+    const double tn = betaq + B.mell / B.meld * (xkl - xklll) * (xkl - xklll) +
+                      B.melt / B.meld * ((ykl - yklll) * (ykl - yklll) + (zkl - zklll) * (zkl - zklll));
+    const double tm = betaq + B.melt / B.meld * (xkl - xklll) * (xkl - xklll) +
+                      B.melt / B.meld * ((ykl - yklll) * (ykl - yklll) + (zkl - zklll) * (zkl - zklll));
+    const double dn = 1.0 / (tn * tn), dnmax = 1.0 / (tm * tm);
+    if (dn > Random() * dnmax) {
+      xkl = xklll * sqrt(B.mell / B.meld);
+      ykl = yklll * sqrt(B.melt / B.meld);
+      zkl = zklll * sqrt(B.melt / B.meld);
+      xkl = xkl + 0.85 * B.a0pi;
+      out_wedge(kx, ky, kz, xkl, ykl, zkl);
+      confbz(kx, ky, kz);
+      Getisym();
+      GetEnergy();
+      Flag_SelfScatter = 0;
+    } else
+      Flag_SelfScatter = 1;
+  }
+  EMC_HD void ElectronImpurityScatter() {
+    const double gamimp = GetImpScRate(energy);
+    if (gamimp < Random() * ImpScGamma) Flag_SelfScatter = 1;
+    else EleImpScUpdateK();
+  }
+
+  // ---- boundaries (src/CellHit.cpp)
+  EMC_HD void mirror(int axis) {
+    int a = ms(3, isym), b = ms(4, isym), c = ms(5, isym);
+    if (axis == 0) { kx = -kx; a = -a; }
+    else if (axis == 1) { ky = -ky; b = -b; }
+    else { kz = -kz; c = -c; }
+    isym = indmat(a, b, c, ms(0, isym) - 1, ms(1, isym) - 1, ms(2, isym) - 1);
+  }
+  EMC_HD void Reflect() {
+    if (idir == UP || idir == DOWN) mirror(0);
+    else if (idir == RIGHT || idir == LEFT) mirror(1);
+    else if (idir == FRONT || idir == BACK) mirror(2);
+  }
+  EMC_HD void Diffuse() {
+    const Saved s0 = save();
+    TALIND(1);
+    get_state();
+    if (Flag_SelfScatter) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
+    GetV();
+    double v;
+    int axis;
+    bool flip;
+    switch (idir) {
+      case UP: v = vx; axis = 0; flip = v < 0.0; break;
+      case RIGHT: v = vy; axis = 1; flip = v > 0.0; break;
+      case DOWN: v = vx; axis = 0; flip = v > 0.0; break;
+      case LEFT: v = vy; axis = 1; flip = v < 0.0; break;
+      case FRONT: v = vz; axis = 2; flip = v < 0.0; break;
+      case BACK: v = vz; axis = 2; flip = v > 0.0; break;
+      default: return;
+    }
+    if (v == 0.0) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
+    if (flip) mirror(axis);
+  }
+  EMC_HD bool neighbour(int& ii, int& jj, int& kk) const {
+    ii = icell; jj = jcell; kk = kcell;
+    switch (idir) {
+      case UP: --ii; break;
+      case DOWN: ++ii; break;
+      case LEFT: --jj; break;
+      case RIGHT: ++jj; break;
+      case FRONT: --kk; break;
+      case BACK: ++kk; break;
+    }
+    return ii >= 0 && ii < M.cnx && jj >= 0 && jj < M.cny && kk >= 0 && kk < M.cnz;
+  }
+  EMC_HD void contact_tally(int icont, int col, long long v) {
+    if (icont < 0 || icont >= EMC_MAX_CONTACT_ || icont >= M.ncontact) { count(CNT_DROP0 + 7); return; }
+    tally_add(&T.contact[icont * 5 + col], v);
+  }
+  static constexpr int EMC_MAX_CONTACT_ = 16;
+  EMC_HD void Generate() {
+    count(CNT_GEN);
+    const int icont = M.attached[cell(icell, jcell, kcell)] - 1;
+    if (icont < 0) count(CNT_DROP0 + 7);
+    else {
+      contact_tally(icont, 0, 1);
+      contact_tally(icont, 1, fxd(charge, EMC_FX_CONTACT_));
+      contact_tally(icont, 2, fxd(energy, EMC_FX_CONTACT_));
+    }
+    if (idir == RIGHT) tally_add(&T.rcur[jcell + 1], fxd(charge, EMC_FX_CONTACT_));
+    if (idir == LEFT) tally_add(&T.lcur[jcell], fxd(charge, EMC_FX_CONTACT_));
+    int ii, jj, kk;
+    neighbour(ii, jj, kk);
+    const int ord = nclone++;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned long long slot = atomicAdd(nclones, 1ull);
+#else
+    const unsigned long long slot = (*nclones)++;
+#endif
+    if (slot >= clone_cap) { count(CNT_DROP0 + 8); return; }
+    CloneRec& c = clones[slot];
+    c.x = x; c.y = y; c.z = z; c.kx = kx; c.ky = ky; c.kz = kz;
+    c.energy = energy; c.charge = charge; c.left_time = left_time;
+    c.key = clone_key(key, step, ord);
+    c.pk = pk_make(par_type, ii, jj, kk);
+    c.itet = itet; c.isym = isym; c.pad = 0;
+  }
+  EMC_HD void Catch(bool gate) {
+    int ii, jj, kk, icont = -1;
+    if (neighbour(ii, jj, kk)) icont = M.attached[cell(ii, jj, kk)] - 1;
+    if (icont < 0) count(CNT_DROP0 + 7);
+    if (!gate) {
+      if (idir == RIGHT) tally_add(&T.rcur[jcell + 1], fxd(charge, EMC_FX_CONTACT_));
+      if (idir == LEFT) tally_add(&T.lcur[jcell], fxd(charge, EMC_FX_CONTACT_));
+    }
+    if (icont >= 0) {
+      contact_tally(icont, 3, 1);
+      contact_tally(icont, 4, fxd(charge, EMC_FX_CONTACT_));
+      contact_tally(icont, 2, fxd(energy, EMC_FX_CONTACT_));  // the reference adds to EnergyGen
+    }
+  }
+  EMC_HD void load_cell() {
+    const size_t c = cell(icell, jcell, kcell);
+    Rho = F.rho[c];
+    DA = M.da[c];
+    const double sg = M.charge_sign[par_type];
+    if (par_type == 0) { Ex = sg * F.ex[c]; Ey = sg * F.ey[c]; Ez = sg * F.ez[c]; }
+    else { Ex = sg * F.hx[c]; Ey = sg * F.hy[c]; Ez = sg * F.hz[c]; }
+  }
+  EMC_HD bool HitCell() {
+    const int imot = M.motion[cell(icell, jcell, kcell) * 6 + idir];
+    int ii, jj, kk;
+    const bool inside = neighbour(ii, jj, kk);
+    if ((imot == PASS || imot == GENERATE || imot == GENREF) && !inside) return false;
+    if (imot == PASS) {
+      switch (idir) {
+        case UP: x = M.lx[icell]; icell--; break;
+        case DOWN: x = M.lx[icell + 1]; icell++; break;
+        case RIGHT:
+          y = M.ly[jcell + 1];
+          jcell++;
+          tally_add(&T.rcur[jcell], fxd(charge, EMC_FX_CONTACT_));
+          break;
+        case LEFT:
+          y = M.ly[jcell];
+          tally_add(&T.lcur[jcell], fxd(charge, EMC_FX_CONTACT_));
+          jcell--;
+          break;
+        case FRONT: z = M.lz[kcell]; kcell--; break;
+        case BACK: z = M.lz[kcell + 1]; kcell++; break;
+      }
+    } else if (imot == REFLECT) {
+      Reflect();
+    } else if (imot == SCATTOX) {
+      if (M.flag_surface) Reflect();
+      else if (Random() > B.difpr[par_type]) Reflect();
+      else Diffuse();
+    } else if (imot == PERIOD) {
+      if (idir == UP) x = M.lx[icell + 1];
+      else if (idir == DOWN) x = M.lx[icell];
+      else if (idir == LEFT) y = M.ly[jcell + 1];
+      else if (idir == RIGHT) y = M.ly[jcell];
+      else if (idir == FRONT) z = M.lz[kcell + 1];
+      else if (idir == BACK) z = M.lz[kcell];
+    } else if (imot == GENERATE || imot == GENREF) {
+      Generate();
+      Reflect();
+    } else if (imot == CATCH || imot == CATCHGATE) {
+      Catch(imot == CATCHGATE);
+      Flag_Catch = 1;
+      drop_reason = -1;
+    } else {
+      Flag_Catch = 1;
+      drop_reason = 5;
+      return true;
+    }
+    load_cell();
+    return true;
+  }
+
+  // TetTime (src/TetHit.cpp:4-106)
+  EMC_HD double TetTime() {
+    double xkl, ykl, zkl, xfl, yfl, zfl;
+    in_wedge(xkl, ykl, zkl, kx, ky, kz);
+    in_wedge(xfl, yfl, zfl, Ex, Ey, Ez);
+    const TetRec& r = B.tr[itet];
+    double best = 0.0;
+    for (int f = 0; f < 4; ++f) {
+      const double nf = r.n[f][0] * xfl + r.n[f][1] * yfl + r.n[f][2] * zfl;
+      double tfp;
+      if (nf < 0) {
+        tfp = (r.n[f][3] - (r.n[f][0] * xkl + r.n[f][1] * ykl + r.n[f][2] * zkl)) / nf;
+        if (tfp < 0) tfp = 0;
+      } else
+        tfp = 1e99;
+      if (f == 0 || tfp < best) { best = tfp; itetdir = f; }
+    }
+    return best;
+  }
+  // HitTet (src/TetHit.cpp:109-142)
+  EMC_HD void HitTet() {
+    const int nb = B.tr[itet].neib[itetdir];
+    if (nb >= 0) itet = nb;
+    else if (nb == -5 || nb == -6) {
+      const int ii = -nb - 2;
+      kx = kx + B.xbz[ii * 48 + isym];
+      ky = ky + B.ybz[ii * 48 + isym];
+      kz = kz + B.zbz[ii * 48 + isym];
+      isym = B.newisym[ii * 48 + isym];
+      GetisymBZ();
+      if (Flag_Catch) return;
+      GetEnergy();
+    } else
+      isym = B.newisym[(-nb - 2) * 48 + isym];
+  }
+  // CellTime (src/CellHit.cpp:4-93)
+  EMC_HD double CellTime() {
+    int ix, iy, iz;
+    double tx, ty, tz, t;
+    if (vx > 0) { ix = DOWN; tx = (M.lx[icell + 1] - x) / vx; }
+    else if (vx < 0) { ix = UP; tx = -(x - M.lx[icell]) / vx; }
+    else { ix = UP; tx = 1e99; }
+    if (vy > 0) { iy = RIGHT; ty = (M.ly[jcell + 1] - y) / vy; }
+    else if (vy < 0) { iy = LEFT; ty = -(y - M.ly[jcell]) / vy; }
+    else { iy = RIGHT; ty = 1e99; }
+    if (vz > 0) { iz = BACK; tz = (M.lz[kcell + 1] - z) / vz; }
+    else if (vz < 0) { iz = FRONT; tz = -(z - M.lz[kcell]) / vz; }
+    else { iz = FRONT; tz = 1e99; }
+    idir = ix; t = tx;
+    if (ty < t) { idir = iy; t = ty; }
+    if (tz < t) { idir = iz; t = tz; }
+    if (t < -MY_ZERO) t = 0;
+    return t;
+  }
+
+  // particle_fly (src/mcmodel.cpp:1874-2161)
+  EMC_HD void fly() {
+    bool getTet = true, getCell = true, getPh = true, getImp = true, getSurf = true;
+    double ssnl = 0, imprnl = 0, phrnl = 0;
+    double TetTf = 0, CellTf = 0;
+    Flag_Catch = 0; drop_reason = -1;
+    SurfScGamma = 0; SurfRoughnessScRate = 0; SurfPhononScRate = 0; ImpScGamma = 0;
+    Flag_SelfScatter = 0; idir = 0; itetdir = 0;
+    load_cell();
+    iband = B.tr[itet].ibt;
+    int loop = 0;
+    while (left_time > 0) {
+      GetV();
+      loop++;
+      n_events++;
+      if (loop > FLY_LOOP_CAP) { Flag_Catch = 1; drop_reason = 3; break; }
+      if (getTet) { TetTf = TetTime(); getTet = false; }
+      if (getCell) { CellTf = CellTime(); getCell = false; }
+      if (getPh) { phrnl = -dm_log(Random()); getPh = false; }
+      const double gam = B.tr[itet].gam;
+      double PhScTf = phrnl / gam;
+      double ImpScTf;
+      if (iband == B.bandofE) {
+        if (getImp) { imprnl = -dm_log(Random()); getImp = false; }
+        ImpScGamma = GetImpScGamma();
+        ImpScTf = imprnl / ImpScGamma;
+      } else {
+        ImpScTf = 2 * M.dt;
+        ImpScGamma = 1.0 / M.scrt0;
+      }
+      const size_t c = cell(icell, jcell, kcell);
+      double SurfScTf;
+      if (M.flag_surface && M.insurf[c]) {
+        if (getSurf) { ssnl = -dm_log(Random()); getSurf = false; }
+        if (par_type == PELEC) { SurfRoughnessScRate = F.e_rough[c]; SurfPhononScRate = F.e_sph[c]; }
+        else { SurfRoughnessScRate = F.h_rough[c]; SurfPhononScRate = F.h_sph[c]; }
+        SurfScGamma = SurfRoughnessScRate + SurfPhononScRate;
+        SurfScTf = (SurfScGamma > 0 && ssnl > 0) ? ssnl / SurfScGamma : 2 * M.dt;
+      } else
+        SurfScTf = 2 * M.dt;
+      // Min(6 args, &flag): the first argument wins ties (src/Constant_GlobalVariable_MathFunction.cpp:92-118)
+      int flag = 1;
+      double Tf = TetTf;
+      if (Tf > CellTf) { Tf = CellTf; flag = 2; }
+      if (Tf > PhScTf) { Tf = PhScTf; flag = 3; }
+      if (Tf > ImpScTf) { Tf = ImpScTf; flag = 4; }
+      if (Tf > SurfScTf) { Tf = SurfScTf; flag = 5; }
+      if (Tf > left_time) { Tf = left_time; flag = 6; }
+      if (Tf < MY_ZERO) { Flag_Catch = 1; drop_reason = 0; break; }
+      TetTf -= Tf; CellTf -= Tf;
+      left_time -= Tf;
+      phrnl -= gam * Tf;
+      imprnl -= ImpScGamma * Tf;
+      ssnl -= SurfScGamma * Tf;
+      kx += Ex * Tf; ky += Ey * Tf; kz += Ez * Tf;
+      energy += (vx * Ex + vy * Ey + vz * Ez) * Tf;
+      x += vx * Tf; y += vy * Tf; z += vz * Tf;
+      if (energy < 0) { Flag_Catch = 1; drop_reason = 1; break; }
+      switch (flag) {
+        case 1:
+          HitTet();
+          getTet = getCell = true;
+          break;
+        case 2:
+          if (HitCell()) getTet = getCell = true;
+          else { Flag_Catch = 1; drop_reason = 4; }
+          break;
+        case 3:
+          PhononScatter();
+          if (!Flag_SelfScatter) { getTet = getCell = true; n_phonon++; }
+          getPh = true;
+          break;
+        case 4:
+          if (par_type == PELEC) ElectronImpurityScatter();
+          if (!Flag_SelfScatter) { getTet = getCell = true; n_impurity++; }
+          getImp = true;
+          break;
+        case 5:
+          // ParticleSurfaceScatter (src/ParSurfaceScatter.cpp:48-61)
+          if (Random() * SurfScGamma < SurfRoughnessScRate) {
+            idir = M.surface_dir[M.nearest_surf[c]];
+            Diffuse();
+          } else {
+            Flag_SelfScatter = 0;
+            select_process(Random() * calscattsum(energy, iband));
+          }
+          if (!Flag_SelfScatter) getTet = getCell = true;
+          getSurf = true;
+          break;
+        default:
+          break;
+      }
+      if (Flag_Catch) break;
+    }
+  }
+};
+
+}  // namespace emc

@@ -1,0 +1,101 @@
+// Table and mesh layouts consumed by the flight, shared by the HIP kernels and
+// the host-side sampler.  Pointers refer to device memory in kernels and to
+// host memory on the host; scalars are normalised as in the reference.
+#pragma once
+#include <stdint.h>
+
+#include "devmath.h"
+
+namespace emc {
+
+// One tetrahedron's data read on every flight event, contiguous (208 B):
+// the four inward face planes (nx, ny, nz, d) of Band::HESSTET, the group
+// velocity and base energy of Band::VELOTET, the variable-Gamma bound gamtet,
+// the top-corner energy and the neighbour codes NeibrTet.
+struct alignas(16) TetRec {
+  double n[4][4];
+  double vt[4];
+  double gam;
+  double e3;
+  int neib[4];
+  int ibt;
+  int pad[3];
+};
+static_assert(sizeof(TetRec) == 208, "TetRec layout");
+
+struct BandDev {
+  const TetRec* tr;
+  const int* tet;  // [4][nt]
+  const double *xkk, *ykk, *zkk, *eek;
+  const double *vgt, *faclow, *facmed, *fachigh;
+  const int *matsym, *indmat, *newisym;
+  const double *xbz, *ybz, *zbz;
+  const int *ntlist, *ptlist, *tlist, *nslist, *pslist, *slist, *ntlists, *ptlists, *tlists;
+  const int* ibt;
+  const double *energy, *dostetmax, *sumscatt, *scatte, *dose, *scattiie, *scatth, *dosh, *scattiih;
+  int nt, scpre, scprh, nbandE, nbandH, bandofE, bandofH, pad0;
+  double dtable, emax, dlist, dlists, a0pi, sieg, mell, melt, meld;
+  double difpr[3];
+  double temptag, templag, templog, temptaf, templaf, temptof, temphop;
+};
+
+struct MeshDev {
+  const double *lx, *ly, *lz, *dx, *dy, *dz;
+  const uint8_t* motion;    // [ncell][6]
+  const int* attached;      // [ncell]
+  const uint8_t* insurf;    // [ncell]
+  const int* nearest_surf;  // [ncell]
+  const double* da;         // [ncell]
+  const int* surface_dir;   // [nsurf]
+  int cnx, cny, cnz, pnx, pny, pnz, ncontact, flag_surface, flag_simp, pad0;
+  double dt, eV0, conc0, scrt0, eps_si, frickel;
+  double charge_sign[3];
+};
+
+struct FieldsDev {
+  const double *ex, *ey, *ez, *hx, *hy, *hz, *rho, *e_rough, *e_sph, *h_rough, *h_sph;
+};
+
+// Counter slots of the flight tallies (int64).
+enum {
+  CNT_DROP0 = 0,  // 9 drop / error reasons
+  CNT_CATCH = 9,
+  CNT_GEN = 10,
+  CNT_PHONON = 11,
+  CNT_IMPURITY = 12,
+  CNT_EVENTS = 13,
+  CNT_N = 16
+};
+
+struct TallyDev {
+  unsigned long long* contact;   // [16][5]
+  unsigned long long* counters;  // [CNT_N]
+  unsigned long long *heat_e, *heat_h;  // [nnode] or null
+  unsigned long long *rcur, *lcur;      // [pny]
+};
+
+// Device particle arrays (SoA).  pk packs type | i | j | k; PK_DEAD marks a hole.
+struct PartDev {
+  double *x, *y, *z, *kx, *ky, *kz, *energy, *charge;
+  uint32_t* pk;
+  int32_t *itet, *isym;
+  uint64_t* key;
+};
+
+// A GENERATE clone waiting to fly the rest of its step.
+struct CloneRec {
+  double x, y, z, kx, ky, kz, energy, charge, left_time;
+  uint64_t key;
+  uint32_t pk;
+  int32_t itet, isym, pad;
+};
+
+EMC_HD inline uint32_t pk_make(int type, int i, int j, int k) {
+  return ((uint32_t)type << 31) | ((uint32_t)i << 21) | ((uint32_t)j << 10) | (uint32_t)k;
+}
+EMC_HD inline int pk_type(uint32_t p) { return (int)(p >> 31); }
+EMC_HD inline int pk_i(uint32_t p) { return (int)((p >> 21) & 0x3FF); }
+EMC_HD inline int pk_j(uint32_t p) { return (int)((p >> 10) & 0x7FF); }
+EMC_HD inline int pk_k(uint32_t p) { return (int)(p & 0x3FF); }
+
+}  // namespace emc

@@ -1,0 +1,302 @@
+"""Python host wrapper over libemc.so (the C ABI in include/emc.h).
+
+The engine runs the flight, deposit, field and Poisson kernels on the GPU.
+There is no CPU fallback: without the built library or a GPU the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (FX_CONTACT, FX_HEAT, FX_Q, FX_STAT, CELL_FIELDS, PARTICLE_F64, PARTICLE_I32)
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libemc.so")
+TABLE_DIR = os.path.join(PKG_DIR, "input")
+DECK_DIR = os.path.join(PKG_DIR, "decks")
+
+_lib = None
+
+
+class EmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"emc error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise EmcError(abi.EMC_ERR_NOGPU, f"{path} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        abi.declare(lib)
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _view(ptr, n, dtype):
+    if not ptr or n <= 0:
+        return np.zeros(0, dtype=dtype)
+    ct = np.ctypeslib.as_ctypes_type(dtype)
+    buf = C.cast(ptr, C.POINTER(ct * n)).contents
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
+class ParticleArrays:
+    """Host SoA buffers matching emc_particles."""
+
+    def __init__(self, n):
+        self.n = n
+        for f in PARTICLE_F64:
+            setattr(self, f, np.zeros(n, np.float64))
+        for f in PARTICLE_I32:
+            setattr(self, f, np.zeros(n, np.int32))
+        self.key = np.zeros(n, np.uint64)
+
+    def struct(self):
+        s = abi.Particles()
+        for f in PARTICLE_F64:
+            setattr(s, f, _ptr(getattr(self, f), C.c_double))
+        for f in PARTICLE_I32:
+            setattr(s, f, _ptr(getattr(self, f), C.c_int32))
+        s.key = _ptr(self.key, C.c_uint64)
+        return s
+
+    def subset(self, idx):
+        out = ParticleArrays(len(idx))
+        for f in PARTICLE_F64 + PARTICLE_I32 + ("key",):
+            getattr(out, f)[:] = getattr(self, f)[idx]
+        return out
+
+    def copy(self, cap=None):
+        cap = self.n if cap is None else cap
+        out = ParticleArrays(cap)
+        for f in PARTICLE_F64 + PARTICLE_I32 + ("key",):
+            getattr(out, f)[:self.n] = getattr(self, f)[:self.n]
+        out.n = self.n
+        return out
+
+    def live(self):
+        return self.subset(np.nonzero(self.i[:self.n] >= 0)[0])
+
+
+class CellFieldArrays:
+    def __init__(self, ncell):
+        for f in CELL_FIELDS:
+            setattr(self, f, np.zeros(ncell, np.float64))
+
+    def struct(self):
+        s = abi.CellFields()
+        for f in CELL_FIELDS:
+            setattr(s, f, _ptr(getattr(self, f), C.c_double))
+        return s
+
+
+class Engine:
+    """One device description (decks) plus its GPU state."""
+
+    def __init__(self, input_txt, device=0, seed=1, table_dir=TABLE_DIR, capacity_factor=1.25,
+                 poisson_rtol=1e-10, poisson_maxit=20000):
+        self.lib = load_library()
+        o = abi.Opts()
+        self.lib.emc_default_opts(C.byref(o))
+        self._table_dir = table_dir.encode()
+        o.table_dir = self._table_dir
+        o.device = device
+        o.seed = seed
+        o.capacity_factor = capacity_factor
+        o.poisson_rtol = poisson_rtol
+        o.poisson_maxit = poisson_maxit
+        st = C.c_int(0)
+        self.ctx = self.lib.emc_create(os.fsencode(input_txt), C.byref(o), C.byref(st))
+        if st.value != 0:
+            msg = self.lib.emc_last_error(self.ctx).decode()
+            self.lib.emc_destroy(self.ctx)
+            self.ctx = None
+            raise EmcError(st.value, msg)
+        self.band = abi.BandView()
+        self._check(self.lib.emc_band_view_get(self.ctx, C.byref(self.band)))
+        self.mesh = abi.MeshView()
+        self._check(self.lib.emc_mesh_view_get(self.ctx, C.byref(self.mesh)))
+
+    def close(self):
+        if self.ctx:
+            self.lib.emc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise EmcError(st, self.lib.emc_last_error(self.ctx).decode())
+
+    # ---- views
+    def band_arrays(self):
+        b = self.band
+        nt, nk = b.nt, b.nk
+        v = {}
+        for f, n in (("xkk", nk), ("ykk", nk), ("zkk", nk), ("eek", nk), ("vgt", nt), ("gamtet", nt),
+                     ("faclow", nt), ("facmedium", nt), ("fachigh", nt), ("maxaovtet", nt),
+                     ("xkct", nt), ("ykct", nt), ("zkct", nt), ("vt", 4 * nt), ("dnt", 16 * nt),
+                     ("energy", 7501), ("dos", 8 * 7501), ("sumdos", 7501 * 3), ("dostetmax", 8 * 7501),
+                     ("sumscatt", 7501 * 8), ("scatte", 14 * 16), ("dose", 14 * 4 * 7501),
+                     ("scattiie", 16 * 7501), ("scatth", 4 * 9), ("dosh", 4 * 3 * 7501), ("scattiih", 9 * 7501),
+                     ("xbz", 240), ("ybz", 240), ("zbz", 240)):
+            v[f] = _view(getattr(b, f), n, np.float64)
+        for f, n in (("tet", 4 * nt), ("neib", 4 * nt), ("ibt", nt), ("matsym", 288), ("indmat", 729),
+                     ("newisym", 240), ("ntlist", 325 * 8), ("ptlist", 325 * 8), ("tlist", b.tlist_len),
+                     ("nslist", 325 * 8), ("pslist", 325 * 8), ("slist", b.slist_len),
+                     ("ntlists", 60 * 8), ("ptlists", 60 * 8), ("tlists", b.tlists_len)):
+            v[f] = _view(getattr(b, f), n, np.int32)
+        return v
+
+    def mesh_arrays(self):
+        m = self.mesh
+        nc, nn = m.ncell, m.nnode
+        v = {}
+        for f, n in (("lx", m.pnx), ("ly", m.pny), ("lz", m.pnz), ("dx", m.cnx), ("dy", m.cny), ("dz", m.cnz),
+                     ("cvol", nc), ("da", nc), ("pvol", nn), ("pdop", nn), ("vadd", nn), ("dirichlet", nn),
+                     ("diag", nn), ("cx", nn), ("cy", nn), ("cz", nn), ("rhs_corr", nn), ("stencil", 7 * nn)):
+            v[f] = _view(getattr(m, f), n, np.float64)
+        for f, n in (("material", nc), ("attached", nc), ("nearest_surf", nc), ("eeff_dir", nc),
+                     ("surface_dir", max(1, m.nsurf))):
+            v[f] = _view(getattr(m, f), n, np.int32)
+        for f, n in (("motion", 6 * nc), ("insurf", nc), ("node_kind", nn)):
+            v[f] = _view(getattr(m, f), n, np.uint8)
+        return v
+
+    # ---- particles
+    def particles_init(self):
+        n = C.c_int64(0)
+        self._check(self.lib.emc_particles_init(self.ctx, C.byref(n)))
+        return n.value
+
+    def count(self):
+        n = C.c_int64(0)
+        self._check(self.lib.emc_particles_count(self.ctx, C.byref(n)))
+        return n.value
+
+    def upload(self, p, n=None):
+        n = p.n if n is None else n
+        s = p.struct()
+        self._check(self.lib.emc_particles_upload(self.ctx, C.byref(s), n))
+
+    def download(self):
+        n = self.count()
+        p = ParticleArrays(n)
+        s = p.struct()
+        got = C.c_int64(0)
+        self._check(self.lib.emc_particles_download(self.ctx, C.byref(s), n, C.byref(got)))
+        p.n = got.value
+        return p
+
+    # ---- fields
+    def fields_set(self, f):
+        s = f.struct()
+        self._check(self.lib.emc_fields_set(self.ctx, C.byref(s)))
+
+    def fields_get(self):
+        f = CellFieldArrays(self.mesh.ncell)
+        s = f.struct()
+        self._check(self.lib.emc_fields_get(self.ctx, C.byref(s)))
+        return f
+
+    # ---- kernels
+    def fly_step(self, step, flag_heat=0):
+        t = abi.FlyTallies()
+        self._check(self.lib.emc_fly_step(self.ctx, step, flag_heat, C.byref(t)))
+        return t
+
+    def deposit(self):
+        nn, nc = self.mesh.nnode, self.mesh.ncell
+        node_q = np.zeros(nn, np.int64)
+        node_stat = np.zeros(10 * nn, np.int64)
+        cell_q = np.zeros(3 * nc, np.int64)
+        cell_n = np.zeros(nc, np.int64)
+        self._check(self.lib.emc_deposit(self.ctx, _ptr(node_q, C.c_int64), _ptr(node_stat, C.c_int64),
+                                         _ptr(cell_q, C.c_int64), _ptr(cell_n, C.c_int64)))
+        return node_q, node_stat, cell_q, cell_n
+
+    def rhs_from_charge(self, node_q):
+        node_q = np.ascontiguousarray(node_q, np.int64)
+        rhs = np.zeros(self.mesh.nnode)
+        self._check(self.lib.emc_rhs_from_charge(self.ctx, _ptr(node_q, C.c_int64), _ptr(rhs, C.c_double)))
+        return rhs
+
+    def poisson_solve(self, rhs, x0=None, rtol=1e-10, maxit=20000):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        sol = np.zeros(self.mesh.nnode) if x0 is None else np.array(x0, np.float64)
+        it = C.c_int(0)
+        rr = C.c_double(0)
+        self._check(self.lib.emc_poisson_solve(self.ctx, _ptr(rhs, C.c_double), _ptr(sol, C.c_double), rtol, maxit,
+                                               C.byref(it), C.byref(rr)))
+        return sol, it.value, rr.value
+
+    def compute_field(self, pot, rho):
+        pot = np.ascontiguousarray(pot, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        self._check(self.lib.emc_compute_field(self.ctx, _ptr(pot, C.c_double), _ptr(rho, C.c_double)))
+        return self.fields_get()
+
+    def step(self, nsteps=1):
+        t = abi.FlyTallies()
+        self._check(self.lib.emc_step(self.ctx, nsteps, C.byref(t)))
+        return t
+
+    def potential(self):
+        p = np.zeros(self.mesh.nnode)
+        self._check(self.lib.emc_potential_get(self.ctx, _ptr(p, C.c_double)))
+        return p
+
+    def timing(self):
+        ms = np.zeros(5)
+        it = C.c_int(0)
+        self._check(self.lib.emc_step_timing(self.ctx, _ptr(ms, C.c_double), C.byref(it)))
+        return ms, it.value
+
+    def heat(self):
+        nn = self.mesh.nnode
+        he = np.zeros(nn, np.int64)
+        hh = np.zeros(nn, np.int64)
+        self._check(self.lib.emc_heat_get(self.ctx, _ptr(he, C.c_int64), _ptr(hh, C.c_int64)))
+        return he, hh
+
+    def currents(self):
+        pny = self.mesh.pny
+        r = np.zeros(pny, np.int64)
+        lc = np.zeros(pny, np.int64)
+        self._check(self.lib.emc_currents_get(self.ctx, _ptr(r, C.c_int64), _ptr(lc, C.c_int64)))
+        return r, lc
+
+    def tallies_reset(self):
+        self._check(self.lib.emc_tallies_reset(self.ctx))
+
+
+def tallies_dict(t):
+    return {
+        "contact": np.array([[t.contact[i][j] for j in range(abi.CT_NCOL)] for i in range(abi.MAX_CONTACT)],
+                            dtype=np.int64),
+        "drops": np.array(list(t.drops), dtype=np.int64),
+        "n_catch": t.n_catch, "n_gen": t.n_gen, "n_phonon": t.n_phonon,
+        "n_impurity": t.n_impurity, "n_events": t.n_events,
+    }
+
+
+def write_deck(out_dir, **kw):
+    """Write FinFET decks (see decks/make_finfet.py)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_finfet", os.path.join(DECK_DIR, "make_finfet.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.write_decks(out_dir, **kw)
+    return os.path.join(out_dir, "input.txt")

@@ -1,0 +1,247 @@
+/*
+ * emc.h - C ABI of the MI355X full-band ensemble Monte Carlo engine.
+ *
+ * One GPU, one context per device description.  Plain C: opaque handle,
+ * plain pointers and sizes, int status codes.  Host views returned by the
+ * *_view calls stay valid until emc_destroy.
+ *
+ * Reference seams this ABI replaces (chenglinye18-crypto/C-MC_electron):
+ *   emc_create          MeshQuantities::MeshQuantities + initialize()   src/mcmodel.cpp:26-146
+ *                       (getInputData :3824, read_grid_file :4595, read_device_file :4190,
+ *                        Band::IELEC src/Band.cpp:4271, init_cell_data :3187,
+ *                        init_point_data :3272, init_poisson_matrix :2898)
+ *   emc_particles_init  init_particle_data / select_kstate        src/particle_init.cpp:112-219,
+ *                                                                  src/mcmodel.cpp:3006-3057
+ *   emc_deposit         statistic + particle_to_density + compute_cell_charge
+ *                                                                  src/mcmodel.cpp:1167-1408, 348-417, 2664-2712
+ *   emc_poisson_solve   PoissonSolver::solve_poisson              include/PoissonSolver.h:6-18,
+ *                                                                  src/PoissonSolver.cpp:70-78
+ *   emc_compute_field   compute_rhs + pot2field(_for_hole) + GetSurfRoughnessPhononScRate
+ *                                                                  src/mcmodel.cpp:2858-2893, 2732-2855,
+ *                                                                  src/GetSurfRoughnessPhononScRate.cpp:4-71
+ *   emc_fly_step        MeshQuantities::update_particle            src/mcmodel.cpp:1460-1505
+ *                       (particle_fly :1874-2161 and callees; local_migrate :1615-1638)
+ *   emc_step            density() + update_particle per step       src/mcmodel.cpp:424-471, 499-566
+ *   emc_run_file        main(input.txt)                            src/main.cpp:3-61
+ */
+#ifndef EMC_H
+#define EMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (extend the reference's error enum, include/utils.h:120-128) */
+#define EMC_OK 0
+#define EMC_ERR_ARG (-1)      /* bad argument or size                       */
+#define EMC_ERR_IO (-2)       /* deck or table file missing / malformed      */
+#define EMC_ERR_DECK (-3)     /* deck semantics (alignment, ranges)          */
+#define EMC_ERR_HIP (-4)      /* HIP runtime error                            */
+#define EMC_ERR_CAPACITY (-5) /* particle or clone capacity exceeded          */
+#define EMC_ERR_NOGPU (-6)    /* no HIP device / kernels not loadable         */
+#define EMC_ERR_STATE (-7)    /* call out of order                             */
+#define EMC_ERR_SOLVER (-8)   /* Poisson solver did not converge              */
+
+/* ---- fixed-point scales of the int64 tallies (value = round(x * scale)) */
+#define EMC_FX_Q 281474976710656.0      /* 2^48: per-step node/cell charge sums      */
+#define EMC_FX_STAT 4294967296.0        /* 2^32: window statistics                   */
+#define EMC_FX_HEAT 1099511627776.0     /* 2^40: heat per node                       */
+#define EMC_FX_CONTACT 1099511627776.0  /* 2^40: contact charge / energy, currents   */
+
+/* ---- drop / error reasons counted by the flight (reference prints, then drops or exits) */
+#define EMC_DROP_TF_ZERO 0     /* free-flight time below MY_ZERO   src/mcmodel.cpp:2002-2006 */
+#define EMC_DROP_NEG_ENERGY 1  /* energy < 0 after drift            src/mcmodel.cpp:2048-2056 */
+#define EMC_DROP_NO_TET 2      /* Getisym found no tetrahedron      src/ParticleUtils.cpp:617-624 */
+#define EMC_DROP_LOOPS 3       /* event-loop cap (65536) reached                              */
+#define EMC_DROP_OFF_MESH 4    /* PASS/GENERATE across the mesh edge (slab edge on one rank)  */
+#define EMC_DROP_BAD_RULE 5    /* motion rule not defined            src/CellHit.cpp:185-190  */
+#define EMC_ERR_AREA 6         /* FASTSURF area < 0                  src/Band.cpp:692-700     */
+#define EMC_ERR_CONTACT 7      /* attached contact index < 1         src/CellHit.cpp:444-447  */
+#define EMC_ERR_CLONE_FULL 8   /* clone buffer full: clone lost                               */
+#define EMC_NREASON 9
+
+/* ---- contact tally columns */
+#define EMC_CT_NGEN 0
+#define EMC_CT_QGEN 1
+#define EMC_CT_EGEN 2
+#define EMC_CT_NCATCH 3
+#define EMC_CT_QCATCH 4
+#define EMC_CT_NCOL 5
+#define EMC_MAX_CONTACT 16
+
+/* Host view of the full-band tables (layouts documented per field; all
+ * energies, wave vectors and rates normalised as in the reference). */
+typedef struct emc_band_view {
+  int nk, nt, nbt;
+  int nband[3], bandof[3];
+  int scpre, scprh;
+  double dtable, emax, dlist, dlists, a0pi, sieg;
+  double mell, melt, meld;
+  double difpr[3];
+  double temptag, templag, templog, temptaf, templaf, temptof, temphop;
+  double Ni;
+  const double *xkk, *ykk, *zkk, *eek;  /* [nk]                                  */
+  const int *tet;                       /* [4][nt] corner k-point, energy ascending */
+  const int *neib;                      /* [4][nt] NeibrTet                        */
+  const int *ibt;                       /* [nt]                                    */
+  const double *dnt;                    /* [4 comp][4 face][nt] face normals, offset */
+  const double *vt;                     /* [nt][4] vx vy vz e0                      */
+  const double *vgt, *gamtet, *faclow, *facmedium, *fachigh, *maxaovtet; /* [nt] */
+  const double *xkct, *ykct, *zkct;     /* [nt]                                    */
+  const int *matsym;                    /* [6][48]                                 */
+  const int *indmat;                    /* [729]                                   */
+  const int *newisym;                   /* [5][48]                                 */
+  const double *xbz, *ybz, *zbz;        /* [5][48]                                 */
+  const int *ntlist, *ptlist, *tlist;   /* [325][8], [325][8], [len]               */
+  const int *nslist, *pslist, *slist;   /* [325][8], [325][8], [len]               */
+  const int *ntlists, *ptlists, *tlists;/* [60][8],  [60][8],  [len]               */
+  int tlist_len, slist_len, tlists_len;
+  const double *energy;                 /* [7501]                                  */
+  const double *dos;                    /* [8][7501]                               */
+  const double *sumdos;                 /* [7501][3]                               */
+  const double *dostetmax;              /* [8][7501]                               */
+  const double *sumscatt;               /* [7501][8]                               */
+  const double *scatte;                 /* [14][4][4] process, final, initial      */
+  const double *dose;                   /* [14][4][7501]                           */
+  const double *scattiie;               /* [4][4][7501] final, initial, itab       */
+  const double *scatth;                 /* [4][3][3]                               */
+  const double *dosh;                   /* [4][3][7501]                            */
+  const double *scattiih;               /* [3][3][7501]                            */
+  double DOSMAX[3];
+} emc_band_view;
+
+/* Host view of the mesh.  cell c = (j*cnx + i)*cnz + k ; node n = (j*pnx + i)*pnz + k. */
+typedef struct emc_mesh_view {
+  int cnx, cny, cnz, pnx, pny, pnz;
+  int64_t ncell, nnode;
+  int ncontact, nsurf;
+  double dt;                        /* normalised time step                  */
+  double spr0, time0, eV0, pot0, conc0, scrt0, field0, curr0, velo0; /* units */
+  double eps_si, eps_ox, frickel, Tn, T0, em0, hq0, ec0, sirho;
+  double surf_delta, surf_ail, surf_XIph, surf_theta, surf_gama, surf_Nbmod, surf_Npf;
+  double surf_Pft, surf_Pfn, surf_Kpha;
+  double mass_eff[3], charge_sign[3];
+  int flag_surface, flag_rough, flag_sphonon, flag_simp, flag_heat;
+  const double *lx, *ly, *lz, *dx, *dy, *dz;
+  const int *material, *attached, *nearest_surf, *eeff_dir; /* [ncell]           */
+  const uint8_t *motion;            /* [ncell][6]                             */
+  const uint8_t *insurf;            /* [ncell]                                */
+  const double *cvol, *da;          /* [ncell]                                */
+  const int *surface_dir;           /* [nsurf]                                */
+  const double *pvol, *pdop, *vadd; /* [nnode]                                */
+  const uint8_t *node_kind;         /* [nnode] 0 unknown, 1 contact, 2 identity */
+  const double *dirichlet;          /* [nnode]                                */
+  const double *diag, *cx, *cy, *cz;/* [nnode] eliminated SPD 7-point operator */
+  const double *rhs_corr;           /* [nnode]                                */
+  const double *stencil;            /* [nnode][7] operator as assembled        */
+} emc_mesh_view;
+
+/* Structure-of-arrays particle buffers (caller-owned on the host). */
+typedef struct emc_particles {
+  double *x, *y, *z, *kx, *ky, *kz, *energy, *charge;
+  int32_t *i, *j, *k, *itet, *isym, *ptype;   /* ptype 0 electron, 1 hole; i<0 = empty */
+  uint64_t *key;                              /* counter-based RNG key / identity      */
+} emc_particles;
+
+/* Per-cell inputs of the flight: fields, cell charge density Rho and the
+ * surface rates, all normalised. */
+typedef struct emc_cell_fields {
+  double *ex, *ey, *ez;       /* electron field  (c_field_*)     */
+  double *hx, *hy, *hz;       /* hole field      (c_h_field_*)   */
+  double *rho;                /* c_par_charge after the 0.5*DA clamp */
+  double *e_rough, *e_sph, *h_rough, *h_sph; /* surface roughness / phonon rates */
+} emc_cell_fields;
+
+/* Flight tallies (exact int64, order-independent). */
+typedef struct emc_fly_tallies {
+  int64_t contact[EMC_MAX_CONTACT][EMC_CT_NCOL];
+  int64_t drops[EMC_NREASON];
+  int64_t n_catch;           /* catch_par_num: caught + dropped            */
+  int64_t n_gen;             /* gen_par                                      */
+  int64_t n_phonon, n_impurity; /* real scattering events (sttt counters)    */
+  int64_t n_events;          /* event-loop iterations                        */
+  int64_t n_clone_passes;
+} emc_fly_tallies;
+
+/* Options for emc_create. */
+typedef struct emc_opts {
+  const char *table_dir;     /* directory with bs.si.asc and zd.si.asc          */
+  int device;                /* HIP device ordinal; -1 = host only (no GPU)     */
+  uint64_t seed;             /* run seed for the particle RNG keys              */
+  double capacity_factor;    /* particle capacity = factor * initial count      */
+  double poisson_rtol;       /* relative residual target of the PCG            */
+  int poisson_maxit;
+} emc_opts;
+
+typedef struct emc_ctx emc_ctx;
+
+void emc_default_opts(emc_opts *o);
+emc_ctx *emc_create(const char *input_txt, const emc_opts *o, int *status);
+void emc_destroy(emc_ctx *ctx);
+const char *emc_last_error(emc_ctx *ctx);
+
+int emc_band_view_get(emc_ctx *ctx, emc_band_view *out);
+int emc_mesh_view_get(emc_ctx *ctx, emc_mesh_view *out);
+
+/* Initial population (host sampling as init_particle_data), uploaded to the GPU
+ * when one is attached.  n_out receives the particle count. */
+int emc_particles_init(emc_ctx *ctx, int64_t *n_out);
+int emc_particles_count(emc_ctx *ctx, int64_t *n_out);
+int emc_particles_upload(emc_ctx *ctx, const emc_particles *host, int64_t n);
+int emc_particles_download(emc_ctx *ctx, emc_particles *host, int64_t cap, int64_t *n_out);
+
+/* Cell fields used by the next emc_fly_step (replaces the field solve). */
+int emc_fields_set(emc_ctx *ctx, const emc_cell_fields *host);
+int emc_fields_get(emc_ctx *ctx, emc_cell_fields *host);
+
+/* One flight step (update_particle): every particle flies dt from the state on
+ * the GPU; GENERATE clones fly the rest of the step; caught particles leave.
+ * flag_heat enables phonon heat tallies.  Tallies of this call go to *out. */
+int emc_fly_step(emc_ctx *ctx, uint64_t step, int flag_heat, emc_fly_tallies *out);
+
+/* Deposit (statistic + particle_to_density + compute_cell_charge) of the current
+ * population.  Outputs (any may be NULL), int64 fixed point:
+ *   node_q[nnode]       sum of q*w (EMC_FX_Q)                 particle_to_density
+ *   node_stat[10][nnode] vxE vyE vzE |q|E E|q|E, then the hole five (EMC_FX_STAT)
+ *   cell_q[3][ncell]    sum q electrons, sum q holes, sum |q| (EMC_FX_Q)
+ *   cell_n[ncell]       particle count per cell (stat_enum)                       */
+int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell_q, int64_t *cell_n);
+
+/* Poisson solve on the eliminated operator: rhs, sol of length nnode.  rhs holds
+ * the reference right-hand side (compute_rhs); sol is the initial guess on entry. */
+int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
+                      int *iters, double *relres);
+
+/* From a node potential: cell fields (pot2field) and surface rates, written
+ * into the context's cell fields; cell_rho must hold the clamped Rho. */
+int emc_compute_field(emc_ctx *ctx, const double *pot, const double *cell_rho);
+
+/* Flight tallies kept on the device across steps: node heat (EMC_FX_HEAT,
+ * heat_to_point src/heat.cpp:169-196) and per-y-plane crossing charge
+ * (EMC_FX_CONTACT, rcurrent/lcurrent include/mcmodel.h:236).  Reset zeroes
+ * them and the window statistics of emc_step (set_stat_zero src/mcmodel.cpp:716). */
+int emc_heat_get(emc_ctx *ctx, int64_t *heat_e, int64_t *heat_h);
+int emc_currents_get(emc_ctx *ctx, int64_t *rcur, int64_t *lcur);
+int emc_tallies_reset(emc_ctx *ctx);
+
+/* Poisson right-hand side of the eliminated system from a node charge sum
+ * (particle_to_density normalisation + compute_rhs, src/mcmodel.cpp:407-416, 2858-2893). */
+int emc_rhs_from_charge(emc_ctx *ctx, const int64_t *node_q, double *rhs_elim);
+
+/* Full step(s) on the GPU: deposit -> rhs -> PCG -> field/rates -> fly. */
+int emc_step(emc_ctx *ctx, int nsteps, emc_fly_tallies *acc);
+int emc_potential_get(emc_ctx *ctx, double *pot);
+
+/* Timers of the last emc_step call, milliseconds per step: deposit, poisson,
+ * field, fly, housekeeping; and PCG iterations of the last solve. */
+int emc_step_timing(emc_ctx *ctx, double *ms5, int *pcg_iters);
+
+/* Run the deck like `main input.txt` and write data/current. */
+int emc_run_file(const char *input_txt, const emc_opts *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMC_H */

@@ -1,0 +1,75 @@
+"""The C-ABI library loads, exports every symbol include/emc.h declares, and
+reports errors with status codes (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import REPO, cfg1_deck, emcgpu
+
+
+def header_functions():
+    src = open(os.path.join(REPO, "include", "emc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(emc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(emcgpu.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 24
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    sig = emcgpu.abi.declare(emcgpu.load_library())
+    assert set(header_functions()) <= set(sig), set(header_functions()) - set(sig)
+
+
+def test_host_only_context_and_views():
+    eng = emcgpu.Engine(cfg1_deck(), device=-1)
+    m = eng.mesh
+    assert (m.cnx, m.cny, m.cnz) == (20, 40, 12)
+    assert m.nnode == 21 * 41 * 13
+    b = eng.band
+    assert b.nt == 55320 and b.nk == 12619 and list(b.nband) == [4, 3, 1]
+    n = eng.particles_init()
+    p = eng.download()
+    assert p.n == n and n > 20000
+    assert set(np.unique(p.ptype)) == {0, 1}
+    # the flight refuses to run without a GPU instead of falling back to the CPU
+    with pytest.raises(emcgpu.EmcError) as e:
+        eng.fly_step(0)
+    assert e.value.code == emcgpu.abi.EMC_ERR_NOGPU
+    eng.close()
+
+
+def test_errors_have_codes(tmp_path):
+    with pytest.raises(emcgpu.EmcError) as e:
+        emcgpu.Engine(str(tmp_path / "missing.txt"), device=-1)
+    assert e.value.code == emcgpu.abi.EMC_ERR_IO
+    # a deck coordinate off the grid lines is rejected (distance2index, src/mcmodel.cpp:4168-4176)
+    d = tmp_path / "bad"
+    emcgpu.write_deck(str(d))
+    ldg = (d / "ldg.txt").read_text().replace("donor 0 10 -20 -7 0 5 1e26", "donor 0 10 -20 -7.5 0 5 1e26")
+    (d / "ldg.txt").write_text(ldg)
+    with pytest.raises(emcgpu.EmcError) as e:
+        emcgpu.Engine(str(d / "input.txt"), device=-1)
+    assert e.value.code == emcgpu.abi.EMC_ERR_DECK and "not aligned" in str(e.value)
+    with pytest.raises(emcgpu.EmcError) as e:
+        emcgpu.Engine(cfg1_deck(), device=-1, table_dir=str(tmp_path))
+    assert e.value.code == emcgpu.abi.EMC_ERR_IO
+
+
+def test_upload_validates_indices():
+    eng = emcgpu.Engine(cfg1_deck(), device=-1)
+    eng.particles_init()
+    p = eng.download()
+    p.itet[3] = 10 ** 6
+    with pytest.raises(emcgpu.EmcError) as e:
+        eng.upload(p)
+    assert e.value.code == emcgpu.abi.EMC_ERR_ARG
+    eng.close()

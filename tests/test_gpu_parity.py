@@ -1,0 +1,173 @@
+"""GPU parity: the HIP path against the oracle on identical inputs.
+
+Bar: integer outputs (cell, tetrahedron, symmetry indices, counts, int64
+tallies) are bit-exact; FP64 state (positions, wave vectors, energies) is
+bit-exact as well (tolerance 0 ulp), which is stricter than the 1e-12
+relative bound of the specification.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from helpers import cfg1_deck, emcgpu, field_like, sort_by_key, uniform_fields
+
+pytestmark = pytest.mark.gpu
+
+F64 = ("x", "y", "z", "kx", "ky", "kz", "energy", "charge")
+I32 = ("i", "j", "k", "itet", "isym", "ptype")
+
+
+def assert_same_population(pg, po, what=""):
+    a, b = sort_by_key(pg), sort_by_key(po)
+    assert a.n == b.n, f"{what}: live count GPU {a.n} vs oracle {b.n}"
+    np.testing.assert_array_equal(a.key, b.key, err_msg=f"{what}: keys")
+    for f in I32:
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f"{what}: {f}")
+    for f in F64:
+        x, y = getattr(a, f), getattr(b, f)
+        bad = np.nonzero(x != y)[0]
+        assert bad.size == 0, f"{what}: {f} differs at {bad.size} particles, first {bad[:3]} {x[bad[:3]]} {y[bad[:3]]}"
+
+
+def assert_same_tallies(tg, to, what=""):
+    dg, do = emcgpu.tallies_dict(tg), emcgpu.tallies_dict(to)
+    for k in dg:
+        np.testing.assert_array_equal(np.asarray(dg[k]), np.asarray(do[k]), err_msg=f"{what}: tally {k}")
+
+
+@pytest.fixture(scope="module")
+def eng():
+    print("[gpu] creating engine on the 20x40x12 FinFET deck", flush=True)
+    e = emcgpu.Engine(cfg1_deck(), device=0)
+    yield e
+    e.close()
+
+
+def test_flight_parity_multi_step(eng):
+    """Five flight steps on a sampled population under a device-like field, heat on."""
+    n = eng.particles_init()
+    assert n > 20000
+    pot, f = field_like(eng, seed=3, scale=20.0)
+    eng.fields_set(f)
+    eng.tallies_reset()
+    p_or = eng.download()
+    heat_o = np.zeros(eng.mesh.nnode, np.int64)
+    rc_o = np.zeros(eng.mesh.pny, np.int64)
+    for step in range(5):
+        print(f"[gpu] flight parity step {step}", flush=True)
+        tg = eng.fly_step(step, flag_heat=1)
+        pg = eng.download()
+        p_or, to, ex = oracle_lib.fly_step(eng, p_or, p_or.n, f, step, flag_heat=1, want_heat=True)
+        heat_o += ex["heat_e"]
+        rc_o += ex["rcur"]
+        assert_same_tallies(tg, to, f"step {step}")
+        assert_same_population(pg, p_or, f"step {step}")
+        p_or = sort_by_key(p_or)
+    he, hh = eng.heat()
+    np.testing.assert_array_equal(he, heat_o)
+    rc, lc = eng.currents()
+    np.testing.assert_array_equal(rc, rc_o)
+
+
+def test_flight_parity_hot_field(eng):
+    """Strong uniform field: tetrahedron and BZ-face crossings, impact ionisation range."""
+    eng.particles_init()
+    f = uniform_fields(eng, ex=2.0, ey=30.0, ez=-3.0)
+    eng.fields_set(f)
+    p_or = eng.download()
+    for step in range(3):
+        print(f"[gpu] hot-field parity step {step}", flush=True)
+        tg = eng.fly_step(100 + step)
+        pg = eng.download()
+        p_or, to, _ = oracle_lib.fly_step(eng, p_or, p_or.n, f, 100 + step)
+        assert_same_tallies(tg, to, f"hot step {step}")
+        assert_same_population(pg, p_or, f"hot step {step}")
+        p_or = sort_by_key(p_or)
+    assert tg.n_events > 2 * p_or.n // 2
+
+
+def test_deposit_parity(eng):
+    eng.particles_init()
+    pot, f = field_like(eng, seed=5)
+    eng.fields_set(f)
+    eng.fly_step(7)
+    p = eng.download()
+    print("[gpu] deposit parity", flush=True)
+    g = eng.deposit()
+    o = oracle_lib.deposit(eng, p, p.n)
+    for name, a, b in zip(("node_q", "node_stat", "cell_q", "cell_n"), g, o):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    live = p.i[:p.n] >= 0
+    total = np.rint(p.charge[:p.n][live] * emcgpu.FX_Q).sum()
+    assert abs(g[0].sum() - total) <= 8 * live.sum()
+
+
+def test_rhs_and_field_parity(eng):
+    import ctypes as C
+    lib = oracle_lib.load()
+    eng.particles_init()
+    node_q, _, cell_q, _ = eng.deposit()
+    rhs_g = eng.rhs_from_charge(node_q)
+    rhs_o = np.zeros(eng.mesh.nnode)
+    lib.orc_rhs(C.byref(eng.mesh), oracle_lib.i64ptr(node_q), None, None, oracle_lib.dptr(rhs_o))
+    np.testing.assert_array_equal(rhs_g, rhs_o)
+    rho = np.zeros(eng.mesh.ncell)
+    lib.orc_cell_rho(C.byref(eng.mesh), oracle_lib.i64ptr(cell_q), oracle_lib.dptr(rho))
+    pot, _ = field_like(eng, seed=9)
+    fg = eng.compute_field(pot, rho)
+    fo = emcgpu.CellFieldArrays(eng.mesh.ncell)
+    lib.orc_field(C.byref(eng.mesh), oracle_lib.dptr(pot), oracle_lib.dptr(fo.ex), oracle_lib.dptr(fo.ey),
+                  oracle_lib.dptr(fo.ez))
+    for k in ("ex", "ey", "ez"):
+        np.testing.assert_array_equal(getattr(fg, k), getattr(fo, k), err_msg=k)
+    lib.orc_surface_rates(C.byref(eng.mesh), oracle_lib.dptr(fo.ex), oracle_lib.dptr(fo.ey), oracle_lib.dptr(fo.ez),
+                          oracle_lib.dptr(rho), oracle_lib.dptr(fo.e_rough), oracle_lib.dptr(fo.e_sph),
+                          oracle_lib.dptr(fo.h_rough), oracle_lib.dptr(fo.h_sph))
+    # pow() differs between the device library and glibc by at most a few ulp
+    for k in ("e_rough", "e_sph", "h_rough", "h_sph"):
+        np.testing.assert_allclose(getattr(fg, k), getattr(fo, k), rtol=1e-13, atol=0, err_msg=k)
+
+
+def test_poisson_against_oracle(eng):
+    import ctypes as C
+    lib = oracle_lib.load()
+    eng.particles_init()
+    node_q, _, _, _ = eng.deposit()
+    rhs = eng.rhs_from_charge(node_q)
+    print("[gpu] poisson", flush=True)
+    xg, it, rel = eng.poisson_solve(rhs, rtol=1e-11)
+    assert rel <= 1e-11 and it > 0
+    xo = np.zeros_like(rhs)
+    rr = C.c_double(0)
+    lib.orc_pcg(C.byref(eng.mesh), oracle_lib.dptr(rhs), oracle_lib.dptr(xo), 1e-12, 100000, C.byref(rr))
+    assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
+    # residual of the operator as assembled by init_matrix, with the contact rows
+    rhs_ref = np.zeros_like(rhs)
+    lib.orc_rhs(C.byref(eng.mesh), oracle_lib.i64ptr(node_q), None, oracle_lib.dptr(rhs_ref), None)
+    ax = np.zeros_like(rhs)
+    lib.orc_apply_full(C.byref(eng.mesh), oracle_lib.dptr(xg), oracle_lib.dptr(ax))
+    assert np.linalg.norm(ax - rhs_ref) <= 1e-9 * np.linalg.norm(rhs_ref)
+
+
+def test_full_step_composition(eng):
+    """emc_step = deposit -> rhs -> PCG -> field -> fly; the flight of the step is
+    bit-exact against the oracle fed with the step's own fields."""
+    eng.particles_init()
+    p0 = eng.download()
+    eng.step(1)
+    f = eng.fields_get()
+    pot = eng.potential()
+    pg = eng.download()
+    # the step used step number 0 of this context's counter
+    import ctypes as C
+    lib = oracle_lib.load()
+    node_q, _, cell_q, _ = oracle_lib.deposit(eng, p0, p0.n)
+    rho = np.zeros(eng.mesh.ncell)
+    lib.orc_cell_rho(C.byref(eng.mesh), oracle_lib.i64ptr(cell_q), oracle_lib.dptr(rho))
+    np.testing.assert_array_equal(f.rho, rho)
+    fo = emcgpu.CellFieldArrays(eng.mesh.ncell)
+    lib.orc_field(C.byref(eng.mesh), oracle_lib.dptr(pot), oracle_lib.dptr(fo.ex), oracle_lib.dptr(fo.ey),
+                  oracle_lib.dptr(fo.ez))
+    np.testing.assert_array_equal(f.ex, fo.ex)
+    po, to, _ = oracle_lib.fly_step(eng, p0, p0.n, f, 0)
+    assert_same_population(pg, po, "full step")
