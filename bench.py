@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: full ensemble Monte Carlo steps of the 2M-electron n-FinFET on one MI355X.
+
+A step is the reference's density() + update_particle(): deposit (statistic,
+particle_to_density, compute_cell_charge), Poisson right-hand side, PCG solve
+(warm start), cell fields and surface rates, full-band flight of every particle
+with GENERATE/CATCH handling.  Workload: mesh 80x160x48 cells (x, y, z) of the
+n-FinFET deck, ElectronNumber=2000000, HoleNumber=500000 (plus the reference's
+one carrier of each type per silicon cell), dt = 0.1 fs, T = 300 K.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this build targets
+one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "c-mc_electron_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import numpy as np  # noqa: E402
+
+import emcgpu  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per particle of one k_fly launch: read x,y,z,kx,ky,kz,E,q (64 B),
+# packed cell word, itet, isym (12 B), key (8 B); write x,y,z,kx,ky,kz,E (56 B), cell word,
+# itet, isym (12 B)
+FLY_BYTES_PER_PARTICLE = 64 + 12 + 8 + 56 + 12
+
+
+def make_workload(refine, electrons, holes, tmp):
+    inp = emcgpu.write_deck(tmp, refine=refine, electrons=electrons, holes=holes, vd=0.5, vg=0.7,
+                            total_step=7000, stat_step=1000)
+    return inp
+
+
+def cpu_baseline(eng, steps):
+    """The oracle (CPU restatement, 1 thread) runs complete steps of the same
+    workload: deposit, RHS, PCG (warm start from the GPU potential), fields,
+    surface rates and the flight of every particle."""
+    import oracle_lib
+    lib = oracle_lib.load()
+    m = eng.mesh
+    p = eng.download()
+    pot = eng.potential()
+    n_ps = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        live = int(np.sum(p.i[:p.n] >= 0))
+        node_q, _, cell_q, _ = oracle_lib.deposit(eng, p, p.n)
+        rhs = np.zeros(m.nnode)
+        lib.orc_rhs(C.byref(m), oracle_lib.i64ptr(node_q), None, None, oracle_lib.dptr(rhs))
+        rr = C.c_double(0)
+        lib.orc_pcg(C.byref(m), oracle_lib.dptr(rhs), oracle_lib.dptr(pot), 1e-10, 100000, C.byref(rr))
+        f = emcgpu.CellFieldArrays(m.ncell)
+        lib.orc_cell_rho(C.byref(m), oracle_lib.i64ptr(cell_q), oracle_lib.dptr(f.rho))
+        lib.orc_field(C.byref(m), oracle_lib.dptr(pot), oracle_lib.dptr(f.ex), oracle_lib.dptr(f.ey),
+                      oracle_lib.dptr(f.ez))
+        f.hx, f.hy, f.hz = f.ex, f.ey, f.ez
+        lib.orc_surface_rates(C.byref(m), oracle_lib.dptr(f.ex), oracle_lib.dptr(f.ey), oracle_lib.dptr(f.ez),
+                              oracle_lib.dptr(f.rho), oracle_lib.dptr(f.e_rough), oracle_lib.dptr(f.e_sph),
+                              oracle_lib.dptr(f.h_rough), oracle_lib.dptr(f.h_sph))
+        p, _, _ = oracle_lib.fly_step(eng, p, p.n, f, 10 ** 9 + s)
+        n_ps += live
+    dt = time.perf_counter() - t0
+    return n_ps / dt, dt, n_ps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--refine", type=int, default=4)
+    ap.add_argument("--electrons", type=int, default=2000000)
+    ap.add_argument("--holes", type=int, default=500000)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    tmp = tempfile.mkdtemp(prefix="emc_bench_")
+    inp = make_workload(a.refine, a.electrons, a.holes, tmp)
+    print(f"[bench] deck {inp}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    eng = emcgpu.Engine(inp, device=0, seed=1)
+    n0 = eng.particles_init()
+    print(f"[bench] {n0} particles on {eng.mesh.cnx}x{eng.mesh.cny}x{eng.mesh.cnz} cells, "
+          f"{eng.mesh.nnode} nodes, setup {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    for w in range(a.warmup):
+        eng.step(1)
+        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+    n_start = eng.count()
+    lib = eng.lib
+    phase = np.zeros(5)
+    pcg_it = []
+    n_ps = 0
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        n_ps += eng.count()
+        eng.step(1)
+        ms, it = eng.timing()
+        phase += ms
+        pcg_it.append(it)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    phase /= max(1, a.steps)
+    value = n_ps / elapsed
+    n_avg = n_ps / max(1, a.steps)
+    fly_ms = phase[4]
+    achieved = FLY_BYTES_PER_PARTICLE * n_avg / (fly_ms * 1e-3) / 1e9 if fly_ms > 0 else None
+    print(f"[bench] {a.steps} steps in {elapsed:.3f}s; per step ms: deposit+rhs {phase[0]:.3f} "
+          f"pcg {phase[1]:.3f} field {phase[2]:.3f} fly(all) {phase[3]:.3f} k_fly {phase[4]:.3f}; "
+          f"pcg iters {np.mean(pcg_it):.1f}", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu and a.cpu_steps > 0:
+        v, dt, nps = cpu_baseline(eng, a.cpu_steps)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{a.cpu_steps} complete steps of the same workload ({nps} particle-steps, {dt:.1f} s) "
+                         f"by the oracle in C on one host core"}
+        print(f"[bench] cpu baseline {v:.4g} particle-steps/s over {dt:.1f}s", file=sys.stderr, flush=True)
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"n-FinFET {eng.mesh.cnx}x{eng.mesh.cny}x{eng.mesh.cnz} cells, "
+                               f"{n_start} particles (ElectronNumber={a.electrons}, HoleNumber={a.holes}), "
+                               f"dt 0.1 fs, full step: deposit + PCG Poisson + fields + flight",
+                   "particles": int(n_start), "nodes": int(eng.mesh.nnode),
+                   "phase_ms": {"deposit_rhs": phase[0], "poisson": phase[1], "field": phase[2],
+                                "flight_total": phase[3], "k_fly": phase[4]},
+                   "pcg_iters_mean": float(np.mean(pcg_it))},
+        "roofline": {"bound": "hbm", "kernel": "k_fly", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": None,
+                     "bytes_per_particle": FLY_BYTES_PER_PARTICLE},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -234,8 +234,9 @@ int emc_rhs_from_charge(emc_ctx *ctx, const int64_t *node_q, double *rhs_elim);
 int emc_step(emc_ctx *ctx, int nsteps, emc_fly_tallies *acc);
 int emc_potential_get(emc_ctx *ctx, double *pot);
 
-/* Timers of the last emc_step call, milliseconds per step: deposit, poisson,
- * field, fly, housekeeping; and PCG iterations of the last solve. */
+/* Timers of the last emc_step call, milliseconds per step (HIP events on the
+ * engine's stream): deposit+rhs, Poisson, field+rates, flight with clone passes
+ * and merge, and the k_fly kernel alone; and PCG iterations of the last solve. */
 int emc_step_timing(emc_ctx *ctx, double *ms5, int *pcg_iters);
 
 /* Run the deck like `main input.txt` and write data/current. */
