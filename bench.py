@@ -72,6 +72,18 @@ def cpu_baseline(eng, steps):
     return n_ps / dt, dt, n_ps
 
 
+def copy_probe(mib):
+    """A device-to-device copy of known size: the bytes the HBM counters should report for
+    it calibrate the traffic counted for the engine's kernels."""
+    import torch
+    src = torch.ones(mib * 2 ** 20 // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": mib * 2 ** 20, "ok": bool(dst[-1].item() == 1.0)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +94,12 @@ def main():
     ap.add_argument("--holes", type=int, default=500000)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)

@@ -77,7 +77,8 @@ struct Gpu {
   unsigned long long clone_cap = 0;
   long long* dead = nullptr;
   unsigned long long dead_cap = 0;
-  unsigned long long* ctr = nullptr;  // [0] nclones [1] ndead [2] ticket
+  unsigned long long* ctr = nullptr;  // [0] nclones [1] ndead [2] ticket [3] slow-list length
+  uint32_t* slow = nullptr;           // [cap] flight slow list
   // fields / mesh work arrays
   double *ex = nullptr, *ey = nullptr, *ez = nullptr, *rho = nullptr, *rates = nullptr;
   unsigned long long *node_q = nullptr, *node_stat = nullptr, *cell_q = nullptr, *cell_n = nullptr;
@@ -269,6 +270,8 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   Gpu& g = *c->gpu;
   if (g.cap >= cap) return EMC_OK;
   if (g.cap > 0) return fail(c, EMC_ERR_CAPACITY, "particle arrays are already allocated with a smaller capacity");
+  // particle slots are listed as 32-bit indices by the flight kernels
+  if (cap > 0x7fffffffLL) return fail(c, EMC_ERR_CAPACITY, "particle capacity above 2^31-1");
   DevBuf& m = g.mem;
   g.cap = cap;
   g.P.x = m.alloc<double>(cap); g.P.y = m.alloc<double>(cap); g.P.z = m.alloc<double>(cap);
@@ -280,6 +283,7 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   g.clones = m.alloc<CloneRec>(g.clone_cap);
   g.dead_cap = g.clone_cap;
   g.dead = m.alloc<long long>(g.dead_cap);
+  g.slow = m.alloc<uint32_t>(cap);
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("particle allocation: ") + hipGetErrorString(m.last));
   return EMC_OK;
 }
@@ -626,6 +630,7 @@ static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
   a.n = g.n; a.cap = g.cap; a.step = step; a.flag_heat = flag_heat;
   a.clones = g.clones; a.nclones = g.ctr; a.clone_cap = g.clone_cap;
   a.dead = g.dead; a.ndead = g.ctr + 1; a.dead_cap = g.dead_cap;
+  a.slow = g.slow; a.nslow = g.ctr + 3;
   return a;
 }
 

@@ -745,82 +745,103 @@ struct Flight {
     return t;
   }
 
-  // particle_fly (src/mcmodel.cpp:1874-2161)
-  EMC_HD void fly() {
-    bool getTet = true, getCell = true, getPh = true, getImp = true, getSurf = true;
-    double ssnl = 0, imprnl = 0, phrnl = 0;
-    double TetTf = 0, CellTf = 0;
+  // particle_fly (src/mcmodel.cpp:1874-2161), split into the parts the fast path reuses.
+  struct Loop {
+    bool getTet, getCell, getPh, getImp, getSurf;
+    double ssnl, imprnl, phrnl, TetTf, CellTf;
+    int loop;
+  };
+  EMC_HD void fly_begin(Loop& L) {
+    L.getTet = L.getCell = L.getPh = L.getImp = L.getSurf = true;
+    L.ssnl = L.imprnl = L.phrnl = 0;
+    L.TetTf = L.CellTf = 0;
+    L.loop = 0;
     Flag_Catch = 0; drop_reason = -1;
     SurfScGamma = 0; SurfRoughnessScRate = 0; SurfPhononScRate = 0; ImpScGamma = 0;
     Flag_SelfScatter = 0; idir = 0; itetdir = 0;
     load_cell();
     iband = B.tr[itet].ibt;
-    int loop = 0;
+  }
+  // candidate times of the current iteration; returns the Min(...) flag and Tf
+  EMC_HD int next_event(Loop& L, double& Tf, double& gam, size_t& c) {
+    if (L.getTet) { L.TetTf = TetTime(); L.getTet = false; }
+    if (L.getCell) { L.CellTf = CellTime(); L.getCell = false; }
+    if (L.getPh) { L.phrnl = -dm_log(Random()); L.getPh = false; }
+    gam = B.tr[itet].gam;
+    const double PhScTf = L.phrnl / gam;
+    double ImpScTf;
+    if (iband == B.bandofE) {
+      if (L.getImp) { L.imprnl = -dm_log(Random()); L.getImp = false; }
+      ImpScGamma = GetImpScGamma();
+      ImpScTf = L.imprnl / ImpScGamma;
+    } else {
+      ImpScTf = 2 * M.dt;
+      ImpScGamma = 1.0 / M.scrt0;
+    }
+    c = cell(icell, jcell, kcell);
+    double SurfScTf;
+    if (M.flag_surface && M.insurf[c]) {
+      if (L.getSurf) { L.ssnl = -dm_log(Random()); L.getSurf = false; }
+      if (par_type == PELEC) { SurfRoughnessScRate = F.e_rough[c]; SurfPhononScRate = F.e_sph[c]; }
+      else { SurfRoughnessScRate = F.h_rough[c]; SurfPhononScRate = F.h_sph[c]; }
+      SurfScGamma = SurfRoughnessScRate + SurfPhononScRate;
+      SurfScTf = (SurfScGamma > 0 && L.ssnl > 0) ? L.ssnl / SurfScGamma : 2 * M.dt;
+    } else
+      SurfScTf = 2 * M.dt;
+    // Min(6 args, &flag): the first argument wins ties (src/Constant_GlobalVariable_MathFunction.cpp:92-118)
+    int flag = 1;
+    Tf = L.TetTf;
+    if (Tf > L.CellTf) { Tf = L.CellTf; flag = 2; }
+    if (Tf > PhScTf) { Tf = PhScTf; flag = 3; }
+    if (Tf > ImpScTf) { Tf = ImpScTf; flag = 4; }
+    if (Tf > SurfScTf) { Tf = SurfScTf; flag = 5; }
+    if (Tf > left_time) { Tf = left_time; flag = 6; }
+    return flag;
+  }
+  // free flight for Tf (src/mcmodel.cpp:2009-2046)
+  EMC_HD void advance(Loop& L, double Tf, double gam) {
+    L.TetTf -= Tf; L.CellTf -= Tf;
+    left_time -= Tf;
+    L.phrnl -= gam * Tf;
+    L.imprnl -= ImpScGamma * Tf;
+    L.ssnl -= SurfScGamma * Tf;
+    kx += Ex * Tf; ky += Ey * Tf; kz += Ez * Tf;
+    energy += (vx * Ex + vy * Ey + vz * Ez) * Tf;
+    x += vx * Tf; y += vy * Tf; z += vz * Tf;
+  }
+
+  EMC_HD void fly() {
+    Loop L;
+    fly_begin(L);
     while (left_time > 0) {
       GetV();
-      loop++;
+      L.loop++;
       n_events++;
-      if (loop > FLY_LOOP_CAP) { Flag_Catch = 1; drop_reason = 3; break; }
-      if (getTet) { TetTf = TetTime(); getTet = false; }
-      if (getCell) { CellTf = CellTime(); getCell = false; }
-      if (getPh) { phrnl = -dm_log(Random()); getPh = false; }
-      const double gam = B.tr[itet].gam;
-      double PhScTf = phrnl / gam;
-      double ImpScTf;
-      if (iband == B.bandofE) {
-        if (getImp) { imprnl = -dm_log(Random()); getImp = false; }
-        ImpScGamma = GetImpScGamma();
-        ImpScTf = imprnl / ImpScGamma;
-      } else {
-        ImpScTf = 2 * M.dt;
-        ImpScGamma = 1.0 / M.scrt0;
-      }
-      const size_t c = cell(icell, jcell, kcell);
-      double SurfScTf;
-      if (M.flag_surface && M.insurf[c]) {
-        if (getSurf) { ssnl = -dm_log(Random()); getSurf = false; }
-        if (par_type == PELEC) { SurfRoughnessScRate = F.e_rough[c]; SurfPhononScRate = F.e_sph[c]; }
-        else { SurfRoughnessScRate = F.h_rough[c]; SurfPhononScRate = F.h_sph[c]; }
-        SurfScGamma = SurfRoughnessScRate + SurfPhononScRate;
-        SurfScTf = (SurfScGamma > 0 && ssnl > 0) ? ssnl / SurfScGamma : 2 * M.dt;
-      } else
-        SurfScTf = 2 * M.dt;
-      // Min(6 args, &flag): the first argument wins ties (src/Constant_GlobalVariable_MathFunction.cpp:92-118)
-      int flag = 1;
-      double Tf = TetTf;
-      if (Tf > CellTf) { Tf = CellTf; flag = 2; }
-      if (Tf > PhScTf) { Tf = PhScTf; flag = 3; }
-      if (Tf > ImpScTf) { Tf = ImpScTf; flag = 4; }
-      if (Tf > SurfScTf) { Tf = SurfScTf; flag = 5; }
-      if (Tf > left_time) { Tf = left_time; flag = 6; }
+      if (L.loop > FLY_LOOP_CAP) { Flag_Catch = 1; drop_reason = 3; break; }
+      double Tf, gam;
+      size_t c;
+      const int flag = next_event(L, Tf, gam, c);
       if (Tf < MY_ZERO) { Flag_Catch = 1; drop_reason = 0; break; }
-      TetTf -= Tf; CellTf -= Tf;
-      left_time -= Tf;
-      phrnl -= gam * Tf;
-      imprnl -= ImpScGamma * Tf;
-      ssnl -= SurfScGamma * Tf;
-      kx += Ex * Tf; ky += Ey * Tf; kz += Ez * Tf;
-      energy += (vx * Ex + vy * Ey + vz * Ez) * Tf;
-      x += vx * Tf; y += vy * Tf; z += vz * Tf;
+      advance(L, Tf, gam);
       if (energy < 0) { Flag_Catch = 1; drop_reason = 1; break; }
       switch (flag) {
         case 1:
           HitTet();
-          getTet = getCell = true;
+          L.getTet = L.getCell = true;
           break;
         case 2:
-          if (HitCell()) getTet = getCell = true;
+          if (HitCell()) L.getTet = L.getCell = true;
           else { Flag_Catch = 1; drop_reason = 4; }
           break;
         case 3:
           PhononScatter();
-          if (!Flag_SelfScatter) { getTet = getCell = true; n_phonon++; }
-          getPh = true;
+          if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_phonon++; }
+          L.getPh = true;
           break;
         case 4:
           if (par_type == PELEC) ElectronImpurityScatter();
-          if (!Flag_SelfScatter) { getTet = getCell = true; n_impurity++; }
-          getImp = true;
+          if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_impurity++; }
+          L.getImp = true;
           break;
         case 5:
           // ParticleSurfaceScatter (src/ParSurfaceScatter.cpp:48-61)
@@ -831,14 +852,31 @@ struct Flight {
             Flag_SelfScatter = 0;
             select_process(Random() * calscattsum(energy, iband));
           }
-          if (!Flag_SelfScatter) getTet = getCell = true;
-          getSurf = true;
+          if (!Flag_SelfScatter) L.getTet = L.getCell = true;
+          L.getSurf = true;
           break;
         default:
           break;
       }
       if (Flag_Catch) break;
     }
+  }
+
+  // The common case of a step: the first event is the end of the time step.  Runs
+  // exactly the first loop iteration of fly(); returns false (state untouched in
+  // memory) when anything else happens first, so fly() can redo the particle.
+  EMC_HD bool fly_fast() {
+    Loop L;
+    fly_begin(L);
+    GetV();
+    double Tf, gam;
+    size_t c;
+    const int flag = next_event(L, Tf, gam, c);
+    if (flag != 6 || Tf < MY_ZERO) return false;
+    advance(L, Tf, gam);
+    if (energy < 0) return false;
+    n_events = 1;
+    return true;
   }
 };
 

@@ -24,6 +24,8 @@ struct FlyArgs {
   long long* dead;             // slots freed by caught particles
   unsigned long long* ndead;   // device counter
   unsigned long long dead_cap;
+  uint32_t* slow;              // [cap] particles k_fly_fast hands to k_fly_slow
+  unsigned long long* nslow;   // device counter, zero before the flight
 };
 
 struct DepositArgs {

@@ -1,6 +1,8 @@
 // HIP kernels of the MI355X ensemble Monte Carlo step (gfx950, wave64).
 //
-//   k_fly        one lane per particle, full-band event loop (fly_core.h)
+//   k_fly_fast   one lane per particle: drift of those with no event this step,
+//                the rest listed for
+//   k_fly_slow   one lane per listed particle, full-band event loop (fly_core.h)
 //   k_fly_clones same for GENERATE clones of this step
 //   k_merge      surviving clones -> holes left by caught particles, then append
 //   k_deposit    CIC charge, |q|-weighted node statistics, NGP cell sums (int64 atomics)
@@ -43,26 +45,72 @@ __device__ inline void finish_drop(const TallyDev& T, const Flight& f) {
 }
 
 // ------------------------------------------------------------------ flight
-__global__ __launch_bounds__(256) void k_fly(FlyArgs a) {
-  long long ev = 0, ph = 0, im = 0, ca = 0;
+__device__ inline void load_particle(Flight& f, const FlyArgs& a, long long q, uint32_t pk) {
+  f.clones = a.clones;
+  f.nclones = a.nclones;
+  f.clone_cap = a.clone_cap;
+  f.x = a.P.x[q]; f.y = a.P.y[q]; f.z = a.P.z[q];
+  f.kx = a.P.kx[q]; f.ky = a.P.ky[q]; f.kz = a.P.kz[q];
+  f.energy = a.P.energy[q]; f.charge = a.P.charge[q];
+  f.icell = pk_i(pk); f.jcell = pk_j(pk); f.kcell = pk_k(pk); f.par_type = pk_type(pk);
+  f.itet = a.P.itet[q]; f.isym = a.P.isym[q];
+  f.key = a.P.key[q]; f.step = a.step;
+  f.left_time = a.M.dt;
+  f.nclone = 0; f.flag_heat = a.flag_heat;
+  f.n_events = 0; f.n_phonon = 0; f.n_impurity = 0;
+  f.rng.init(f.key, a.step);
+}
+
+// Fast pass: every particle whose first event is the end of the time step drifts
+// and is written back (position, wave vector and energy; cell, tetrahedron and
+// wedge are unchanged).  The others are listed, wave-compacted, for k_fly_slow,
+// which restarts them from the untouched state: the RNG is counter based, so the
+// full event loop draws the same numbers and the result equals one pass of fly().
+__global__ __launch_bounds__(256) void k_fly_fast(FlyArgs a) {
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += stride) {
+  long long ev = 0;
+  const int lane = threadIdx.x & 63;
+  // the loop bound is uniform per block so every lane reaches the ballot
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+    const long long q = base + threadIdx.x;
+    bool slow = false;
+    if (q < a.n) {
+      const uint32_t pk = a.P.pk[q];
+      if (pk != PK_DEAD) {
+        Flight f(a.B, a.M, a.F, a.T);
+        load_particle(f, a, q, pk);
+        if (f.fly_fast()) {
+          ev += f.n_events;
+          a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
+          a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
+          a.P.energy[q] = f.energy;
+        } else
+          slow = true;
+      }
+    }
+    const unsigned long long m = __ballot(slow);
+    if (m) {
+      const int leader = __builtin_ctzll(m);
+      unsigned long long b0 = 0;
+      if (lane == leader) b0 = atomicAdd(a.nslow, (unsigned long long)__popcll(m));
+      b0 = __shfl(b0, leader, 64);
+      if (slow) a.slow[b0 + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)q;
+    }
+  }
+  ev = wave_sum(ev);
+  if (lane == 0 && ev) atomicAdd(&a.T.counters[CNT_EVENTS], (unsigned long long)ev);
+}
+
+// Full event loop for the particles k_fly_fast listed; one lane per entry.
+__global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
+  long long ev = 0, ph = 0, im = 0, ca = 0;
+  const long long ns = (long long)*a.nslow;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ns; e += stride) {
+    const long long q = a.slow[e];
     const uint32_t pk = a.P.pk[q];
-    if (pk == PK_DEAD) continue;
     Flight f(a.B, a.M, a.F, a.T);
-    f.clones = a.clones;
-    f.nclones = a.nclones;
-    f.clone_cap = a.clone_cap;
-    f.x = a.P.x[q]; f.y = a.P.y[q]; f.z = a.P.z[q];
-    f.kx = a.P.kx[q]; f.ky = a.P.ky[q]; f.kz = a.P.kz[q];
-    f.energy = a.P.energy[q]; f.charge = a.P.charge[q];
-    f.icell = pk_i(pk); f.jcell = pk_j(pk); f.kcell = pk_k(pk); f.par_type = pk_type(pk);
-    f.itet = a.P.itet[q]; f.isym = a.P.isym[q];
-    f.key = a.P.key[q]; f.step = a.step;
-    f.left_time = a.M.dt;
-    f.nclone = 0; f.flag_heat = a.flag_heat;
-    f.n_events = 0; f.n_phonon = 0; f.n_impurity = 0;
-    f.rng.init(f.key, a.step);
+    load_particle(f, a, q, pk);
     f.fly();
     ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
     if (f.Flag_Catch) {
@@ -407,8 +455,12 @@ static int grid_for(long long n, int bs, int maxg = 65536) {
   return (int)g;
 }
 
+// a.nslow must read zero on entry; the slow grid covers every particle, blocks
+// past the listed count exit at once
 hipError_t emc_launch_fly(const FlyArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_fly, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fly_fast, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_fly_slow, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_fly_clones(const FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s) {
