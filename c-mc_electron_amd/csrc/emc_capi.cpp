@@ -17,6 +17,7 @@
 #include "device.h"
 #include "fly_core.h"
 #include "kernels.h"
+#include "multigrid.h"
 
 using namespace emc;
 
@@ -72,7 +73,15 @@ struct Gpu {
   FieldsDev F{};
   TallyDev T{};
   PartDev P{};
+  PartDev P2{};                       // second particle buffer: the tile sort writes into it
   long long n = 0, cap = 0;
+  // tile sort / deposit
+  int ntx = 0, nty = 0, ntz = 0, ntile = 0;
+  unsigned *tcount = nullptr, *toff = nullptr, *tcursor = nullptr;
+  unsigned long long* blk = nullptr;
+  unsigned* n_host = nullptr;         // pinned: live count after the last sort
+  hipEvent_t n_ready = nullptr;
+  bool n_pending = false;
   CloneRec* clones = nullptr;
   unsigned long long clone_cap = 0;
   long long* dead = nullptr;
@@ -87,7 +96,12 @@ struct Gpu {
   // Poisson
   double *diag = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *dirichlet = nullptr, *corr = nullptr;
   double *pvol = nullptr, *pdop = nullptr, *rhs = nullptr, *pot = nullptr;
-  double *r = nullptr, *p = nullptr, *q = nullptr, *part = nullptr, *part2 = nullptr, *part3 = nullptr, *scal = nullptr;
+  double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *scal = nullptr;
+  double *part_a = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_rz0 = nullptr, *part_rz1 = nullptr;
+  int* done = nullptr;
+  MgArgs mg{};                        // device view of the multigrid hierarchy
+  double* scal_host = nullptr;        // pinned [2][8]: r.r etc. after iterations (ping-pong)
+  hipEvent_t scal_ready[2] = {nullptr, nullptr};
   uint8_t* kind = nullptr;
   const double* cvol = nullptr;
   const double* dx = nullptr;
@@ -98,6 +112,15 @@ struct Gpu {
   int pcg_blocks = 0;
   hipEvent_t ev[8] = {};
 };
+
+// Particle arrays of one buffer, allocated together.
+static void alloc_part(DevBuf& m, PartDev& P, long long cap) {
+  P.x = m.alloc<double>(cap); P.y = m.alloc<double>(cap); P.z = m.alloc<double>(cap);
+  P.kx = m.alloc<double>(cap); P.ky = m.alloc<double>(cap); P.kz = m.alloc<double>(cap);
+  P.energy = m.alloc<double>(cap); P.charge = m.alloc<double>(cap);
+  P.pk = m.alloc<uint32_t>(cap); P.itet = m.alloc<int32_t>(cap); P.isym = m.alloc<int32_t>(cap);
+  P.key = m.alloc<uint64_t>(cap);
+}
 
 }  // namespace
 
@@ -110,6 +133,7 @@ struct emc_ctx {
   std::vector<int> surface_dir;
   HostParticles hp;
   std::unique_ptr<Gpu> gpu;
+  MgHierarchy mg;                     // built on first use
   int last_pcg_iters = 0;
   double last_relres = 0;
   double timing[5] = {0, 0, 0, 0, 0};
@@ -179,6 +203,13 @@ static void build_tetrec(emc_ctx* c) {
   }
 }
 
+// Multigrid hierarchy of the Poisson preconditioner, built once per context.
+static std::string ensure_mg(emc_ctx* c) {
+  if (!c->mg.lv.empty()) return "";
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  return mg_build(c->dev, c->mg, 600, nth);
+}
+
 // ---------------------------------------------------------------- GPU setup
 static int gpu_setup(emc_ctx* c) {
   int ndev = 0;
@@ -239,6 +270,14 @@ static int gpu_setup(emc_ctx* c) {
   g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
   g.counters = m.alloc<unsigned long long>(CNT_N);
   g.ctr = m.alloc<unsigned long long>(4);
+  g.ntx = (d.cnx + TILE_I - 1) / TILE_I;
+  g.nty = (d.cny + TILE_J - 1) / TILE_J;
+  g.ntz = (d.cnz + TILE_K - 1) / TILE_K;
+  g.ntile = g.ntx * g.nty * g.ntz;
+  g.tcount = m.alloc<unsigned>(g.ntile);
+  g.tcursor = m.alloc<unsigned>(g.ntile);
+  g.toff = m.alloc<unsigned>(g.ntile + 1);
+  g.blk = m.alloc<unsigned long long>((size_t)g.ntile * DEP_NQ * TILE_NODES);
   g.T.contact = g.contact; g.T.counters = g.counters; g.T.rcur = g.rcur; g.T.lcur = g.lcur;
   g.T.heat_e = g.heat_e; g.T.heat_h = g.heat_h;
 
@@ -246,12 +285,39 @@ static int gpu_setup(emc_ctx* c) {
   g.dirichlet = m.dup(d.dirichlet_val); g.corr = m.dup(d.rhs_corr); g.kind = m.dup(d.node_kind);
   g.pvol = m.dup(d.pvol); g.pdop = m.dup(d.pdop);
   g.rhs = m.alloc<double>(NN); g.pot = m.alloc<double>(NN);
-  g.r = m.alloc<double>(NN); g.p = m.alloc<double>(NN); g.q = m.alloc<double>(NN);
+  g.r = m.alloc<double>(NN); g.p = m.alloc<double>(NN); g.q = m.alloc<double>(NN); g.z = m.alloc<double>(NN);
   g.pcg_blocks = emc_pcg_blocks(NN);
-  g.part = m.alloc<double>(3 * g.pcg_blocks); g.part2 = m.alloc<double>(3 * g.pcg_blocks);
-  g.part3 = m.alloc<double>(3 * g.pcg_blocks); g.scal = m.alloc<double>(8);
+  for (double** pp : {&g.part_a, &g.part_pq, &g.part_rr, &g.part_rz0, &g.part_rz1}) *pp = m.alloc<double>(3 * g.pcg_blocks);
+  g.scal = m.alloc<double>(8);
+  g.done = m.alloc<int>(1);
+  {
+    const std::string e = ensure_mg(c);
+    if (!e.empty()) return fail(c, EMC_ERR_DECK, e);
+    const MgHierarchy& h = c->mg;
+    if ((int)h.lv.size() > MG_MAXL) return fail(c, EMC_ERR_DECK, "mesh needs more multigrid levels than supported");
+    MgArgs& A = g.mg;
+    A.nl = (int)h.lv.size();
+    A.nc = (int)h.lv.back().n;
+    A.cinv = m.dup(h.cinv);
+    A.diag = g.diag; A.cx = g.cx; A.cy = g.cy; A.cz = g.cz;
+    A.done = g.done;
+    for (int l = 0; l < A.nl; ++l) {
+      const MgLevel& L = h.lv[l];
+      MgLevelDev& D = A.L[l];
+      D.nx = L.nx; D.ny = L.ny; D.nz = L.nz; D.n = L.n;
+      D.a27 = l > 0 ? m.dup(L.a27) : nullptr;
+      D.wdinv = m.dup(L.wdinv);
+      D.x = m.alloc<double>(L.n); D.t = m.alloc<double>(L.n); D.r = m.alloc<double>(L.n);
+      D.b = l > 0 ? m.alloc<double>(L.n) : g.r;
+    }
+  }
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
+  HIPCHECK(c, hipMemset(g.tcount, 0, g.ntile * sizeof(unsigned)));
+  HIPCHECK(c, hipHostMalloc((void**)&g.n_host, sizeof(unsigned), hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 16 * sizeof(double), hipHostMallocDefault));
+  for (auto& e : g.scal_ready) HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHECK(c, hipEventCreateWithFlags(&g.n_ready, hipEventDisableTiming));
   HIPCHECK(c, hipMemset(g.ex, 0, NC * sizeof(double)));
   HIPCHECK(c, hipMemset(g.ey, 0, NC * sizeof(double)));
   HIPCHECK(c, hipMemset(g.ez, 0, NC * sizeof(double)));
@@ -274,11 +340,8 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   if (cap > 0x7fffffffLL) return fail(c, EMC_ERR_CAPACITY, "particle capacity above 2^31-1");
   DevBuf& m = g.mem;
   g.cap = cap;
-  g.P.x = m.alloc<double>(cap); g.P.y = m.alloc<double>(cap); g.P.z = m.alloc<double>(cap);
-  g.P.kx = m.alloc<double>(cap); g.P.ky = m.alloc<double>(cap); g.P.kz = m.alloc<double>(cap);
-  g.P.energy = m.alloc<double>(cap); g.P.charge = m.alloc<double>(cap);
-  g.P.pk = m.alloc<uint32_t>(cap); g.P.itet = m.alloc<int32_t>(cap); g.P.isym = m.alloc<int32_t>(cap);
-  g.P.key = m.alloc<uint64_t>(cap);
+  alloc_part(m, g.P, cap);
+  alloc_part(m, g.P2, cap);
   g.clone_cap = (unsigned long long)std::max<long long>(4096, cap / 16);
   g.clones = m.alloc<CloneRec>(g.clone_cap);
   g.dead_cap = g.clone_cap;
@@ -371,12 +434,17 @@ static void sample_population(emc_ctx* c) {
   for (auto& t : th) t.join();
 }
 
+static int settle_n(emc_ctx* c);
+
 static int gpu_upload_host(emc_ctx* c) {
   Gpu& g = *c->gpu;
   const HostParticles& hp = c->hp;
   const long long n = (long long)hp.size();
   const long long want = (long long)std::ceil(n * std::max(1.0, c->opts.capacity_factor)) + 1024;
-  int st = gpu_alloc_particles(c, want);
+  int st = settle_n(c);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  st = gpu_alloc_particles(c, want);
   if (st) return st;
   if (n > g.cap) return fail(c, EMC_ERR_CAPACITY, "particle count exceeds the device capacity");
   std::vector<uint32_t> pk(n);
@@ -439,6 +507,11 @@ void emc_destroy(emc_ctx* c) {
     (void)hipSetDevice(c->gpu->device);
     (void)hipDeviceSynchronize();
     for (auto e : c->gpu->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (c->gpu->n_ready) (void)hipEventDestroy(c->gpu->n_ready);
+    if (c->gpu->n_host) (void)hipHostFree(c->gpu->n_host);
+    if (c->gpu->scal_host) (void)hipHostFree(c->gpu->scal_host);
+    for (auto e : c->gpu->scal_ready)
       if (e) (void)hipEventDestroy(e);
     if (c->gpu->stream) (void)hipStreamDestroy(c->gpu->stream);
     c->gpu->mem.free_all();
@@ -516,6 +589,10 @@ int emc_particles_init(emc_ctx* c, int64_t* n_out) {
 
 int emc_particles_count(emc_ctx* c, int64_t* n_out) {
   if (!c || !n_out) return EMC_ERR_ARG;
+  if (c->gpu) {
+    int st = settle_n(c);
+    if (st) return st;
+  }
   *n_out = c->gpu ? c->gpu->n : (int64_t)c->hp.size();
   return EMC_OK;
 }
@@ -555,6 +632,8 @@ int emc_particles_download(emc_ctx* c, emc_particles* h, int64_t cap, int64_t* n
     return EMC_OK;
   }
   Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
   const long long n = g.n;
   if (n_out) *n_out = n;
   if (n > cap) return fail(c, EMC_ERR_CAPACITY, "host buffer too small");
@@ -638,6 +717,8 @@ static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
 // the clone passes; updates g.n.
 static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
   Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
   HIPCHECK(c, emc_launch_zero(g.ctr, 4, g.stream));
   FlyArgs a = fly_args(c, step, flag_heat);
   HIPCHECK(c, hipEventRecord(g.ev[5], g.stream));
@@ -693,12 +774,44 @@ static int zero_tallies(emc_ctx* c) {
   return EMC_OK;
 }
 
-static DepositArgs deposit_args(emc_ctx* c) {
+// The live count after a tile sort is known on the device first; the host picks it
+// up (pinned copy + event) at the next point that needs it.
+static int settle_n(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  if (!g.n_pending) return EMC_OK;
+  HIPCHECK(c, hipEventSynchronize(g.n_ready));
+  g.n = (long long)*g.n_host;
+  g.n_pending = false;
+  return EMC_OK;
+}
+
+// Counting sort of the population by deposit tile into the second buffer, which
+// becomes the current one.  Empty slots are dropped.
+static int run_sort(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
+  SortArgs a{};
+  a.P = g.P; a.Q = g.P2; a.n = g.n;
+  a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
+  a.tcount = g.tcount; a.toff = g.toff; a.tcursor = g.tcursor;
+  HIPCHECK(c, emc_launch_tile_sort(a, g.stream));
+  std::swap(g.P, g.P2);
+  HIPCHECK(c, hipMemcpyAsync(g.n_host, g.toff + g.ntile, sizeof(unsigned), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipEventRecord(g.n_ready, g.stream));
+  g.n_pending = true;
+  return EMC_OK;
+}
+
+static DepositArgs deposit_args(emc_ctx* c, int stat_accumulate) {
   Gpu& g = *c->gpu;
   DepositArgs a{};
-  a.B = g.B; a.M = g.M; a.P = g.P; a.n = g.n;
+  a.B = g.B; a.M = g.M; a.P = g.P;
   a.nnode = c->dev.nnode; a.ncell = c->dev.ncell;
+  a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
+  a.toff = g.toff; a.blk = g.blk;
   a.node_q = g.node_q; a.node_stat = g.node_stat; a.cell_q = g.cell_q; a.cell_n = g.cell_n;
+  a.stat_accumulate = stat_accumulate;
   return a;
 }
 
@@ -724,34 +837,52 @@ static FieldArgs field_args(emc_ctx* c, const double* pot, bool rho_from_cells) 
   return a;
 }
 
-static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x) {
+static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
   Gpu& g = *c->gpu;
   const Device& d = c->dev;
   PcgArgs a{};
   a.n = d.nnode; a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz; a.nblocks = g.pcg_blocks;
   a.diag = g.diag; a.cx = g.cx; a.cy = g.cy; a.cz = g.cz; a.b = b; a.kind = g.kind;
-  a.x = x; a.r = g.r; a.p = g.p; a.q = g.q;
-  a.part = g.part; a.part2 = g.part2; a.part3 = g.part3; a.scal = g.scal;
+  a.x = x; a.r = g.r; a.p = g.p; a.q = g.q; a.z = g.z;
+  a.part_a = g.part_a; a.part_pq = g.part_pq; a.part_rr = g.part_rr; a.part_rz0 = g.part_rz0; a.part_rz1 = g.part_rz1;
+  a.scal = g.scal; a.done = g.done; a.rtol = rtol;
   return a;
 }
 
-// Jacobi-PCG; convergence read back every `chk` iterations.
+// Multigrid-preconditioned CG to ||r|| <= rtol ||b|| (the solve of
+// PoissonSolver::solve_poisson, src/PoissonSolver.cpp:70-78, on the eliminated
+// system).  Iteration k+1 is queued before the host reads the residual of
+// iteration k, so the GPU never waits for the check; once converged the device
+// flag turns the queued kernels into no-ops.
 static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres) {
   Gpu& g = *c->gpu;
-  PcgArgs a = pcg_args(c, b, x);
-  HIPCHECK(c, emc_launch_pcg_start(a, g.stream));
-  double s[3];
-  HIPCHECK(c, hipMemcpyAsync(s, g.scal, sizeof(s), hipMemcpyDeviceToHost, g.stream));
+  PcgArgs a = pcg_args(c, b, x, rtol);
+  HIPCHECK(c, emc_launch_pcg_start(a, g.mg, g.stream));
+  double* sh = g.scal_host;
+  HIPCHECK(c, hipMemcpyAsync(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
-  const double bn = std::sqrt(s[2]) > 0 ? std::sqrt(s[2]) : 1.0;
-  double rel = std::sqrt(s[0]) / bn;
+  const double bb = sh[2];
+  const double bn = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
+  double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
+  if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
   int it = 0;
-  const int chk = 16;
-  while (rel > rtol && it < maxit) {
-    for (int q = 0; q < chk && it < maxit; ++q, ++it) HIPCHECK(c, emc_launch_pcg_iter(a, it, g.stream));
-    HIPCHECK(c, hipMemcpyAsync(s, g.scal, sizeof(double) * 2, hipMemcpyDeviceToHost, g.stream));
+  if (rel > rtol) {
+    for (int k = 0; k < maxit; ++k) {
+      HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
+      double* slot = sh + 8 * (k & 1);
+      HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+      HIPCHECK(c, hipEventRecord(g.scal_ready[k & 1], g.stream));
+      if (k == 0) continue;
+      // residual after iteration k-1 (iteration k is already queued)
+      HIPCHECK(c, hipEventSynchronize(g.scal_ready[(k - 1) & 1]));
+      const double* prev = sh + 8 * ((k - 1) & 1);
+      if (!std::isfinite(prev[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
+      if (prev[0] <= prev[4]) break;
+    }
     HIPCHECK(c, hipStreamSynchronize(g.stream));
-    rel = std::sqrt(std::max(0.0, s[0])) / bn;
+    HIPCHECK(c, hipMemcpy(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost));
+    it = (int)sh[3];
+    rel = std::sqrt(std::max(0.0, sh[0])) / bn;
     if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
   }
   if (iters) *iters = it;
@@ -814,12 +945,12 @@ int emc_deposit(emc_ctx* c, int64_t* node_q, int64_t* node_stat, int64_t* cell_q
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
   Gpu& g = *c->gpu;
   const size_t NN = c->dev.nnode, NC = c->dev.ncell;
-  HIPCHECK(c, emc_launch_zero(g.node_q, NN, g.stream));
-  HIPCHECK(c, emc_launch_zero(g.node_stat, 10 * NN, g.stream));
-  HIPCHECK(c, emc_launch_zero(g.cell_q, 3 * NC, g.stream));
-  HIPCHECK(c, emc_launch_zero(g.cell_n, NC, g.stream));
-  HIPCHECK(c, emc_launch_deposit(deposit_args(c), g.stream));
+  int st = run_sort(c);
+  if (st) return st;
+  HIPCHECK(c, emc_launch_deposit(deposit_args(c, 0), g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
+  st = settle_n(c);
+  if (st) return st;
   if (node_q) HIPCHECK(c, hipMemcpy(node_q, g.node_q, NN * 8, hipMemcpyDeviceToHost));
   if (node_stat) HIPCHECK(c, hipMemcpy(node_stat, g.node_stat, 10 * NN * 8, hipMemcpyDeviceToHost));
   if (cell_q) HIPCHECK(c, hipMemcpy(cell_q, g.cell_q, 3 * NC * 8, hipMemcpyDeviceToHost));
@@ -854,6 +985,18 @@ int emc_poisson_solve(emc_ctx* c, const double* rhs, double* sol, double rtol, i
   return st;
 }
 
+int emc_poisson_mg_level(emc_ctx* c, int level, int dims[3], double* a27, double* omega) {
+  if (!c || !dims) return EMC_ERR_ARG;
+  const std::string e = ensure_mg(c);
+  if (!e.empty()) return fail(c, EMC_ERR_DECK, e);
+  if (level < 0 || level >= (int)c->mg.lv.size()) return fail(c, EMC_ERR_ARG, "no such multigrid level");
+  const MgLevel& L = c->mg.lv[level];
+  dims[0] = L.nx; dims[1] = L.ny; dims[2] = L.nz;
+  if (a27 && level > 0) std::memcpy(a27, L.a27.data(), L.a27.size() * sizeof(double));
+  if (omega) *omega = L.omega;
+  return EMC_OK;
+}
+
 int emc_compute_field(emc_ctx* c, const double* pot, const double* cell_rho) {
   if (!c || !pot || !cell_rho) return EMC_ERR_ARG;
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
@@ -880,20 +1023,18 @@ int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the step runs on the GPU only");
   Gpu& g = *c->gpu;
   if (g.cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
-  const size_t NN = c->dev.nnode, NC = c->dev.ncell;
+  const size_t NN = c->dev.nnode;
   double tsum[5] = {0, 0, 0, 0, 0};
   for (int s = 0; s < nsteps; ++s) {
     const uint64_t step = c->next_step++;
     HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
-    HIPCHECK(c, emc_launch_zero(g.node_q, NN, g.stream));
-    HIPCHECK(c, emc_launch_zero(g.cell_q, 3 * NC, g.stream));
-    HIPCHECK(c, emc_launch_zero(g.cell_n, NC, g.stream));
-    DepositArgs da = deposit_args(c);
-    HIPCHECK(c, emc_launch_deposit(da, g.stream));
+    int st = run_sort(c);
+    if (st) return st;
+    HIPCHECK(c, emc_launch_deposit(deposit_args(c, 1), g.stream));
     RhsArgs ra{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
     HIPCHECK(c, emc_launch_rhs(ra, g.stream));
     HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
-    int st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr);
+    st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr);
     if (st) return st;
     HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
     HIPCHECK(c, emc_launch_field(field_args(c, g.pot, true), g.stream));

@@ -28,13 +28,36 @@ struct FlyArgs {
   unsigned long long* nslow;   // device counter, zero before the flight
 };
 
+// Deposit tiles: TILE_I x TILE_J x TILE_K cells (i, j, k) per workgroup; a tile's
+// node block spans (TILE_I+1)(TILE_J+1)(TILE_K+1) nodes.  Tile index
+// t = (tj*ntx + ti)*ntz + tk follows the node order (k fastest, then i, then j).
+constexpr int TILE_I = 8, TILE_J = 8, TILE_K = 4;
+constexpr int TILE_NODES = (TILE_I + 1) * (TILE_J + 1) * (TILE_K + 1);
+constexpr int TILE_CELLS = TILE_I * TILE_J * TILE_K;
+constexpr int DEP_NQ = 11;  // node charge + 5 statistics x 2 carrier types
+
+// Counting sort of the particles by tile (drops empty slots).
+struct SortArgs {
+  PartDev P;                  // source
+  PartDev Q;                  // destination, tile order
+  long long n;
+  int ntx, nty, ntz, ntile;
+  unsigned* tcount;           // [ntile] zero on entry; left zero by k_tile_scan
+  unsigned* toff;             // [ntile + 1]
+  unsigned* tcursor;          // [ntile] zeroed by k_tile_scan
+};
+
 struct DepositArgs {
   BandDev B;
   MeshDev M;
-  PartDev P;
-  long long n;
+  PartDev P;                  // tile order
   size_t nnode, ncell;
+  int ntx, nty, ntz, ntile;
+  const unsigned* toff;       // [ntile + 1]
+  unsigned long long* blk;    // [ntile][DEP_NQ][TILE_NODES] per-tile node sums
   unsigned long long *node_q, *node_stat, *cell_q, *cell_n;  // any may be null
+  int stat_accumulate;        // 1: node_stat += this step's sums, 0: node_stat = sums
+  int pad0;
 };
 
 struct RhsArgs {
@@ -61,14 +84,37 @@ struct FieldArgs {
   double mass_eff[2];
 };
 
+// CG on the eliminated 7-point operator (level 0 of the multigrid hierarchy).
 struct PcgArgs {
   size_t n;
   int pnx, pny, pnz, nblocks;
   const double *diag, *cx, *cy, *cz, *b;
   const uint8_t* kind;
-  double *x, *r, *p, *q;
-  double *part, *part2, *part3;  // [3*nblocks] each
-  double* scal;                  // [0] r.r  [1] r.z  [2] b.b
+  double *x, *r, *p, *q, *z;
+  double *part_a, *part_pq, *part_rr, *part_rz0, *part_rz1;  // [3*nblocks] block partial sums
+  double* scal;  // [0] r.r [1] r.z [2] b.b [3] iterations done [4] r.r at the tolerance
+  int* done;     // set once r.r reaches scal[4]: every later CG / V-cycle kernel returns at once
+  double rtol;
+  int pad0;
+};
+
+constexpr int MG_MAXL = 10;
+
+struct MgLevelDev {
+  int nx, ny, nz, pad;
+  size_t n;
+  const double* a27;     // [27][n], levels >= 1
+  const double* wdinv;   // [n] omega/diag, 0 at fixed nodes
+  double *x, *t, *r, *b; // work vectors; b of level 0 is the CG residual
+};
+
+// One V(2,2) cycle of the hierarchy built by mg_build (multigrid.h).
+struct MgArgs {
+  MgLevelDev L[MG_MAXL];
+  int nl, nc;                          // levels, nodes of the coarsest level
+  const double* cinv;                  // [nc*nc] coarsest inverse, row major
+  const double *diag, *cx, *cy, *cz;   // level-0 operator
+  const int* done;
 };
 
 }  // namespace emc
@@ -77,10 +123,11 @@ hipError_t emc_launch_fly(const emc::FlyArgs& a, hipStream_t s);
 hipError_t emc_launch_fly_clones(const emc::FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s);
 hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long nclones, unsigned long long ndead,
                             unsigned long long* ticket, hipStream_t s);
+hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);
 hipError_t emc_launch_field(const emc::FieldArgs& a, hipStream_t s);
 hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 int emc_pcg_blocks(size_t n);
-hipError_t emc_launch_pcg_start(const emc::PcgArgs& a, hipStream_t s);
-hipError_t emc_launch_pcg_iter(const emc::PcgArgs& a, int it, hipStream_t s);
+hipError_t emc_launch_pcg_start(const emc::PcgArgs& a, const emc::MgArgs& m, hipStream_t s);
+hipError_t emc_launch_pcg_iter(const emc::PcgArgs& a, const emc::MgArgs& m, int it, hipStream_t s);

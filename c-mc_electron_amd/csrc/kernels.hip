@@ -5,10 +5,13 @@
 //   k_fly_slow   one lane per listed particle, full-band event loop (fly_core.h)
 //   k_fly_clones same for GENERATE clones of this step
 //   k_merge      surviving clones -> holes left by caught particles, then append
-//   k_deposit    CIC charge, |q|-weighted node statistics, NGP cell sums (int64 atomics)
+//   k_tile_*     counting sort of the particles by 8x8x4-cell tile (drops empty slots)
+//   k_deposit_*  CIC charge, |q|-weighted node statistics, NGP cell sums: LDS per
+//                tile, then a per-node gather of the tile blocks (int64, no global atomics)
 //   k_rhs        node density and Poisson right-hand side (eliminated system)
 //   k_field      cell fields from node potential, Rho clamp, surface rates
-//   k_pcg_*      Jacobi-preconditioned CG on the 7-point operator
+//   k_pcg_*      CG on the 7-point operator, preconditioned by
+//   k_mg_*       one multigrid V(2,2) cycle (damped Jacobi, Galerkin coarse levels)
 //
 // All HBM-bound integer/FP64 work: no MFMA.  Compiled with -ffp-contract=off so
 // the FP64 results match the CPU restatement bit for bit.
@@ -185,28 +188,132 @@ __global__ __launch_bounds__(256) void k_merge(FlyArgs a, unsigned long long ncl
   }
 }
 
+// ------------------------------------------------------------------ tile sort
+__device__ inline int tile_of(const SortArgs& a, uint32_t pk) {
+  return ((pk_j(pk) / TILE_J) * a.ntx + pk_i(pk) / TILE_I) * a.ntz + pk_k(pk) / TILE_K;
+}
+
+// Lanes holding the same key in consecutive positions form a run; the run's first
+// lane adds the run length to ctr[key] with one atomic and every lane of the run
+// gets old value + its rank in the run.  key < 0 takes no part.  All 64 lanes of
+// the wave must call it together.
+__device__ inline unsigned wave_run_add(unsigned* ctr, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int hl = 63 - __builtin_clzll(heads & upto);
+  const unsigned long long above = heads & ~upto;
+  const int nh = above ? __builtin_ctzll(above) : 64;
+  unsigned base = 0;
+  if (head && key >= 0) base = atomicAdd(&ctr[key], (unsigned)(nh - lane));
+  base = __shfl(base, hl, 64);
+  return base + (unsigned)(lane - hl);
+}
+
+__global__ __launch_bounds__(256) void k_tile_hist(SortArgs a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+    const long long q = base + threadIdx.x;
+    int t = -1;
+    if (q < a.n) {
+      const uint32_t pk = a.P.pk[q];
+      if (pk != PK_DEAD) t = tile_of(a, pk);
+    }
+    (void)wave_run_add(a.tcount, t);
+  }
+}
+
+// One block of 1024 threads: exclusive scan of the tile counts into toff, then the
+// counts and cursors are cleared for the next sort.
+__global__ __launch_bounds__(1024) void k_tile_scan(SortArgs a) {
+  __shared__ unsigned wsum[16];
+  const int nt = a.ntile;
+  const int tid = threadIdx.x;
+  const int chunk = (nt + 1023) / 1024;
+  const int b0 = tid * chunk, b1 = min(nt, b0 + chunk);
+  unsigned loc = 0;
+  for (int t = b0; t < b1; ++t) loc += a.tcount[t];
+  // inclusive scan of loc over the block
+  const int lane = tid & 63, w = tid >> 6;
+  unsigned v = loc;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  unsigned woff = 0;
+  for (int q = 0; q < w; ++q) woff += wsum[q];
+  unsigned run = woff + v - loc;  // exclusive prefix of this thread's chunk
+  for (int t = b0; t < b1; ++t) {
+    const unsigned cnt = a.tcount[t];
+    a.toff[t] = run;
+    run += cnt;
+    a.tcount[t] = 0;
+    a.tcursor[t] = 0;
+  }
+  if (tid == 1023) {
+    unsigned tot = 0;
+    for (int q = 0; q < 16; ++q) tot += wsum[q];
+    a.toff[nt] = tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tile_scatter(SortArgs a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+    const long long q = base + threadIdx.x;
+    int t = -1;
+    uint32_t pk = PK_DEAD;
+    if (q < a.n) {
+      pk = a.P.pk[q];
+      if (pk != PK_DEAD) t = tile_of(a, pk);
+    }
+    const unsigned r = wave_run_add(a.tcursor, t);
+    if (t < 0) continue;
+    const size_t d = (size_t)a.toff[t] + r;
+    a.Q.x[d] = a.P.x[q]; a.Q.y[d] = a.P.y[q]; a.Q.z[d] = a.P.z[q];
+    a.Q.kx[d] = a.P.kx[q]; a.Q.ky[d] = a.P.ky[q]; a.Q.kz[d] = a.P.kz[q];
+    a.Q.energy[d] = a.P.energy[q]; a.Q.charge[d] = a.P.charge[q];
+    a.Q.pk[d] = pk; a.Q.itet[d] = a.P.itet[q]; a.Q.isym[d] = a.P.isym[q]; a.Q.key[d] = a.P.key[q];
+  }
+}
+
 // ------------------------------------------------------------------ deposit
 // statistic (src/mcmodel.cpp:1167-1408), particle_to_density (:348-417) and
-// compute_cell_charge (:2664-2712) in one pass over the particles.
-__global__ __launch_bounds__(256) void k_deposit(DepositArgs a) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  const size_t NN = a.nnode, NC = a.ncell;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += stride) {
+// compute_cell_charge (:2664-2712) in one pass over the tile-sorted particles.
+// One workgroup per tile sums the fixed-point contributions in LDS; each cell
+// belongs to one tile and is stored directly, the node block goes to `blk` and
+// k_deposit_gather adds the (up to 8) blocks that share a node.  Integer sums:
+// the result does not depend on the order of the particles.
+__global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
+  __shared__ unsigned long long sn[DEP_NQ * TILE_NODES];
+  __shared__ unsigned long long sc[4 * TILE_CELLS];
+  const int t = blockIdx.x;
+  const int tk = t % a.ntz, ti = (t / a.ntz) % a.ntx, tj = t / (a.ntz * a.ntx);
+  const int i0 = ti * TILE_I, j0 = tj * TILE_J, k0 = tk * TILE_K;
+  const unsigned p0 = a.toff[t], p1 = a.toff[t + 1];
+  const size_t NC = a.ncell;
+  for (int e = threadIdx.x; e < DEP_NQ * TILE_NODES; e += blockDim.x) sn[e] = 0;
+  for (int e = threadIdx.x; e < 4 * TILE_CELLS; e += blockDim.x) sc[e] = 0;
+  __syncthreads();
+  const bool want_stat = a.node_stat != nullptr;
+  for (unsigned q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
     const uint32_t pk = a.P.pk[q];
-    if (pk == PK_DEAD) continue;
     const int i = pk_i(pk), j = pk_j(pk), k = pk_k(pk), pt = pk_type(pk);
     const double x = a.P.x[q], y = a.P.y[q], z = a.P.z[q], charge = a.P.charge[q], energy = a.P.energy[q];
     const double rx = (x - a.M.lx[i]) / a.M.dx[i];
     const double ry = (y - a.M.ly[j]) / a.M.dy[j];
     const double rz = (z - a.M.lz[k]) / a.M.dz[k];
-    const size_t c = ((size_t)j * a.M.cnx + i) * a.M.cnz + k;
-    const int pnx = a.M.pnx, pnz = a.M.pnz;
-    auto node = [&](int ii, int jj, int kk) { return ((size_t)jj * pnx + ii) * pnz + kk; };
-    const size_t nd[8] = {node(i, j, k),         node(i + 1, j, k),         node(i, j, k + 1),
-                          node(i + 1, j, k + 1), node(i, j + 1, k),         node(i + 1, j + 1, k),
-                          node(i, j + 1, k + 1), node(i + 1, j + 1, k + 1)};
-    if (a.cell_n) atomicAdd(&a.cell_n[c], 1ull);
-    if (a.node_stat) {
+    const int li = i - i0, lj = j - j0, lk = k - k0;
+    auto lnode = [&](int di, int dj, int dk) { return ((lj + dj) * (TILE_I + 1) + (li + di)) * (TILE_K + 1) + (lk + dk); };
+    const int nd[8] = {lnode(0, 0, 0), lnode(1, 0, 0), lnode(0, 0, 1), lnode(1, 0, 1),
+                       lnode(0, 1, 0), lnode(1, 1, 0), lnode(0, 1, 1), lnode(1, 1, 1)};
+    const int lc = (lj * TILE_I + li) * TILE_K + lk;
+    atomicAdd(&sc[3 * TILE_CELLS + lc], 1ull);
+    if (want_stat) {
       const int isym = a.P.isym[q];
       const double* vt = a.B.tr[a.P.itet[q]].vt;
       const int* ms = a.B.matsym;
@@ -214,7 +321,7 @@ __global__ __launch_bounds__(256) void k_deposit(DepositArgs a) {
       const double vx = v3[ms[0 * 48 + isym] - 1] * ms[3 * 48 + isym];
       const double vy = v3[ms[1 * 48 + isym] - 1] * ms[4 * 48 + isym];
       const double vz = v3[ms[2 * 48 + isym] - 1] * ms[5 * 48 + isym];
-      unsigned long long* S = a.node_stat + (size_t)(pt == 0 ? 0 : 5) * NN;
+      unsigned long long* S = sn + (size_t)(pt == 0 ? 1 : 6) * TILE_NODES;
       const double cc[8] = {charge * (1 - rx) * (1 - ry) * (1 - rz), charge * rx * (1 - ry) * (1 - rz),
                             charge * (1 - rx) * (1 - ry) * rz,       charge * rx * (1 - ry) * rz,
                             charge * (1 - rx) * ry * (1 - rz),       charge * rx * ry * (1 - rz),
@@ -222,25 +329,77 @@ __global__ __launch_bounds__(256) void k_deposit(DepositArgs a) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const double f = fabs(cc[e]);
-        atomicAdd(&S[0 * NN + nd[e]], (unsigned long long)fxd(vx * f, 4294967296.0));
-        atomicAdd(&S[1 * NN + nd[e]], (unsigned long long)fxd(vy * f, 4294967296.0));
-        atomicAdd(&S[2 * NN + nd[e]], (unsigned long long)fxd(vz * f, 4294967296.0));
-        atomicAdd(&S[3 * NN + nd[e]], (unsigned long long)fxd(f, 4294967296.0));
-        atomicAdd(&S[4 * NN + nd[e]], (unsigned long long)fxd(f * energy, 4294967296.0));
+        atomicAdd(&S[0 * TILE_NODES + nd[e]], (unsigned long long)fxd(vx * f, 4294967296.0));
+        atomicAdd(&S[1 * TILE_NODES + nd[e]], (unsigned long long)fxd(vy * f, 4294967296.0));
+        atomicAdd(&S[2 * TILE_NODES + nd[e]], (unsigned long long)fxd(vz * f, 4294967296.0));
+        atomicAdd(&S[3 * TILE_NODES + nd[e]], (unsigned long long)fxd(f, 4294967296.0));
+        atomicAdd(&S[4 * TILE_NODES + nd[e]], (unsigned long long)fxd(f * energy, 4294967296.0));
       }
     }
-    if (a.node_q) {
+    {
       const double cc = charge;
       const double w[8] = {cc * (1 - ry) * (1 - rx) * (1 - rz), cc * (1 - ry) * rx * (1 - rz),
                            cc * (1 - ry) * (1 - rx) * rz,       cc * (1 - ry) * rx * rz,
                            cc * ry * (1 - rx) * (1 - rz),       cc * ry * rx * (1 - rz),
                            cc * ry * (1 - rx) * rz,             cc * ry * rx * rz};
 #pragma unroll
-      for (int e = 0; e < 8; ++e) atomicAdd(&a.node_q[nd[e]], (unsigned long long)fxd(w[e], 281474976710656.0));
+      for (int e = 0; e < 8; ++e) atomicAdd(&sn[nd[e]], (unsigned long long)fxd(w[e], 281474976710656.0));
     }
+    atomicAdd(&sc[(pt == 0 ? 0 : TILE_CELLS) + lc], (unsigned long long)fxd(charge, 281474976710656.0));
+    atomicAdd(&sc[2 * TILE_CELLS + lc], (unsigned long long)fxd(fabs(charge), 281474976710656.0));
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TILE_CELLS; e += blockDim.x) {
+    const int lk = e % TILE_K, li = (e / TILE_K) % TILE_I, lj = e / (TILE_K * TILE_I);
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    if (i >= a.M.cnx || j >= a.M.cny || k >= a.M.cnz) continue;
+    const size_t c = ((size_t)j * a.M.cnx + i) * a.M.cnz + k;
+    if (a.cell_n) a.cell_n[c] = sc[3 * TILE_CELLS + e];
     if (a.cell_q) {
-      atomicAdd(&a.cell_q[(pt == 0 ? 0 : NC) + c], (unsigned long long)fxd(charge, 281474976710656.0));
-      atomicAdd(&a.cell_q[2 * NC + c], (unsigned long long)fxd(fabs(charge), 281474976710656.0));
+      a.cell_q[c] = sc[e];
+      a.cell_q[NC + c] = sc[TILE_CELLS + e];
+      a.cell_q[2 * NC + c] = sc[2 * TILE_CELLS + e];
+    }
+  }
+  if (p1 > p0) {
+    unsigned long long* b = a.blk + (size_t)t * DEP_NQ * TILE_NODES;
+    for (int e = threadIdx.x; e < DEP_NQ * TILE_NODES; e += blockDim.x) b[e] = sn[e];
+  }
+}
+
+// One thread per node: sum the node's entries of the non-empty tiles around it.
+__global__ __launch_bounds__(256) void k_deposit_gather(DepositArgs a) {
+  const size_t NN = a.nnode;
+  const int pnx = a.M.pnx, pnz = a.M.pnz;
+  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < NN; n += (size_t)gridDim.x * blockDim.x) {
+    const int K = (int)(n % pnz), I = (int)((n / pnz) % pnx), J = (int)(n / ((size_t)pnz * pnx));
+    int tx[2], lx[2], nx = 0, ty[2], ly[2], ny = 0, tz[2], lz[2], nz = 0;
+    if (I / TILE_I < a.ntx) { tx[nx] = I / TILE_I; lx[nx++] = I % TILE_I; }
+    if (I % TILE_I == 0 && I > 0) { tx[nx] = I / TILE_I - 1; lx[nx++] = TILE_I; }
+    if (J / TILE_J < a.nty) { ty[ny] = J / TILE_J; ly[ny++] = J % TILE_J; }
+    if (J % TILE_J == 0 && J > 0) { ty[ny] = J / TILE_J - 1; ly[ny++] = TILE_J; }
+    if (K / TILE_K < a.ntz) { tz[nz] = K / TILE_K; lz[nz++] = K % TILE_K; }
+    if (K % TILE_K == 0 && K > 0) { tz[nz] = K / TILE_K - 1; lz[nz++] = TILE_K; }
+    unsigned long long s[DEP_NQ];
+#pragma unroll
+    for (int q = 0; q < DEP_NQ; ++q) s[q] = 0;
+    for (int b = 0; b < ny; ++b)
+      for (int aa = 0; aa < nx; ++aa)
+        for (int c = 0; c < nz; ++c) {
+          const int t = (ty[b] * a.ntx + tx[aa]) * a.ntz + tz[c];
+          if (a.toff[t + 1] == a.toff[t]) continue;
+          const int ln = (ly[b] * (TILE_I + 1) + lx[aa]) * (TILE_K + 1) + lz[c];
+          const unsigned long long* blk = a.blk + (size_t)t * DEP_NQ * TILE_NODES + ln;
+#pragma unroll
+          for (int q = 0; q < DEP_NQ; ++q) s[q] += blk[(size_t)q * TILE_NODES];
+        }
+    if (a.node_q) a.node_q[n] = s[0];
+    if (a.node_stat) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q) {
+        unsigned long long* dst = a.node_stat + (size_t)q * NN + n;
+        *dst = (a.stat_accumulate ? *dst : 0ull) + s[1 + q];
+      }
     }
   }
 }
@@ -367,75 +526,243 @@ __device__ inline void node_ijk(const PcgArgs& a, size_t n, int& i, int& j, int&
   j = (int)(t / a.pnx);
 }
 
-// fixed nodes take their value from b
+#define GRID_LOOP(n, N) for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < (N); n += (size_t)gridDim.x * blockDim.x)
+
+// fixed nodes take their value from b; the solve starts not done
 __global__ __launch_bounds__(256) void k_pcg_fix(PcgArgs a) {
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.n; n += (size_t)gridDim.x * blockDim.x)
-    if (a.kind[n] != 0) a.x[n] = a.b[n];
+  GRID_LOOP(n, a.n) if (a.kind[n] != 0) a.x[n] = a.b[n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) { *a.done = 0; a.scal[3] = 0; }
 }
 
-// r = b - A x, p = z = r/diag ; partials (r.z, r.r, b.b) -> pa
-__global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a, double* pa) {
-  double rz = 0, rr = 0, bb = 0;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.n; n += (size_t)gridDim.x * blockDim.x) {
+// r = b - A x ; partials (r.r, b.b) -> part_a
+__global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a) {
+  double rr = 0, bb = 0;
+  GRID_LOOP(n, a.n) {
     int i, j, k;
     node_ijk(a, n, i, j, k);
     const double r = a.b[n] - apply_op(a, a.x, n, i, j, k);
-    const double z = r / a.diag[n];
     a.r[n] = r;
-    a.p[n] = z;
-    rz += r * z;
     rr += r * r;
     bb += a.b[n] * a.b[n];
   }
-  block_reduce3(rz, rr, bb, pa);
+  block_reduce3(rr, bb, 0.0, a.part_a);
 }
 
-// scal[0] = r.r, scal[1] = r.z, scal[2] = b.b of the starting residual
-__global__ __launch_bounds__(256) void k_pcg_norms(PcgArgs a, const double* pa) {
-  double t[3];
-  total3(pa, a.nblocks, t);
-  if (threadIdx.x == 0) { a.scal[0] = t[1]; a.scal[1] = t[0]; a.scal[2] = t[2]; }
+// p = z ; scal = (r.r, r.z, b.b, 0, rtol^2 b.b)
+__global__ __launch_bounds__(256) void k_pcg_begin(PcgArgs a) {
+  double ta[3], tz[3];
+  total3(a.part_a, a.nblocks, ta);
+  total3(a.part_rz0, a.nblocks, tz);
+  GRID_LOOP(n, a.n) a.p[n] = a.z[n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.scal[0] = ta[0]; a.scal[1] = tz[0]; a.scal[2] = ta[1];
+    a.scal[4] = a.rtol * a.rtol * ta[1];
+  }
 }
 
-// q = A p ; partial p.q -> pb
-__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, double* pb) {
+// q = A p ; partial p.q
+__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a) {
+  if (*a.done) return;
   double pq = 0;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.n; n += (size_t)gridDim.x * blockDim.x) {
+  GRID_LOOP(n, a.n) {
     int i, j, k;
     node_ijk(a, n, i, j, k);
     const double q = apply_op(a, a.p, n, i, j, k);
     a.q[n] = q;
     pq += a.p[n] * q;
   }
-  block_reduce3(pq, 0.0, 0.0, pb);
+  block_reduce3(pq, 0.0, 0.0, a.part_pq);
 }
 
-// alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partials (r.z, r.r) -> pc
-__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* pa, const double* pb, double* pc) {
+// alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
+__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_old) {
+  if (*a.done) return;
   double ta[3], tb[3];
-  total3(pa, a.nblocks, ta);
-  total3(pb, a.nblocks, tb);
+  total3(rz_old, a.nblocks, ta);
+  total3(a.part_pq, a.nblocks, tb);
   const double alpha = (tb[0] != 0.0) ? ta[0] / tb[0] : 0.0;
-  double rz = 0, rr = 0;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.n; n += (size_t)gridDim.x * blockDim.x) {
+  double rr = 0;
+  GRID_LOOP(n, a.n) {
     a.x[n] += alpha * a.p[n];
     const double r = a.r[n] - alpha * a.q[n];
     a.r[n] = r;
-    rz += r * (r / a.diag[n]);
     rr += r * r;
   }
-  block_reduce3(rz, rr, 0.0, pc);
+  block_reduce3(rr, 0.0, 0.0, a.part_rr);
 }
 
-// beta = r.z(new) / r.z(old) ; p = z + beta p ; scal[0] = r.r, scal[1] = r.z
-__global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, const double* pa, const double* pc) {
-  double ta[3], tc[3];
-  total3(pa, a.nblocks, ta);
-  total3(pc, a.nblocks, tc);
-  const double beta = (ta[0] != 0.0) ? tc[0] / ta[0] : 0.0;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.n; n += (size_t)gridDim.x * blockDim.x)
-    a.p[n] = a.r[n] / a.diag[n] + beta * a.p[n];
-  if (blockIdx.x == 0 && threadIdx.x == 0) { a.scal[0] = tc[1]; a.scal[1] = tc[0]; }
+// beta = r.z(new) / r.z(old) ; p = z + beta p ; convergence flag
+__global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, const double* rz_old, const double* rz_new) {
+  if (*a.done) return;
+  double to[3], tn[3], tr[3];
+  total3(rz_old, a.nblocks, to);
+  total3(rz_new, a.nblocks, tn);
+  total3(a.part_rr, a.nblocks, tr);
+  const double beta = (to[0] != 0.0) ? tn[0] / to[0] : 0.0;
+  GRID_LOOP(n, a.n) a.p[n] = a.z[n] + beta * a.p[n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.scal[0] = tr[0];
+    a.scal[1] = tn[0];
+    a.scal[3] += 1.0;
+    if (tr[0] <= a.scal[4]) *a.done = 1;
+  }
+}
+
+// ------------------------------------------------------------------ multigrid V-cycle
+__device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, int& k) {
+  k = (int)(n % L.nz);
+  const size_t t = n / L.nz;
+  i = (int)(t % L.nx);
+  j = (int)(t / L.nx);
+}
+
+// (A_l x)(n); level 0 is the 7-point operator, coarser levels the 27-point Galerkin one
+template <bool FINE>
+__device__ inline double mg_apply(const MgArgs& m, const MgLevelDev& L, const double* __restrict__ x, size_t n, int i,
+                                  int j, int k) {
+  if (FINE) {
+    const size_t sx = L.nz, sy = (size_t)L.nx * L.nz;
+    double v = m.diag[n] * x[n];
+    if (i + 1 < L.nx) v += m.cx[n] * x[n + sx];
+    if (i > 0) v += m.cx[n - sx] * x[n - sx];
+    if (j + 1 < L.ny) v += m.cy[n] * x[n + sy];
+    if (j > 0) v += m.cy[n - sy] * x[n - sy];
+    if (k + 1 < L.nz) v += m.cz[n] * x[n + 1];
+    if (k > 0) v += m.cz[n - 1] * x[n - 1];
+    return v;
+  } else {
+    double v = 0;
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj) {
+      if (j + dj < 0 || j + dj >= L.ny) continue;
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        if (i + di < 0 || i + di >= L.nx) continue;
+        const size_t row = ((size_t)(j + dj) * L.nx + (i + di)) * L.nz;
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+          if (k + dk < 0 || k + dk >= L.nz) continue;
+          v += L.a27[(size_t)(((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1)) * L.n + n] * x[row + (k + dk)];
+        }
+      }
+    }
+    return v;
+  }
+}
+
+// x = wdinv b (one Jacobi sweep from zero)
+__global__ __launch_bounds__(256) void k_mg_first(MgArgs m, int l) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[l];
+  GRID_LOOP(n, L.n) L.x[n] = L.wdinv[n] * L.b[n];
+}
+
+// dst = src + wdinv (b - A src)
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_mg_sweep(MgArgs m, int l, const double* src, double* dst) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[l];
+  GRID_LOOP(n, L.n) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    dst[n] = src[n] + L.wdinv[n] * (L.b[n] - mg_apply<FINE>(m, L, src, n, i, j, k));
+  }
+}
+
+// last sweep of level 0 into z, with the partial r.z of the CG (r = b of level 0)
+__global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, const double* src, double* z, double* part_rz) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[0];
+  double rz = 0;
+  GRID_LOOP(n, L.n) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    const double v = src[n] + L.wdinv[n] * (L.b[n] - mg_apply<true>(m, L, src, n, i, j, k));
+    z[n] = v;
+    rz += L.b[n] * v;
+  }
+  block_reduce3(rz, 0.0, 0.0, part_rz);
+}
+
+// r = b - A src
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m, int l, const double* src) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[l];
+  GRID_LOOP(n, L.n) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    L.r[n] = L.b[n] - mg_apply<FINE>(m, L, src, n, i, j, k);
+  }
+}
+
+__device__ inline double pw1(int f, int C) {
+  const int d = f - 2 * C;
+  return d == 0 ? 1.0 : 0.5;
+}
+
+// b_{l+1} = P^T r_l: coarse node I gathers the fine residuals around fine node 2I
+// (residuals of fixed fine nodes are zero, so the zeroed rows of P need no test)
+__global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
+  if (*m.done) return;
+  const MgLevelDev& F = m.L[l];
+  const MgLevelDev& C = m.L[l + 1];
+  GRID_LOOP(I, C.n) {
+    int ci, cj, ck;
+    lvl_ijk(C, I, ci, cj, ck);
+    double v = 0;
+    for (int fj = max(2 * cj - 1, 0); fj <= min(2 * cj + 1, F.ny - 1); ++fj)
+      for (int fi = max(2 * ci - 1, 0); fi <= min(2 * ci + 1, F.nx - 1); ++fi) {
+        const double w = pw1(fj, cj) * pw1(fi, ci);
+        const size_t row = ((size_t)fj * F.nx + fi) * F.nz;
+        for (int fk = max(2 * ck - 1, 0); fk <= min(2 * ck + 1, F.nz - 1); ++fk)
+          v += w * pw1(fk, ck) * F.r[row + fk];
+      }
+    C.b[I] = v;
+  }
+}
+
+// dst_l += P e_{l+1} on the unknown nodes (wdinv > 0)
+__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m, int l, double* dst, const double* e) {
+  if (*m.done) return;
+  const MgLevelDev& F = m.L[l];
+  const MgLevelDev& C = m.L[l + 1];
+  GRID_LOOP(n, F.n) {
+    if (F.wdinv[n] == 0.0) continue;
+    int i, j, k;
+    lvl_ijk(F, n, i, j, k);
+    double v = 0;
+    for (int cj = j >> 1; cj <= (j + 1) >> 1; ++cj) {
+      if (cj >= C.ny) continue;
+      for (int ci = i >> 1; ci <= (i + 1) >> 1; ++ci) {
+        if (ci >= C.nx) continue;
+        const double w = pw1(j, cj) * pw1(i, ci);
+        const size_t row = ((size_t)cj * C.nx + ci) * C.nz;
+        for (int ck = k >> 1; ck <= (k + 1) >> 1; ++ck) {
+          if (ck >= C.nz) continue;
+          v += w * pw1(k, ck) * e[row + ck];
+        }
+      }
+    }
+    dst[n] += v;
+  }
+}
+
+// coarsest level: x = C^-1 b (dense); with a single level the result is z and the
+// partial r.z is taken here
+__global__ __launch_bounds__(256) void k_mg_coarse(MgArgs m, double* out, double* part_rz) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[m.nl - 1];
+  const int nc = m.nc;
+  double rz = 0;
+  GRID_LOOP(r, (size_t)nc) {
+    const double* row = m.cinv + r * nc;
+    double v = 0;
+    for (int c = 0; c < nc; ++c) v += row[c] * L.b[c];
+    out[r] = v;
+    rz += L.b[r] * v;
+  }
+  if (part_rz) block_reduce3(rz, 0.0, 0.0, part_rz);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -474,8 +801,16 @@ hipError_t emc_launch_merge(const FlyArgs& a, unsigned long long nclones, unsign
   hipLaunchKernelGGL(k_merge, dim3(grid_for((long long)nclones, 256)), dim3(256), 0, s, a, nclones, ndead, ticket);
   return hipGetLastError();
 }
+// sort P into Q by tile; toff[ntile] is the live count afterwards
+hipError_t emc_launch_tile_sort(const SortArgs& a, hipStream_t s) {
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
 hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_deposit, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_deposit_tile, dim3(a.ntile), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_deposit_gather, dim3(grid_for((long long)a.nnode, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
@@ -492,20 +827,61 @@ hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 int emc_pcg_blocks(size_t n) { return grid_for((long long)n, 256, 1024); }
-hipError_t emc_launch_pcg_start(const PcgArgs& a, hipStream_t s) {
+
+// z = M r: one V(2,2) cycle; part_rz receives the block partials of r.z (CG grid)
+static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hipStream_t s) {
+  const int nl = m.nl;
+  if (nl == 1) {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(a.nblocks), dim3(256), 0, s, m, a.z, part_rz);
+    return;
+  }
+  for (int l = 0; l < nl - 1; ++l) {
+    const MgLevelDev& L = m.L[l];
+    const dim3 g(grid_for((long long)L.n, 256));
+    hipLaunchKernelGGL(k_mg_first, g, dim3(256), 0, s, m, l);
+    if (l == 0) {
+      hipLaunchKernelGGL(k_mg_sweep<true>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
+      hipLaunchKernelGGL(k_mg_resid<true>, g, dim3(256), 0, s, m, l, (const double*)L.t);
+    } else {
+      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
+      hipLaunchKernelGGL(k_mg_resid<false>, g, dim3(256), 0, s, m, l, (const double*)L.t);
+    }
+    hipLaunchKernelGGL(k_mg_restrict, dim3(grid_for((long long)m.L[l + 1].n, 256)), dim3(256), 0, s, m, l);
+  }
+  const MgLevelDev& Lc = m.L[nl - 1];
+  hipLaunchKernelGGL(k_mg_coarse, dim3(grid_for((long long)m.nc, 256)), dim3(256), 0, s, m, Lc.x, (double*)nullptr);
+  for (int l = nl - 2; l >= 0; --l) {
+    const MgLevelDev& L = m.L[l];
+    const double* e = (l + 1 == nl - 1) ? m.L[l + 1].x : m.L[l + 1].t;
+    const dim3 g(grid_for((long long)L.n, 256));
+    hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m, l, L.t, e);
+    if (l == 0) {
+      hipLaunchKernelGGL(k_mg_sweep<true>, g, dim3(256), 0, s, m, l, (const double*)L.t, L.x);
+      hipLaunchKernelGGL(k_mg_sweep_last, dim3(a.nblocks), dim3(256), 0, s, m, (const double*)L.x, a.z, part_rz);
+    } else {
+      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.t, L.x);
+      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
+    }
+  }
+}
+
+hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s) {
   const int g = a.nblocks;
   hipLaunchKernelGGL(k_pcg_fix, dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_resid, dim3(g), dim3(256), 0, s, a, a.part);
-  hipLaunchKernelGGL(k_pcg_norms, dim3(1), dim3(256), 0, s, a, (const double*)a.part);
+  hipLaunchKernelGGL(k_pcg_resid, dim3(g), dim3(256), 0, s, a);
+  launch_vcycle(a, m, a.part_rz0, s);
+  hipLaunchKernelGGL(k_pcg_begin, dim3(g), dim3(256), 0, s, a);
   return hipGetLastError();
 }
-// iteration `it` ping-pongs the r.z partials between part (even) and part3 (odd)
-hipError_t emc_launch_pcg_iter(const PcgArgs& a, int it, hipStream_t s) {
+
+// iteration `it`: r.z of the previous iteration is in part_rz0 (even it) or part_rz1
+hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStream_t s) {
   const int g = a.nblocks;
-  double* pa = (it & 1) ? a.part3 : a.part;
-  double* pc = (it & 1) ? a.part : a.part3;
-  hipLaunchKernelGGL(k_pcg_spmv, dim3(g), dim3(256), 0, s, a, a.part2);
-  hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(256), 0, s, a, (const double*)pa, (const double*)a.part2, pc);
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(g), dim3(256), 0, s, a, (const double*)pa, (const double*)pc);
+  double* rz_old = (it & 1) ? a.part_rz1 : a.part_rz0;
+  double* rz_new = (it & 1) ? a.part_rz0 : a.part_rz1;
+  hipLaunchKernelGGL(k_pcg_spmv, dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(256), 0, s, a, (const double*)rz_old);
+  launch_vcycle(a, m, rz_new, s);
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(g), dim3(256), 0, s, a, (const double*)rz_old, (const double*)rz_new);
   return hipGetLastError();
 }

@@ -242,6 +242,26 @@ class Engine:
                                                C.byref(it), C.byref(rr)))
         return sol, it.value, rr.value
 
+    def mg_levels(self):
+        """The Poisson preconditioner's multigrid hierarchy: list of (dims, a27 or
+        None, omega) per level; a27 has shape (27, n)."""
+        out = []
+        level = 0
+        while True:
+            dims = (C.c_int * 3)()
+            om = C.c_double(0)
+            st = self.lib.emc_poisson_mg_level(self.ctx, level, dims, None, C.byref(om))
+            if st != 0:
+                break
+            a27 = None
+            if level > 0:
+                a27 = np.zeros(27 * dims[0] * dims[1] * dims[2])
+                self._check(self.lib.emc_poisson_mg_level(self.ctx, level, dims, _ptr(a27, C.c_double), None))
+                a27 = a27.reshape(27, -1)
+            out.append((tuple(dims), a27, om.value))
+            level += 1
+        return out
+
     def compute_field(self, pot, rho):
         pot = np.ascontiguousarray(pot, np.float64)
         rho = np.ascontiguousarray(rho, np.float64)

@@ -206,13 +206,25 @@ int emc_fly_step(emc_ctx *ctx, uint64_t step, int flag_heat, emc_fly_tallies *ou
  *   node_q[nnode]       sum of q*w (EMC_FX_Q)                 particle_to_density
  *   node_stat[10][nnode] vxE vyE vzE |q|E E|q|E, then the hole five (EMC_FX_STAT)
  *   cell_q[3][ncell]    sum q electrons, sum q holes, sum |q| (EMC_FX_Q)
- *   cell_n[ncell]       particle count per cell (stat_enum)                       */
+ *   cell_n[ncell]       particle count per cell (stat_enum)
+ * The deposit first sorts the population on the device by 8x8x4-cell tile and
+ * drops empty slots, so the order of emc_particles_download changes (the sums do
+ * not: they are integer).                                                          */
 int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell_q, int64_t *cell_n);
 
 /* Poisson solve on the eliminated operator: rhs, sol of length nnode.  rhs holds
- * the reference right-hand side (compute_rhs); sol is the initial guess on entry. */
+ * the reference right-hand side (compute_rhs); sol is the initial guess on entry.
+ * CG preconditioned by one geometric multigrid V(2,2) cycle, to ||r|| <= rtol ||b||. */
 int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
                       int *iters, double *relres);
+
+/* Level `level` of the multigrid hierarchy used by emc_poisson_solve (no reference
+ * counterpart: the reference factorises with KLU).  dims[3] = nodes along i, j, k;
+ * a27 (may be NULL; levels >= 1) receives the [27][n] Galerkin coefficients, entry
+ * ((dj+1)*3 + (di+1))*3 + (dk+1) coupling node n to n + (di, dj, dk); omega the
+ * Jacobi damping.  Level 0 is the mesh view's 7-point operator.  EMC_ERR_ARG past
+ * the last level.  Works without a GPU. */
+int emc_poisson_mg_level(emc_ctx *ctx, int level, int dims[3], double *a27, double *omega);
 
 /* From a node potential: cell fields (pot2field) and surface rates, written
  * into the context's cell fields; cell_rho must hold the clamped Rho. */

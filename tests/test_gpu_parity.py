@@ -100,6 +100,14 @@ def test_deposit_parity(eng):
     live = p.i[:p.n] >= 0
     total = np.rint(p.charge[:p.n][live] * emcgpu.FX_Q).sum()
     assert abs(g[0].sum() - total) <= 8 * live.sum()
+    # the deposit sorts the population by tile: same particles, empty slots gone
+    assert not live.all(), "the flight step should have caught some particles"
+    p2 = eng.download()
+    assert p2.n == live.sum() and np.all(p2.i[:p2.n] >= 0)
+    assert_same_population(p2, p.subset(np.nonzero(live)[0]), "tile sort")
+    tile = ((p2.j[:p2.n] // 8) * ((eng.mesh.cnx + 7) // 8) + p2.i[:p2.n] // 8) * ((eng.mesh.cnz + 3) // 4) + \
+        p2.k[:p2.n] // 4
+    assert np.all(np.diff(tile) >= 0), "population not in tile order"
 
 
 def test_rhs_and_field_parity(eng):
