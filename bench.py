@@ -119,13 +119,16 @@ def main():
     phase = np.zeros(5)
     pcg_it = []
     n_ps = 0
+    full_loop = events = 0
     t0 = time.perf_counter()
     for s in range(a.steps):
         n_ps += eng.count()
-        eng.step(1)
+        t = eng.step(1)
         ms, it = eng.timing()
         phase += ms
         pcg_it.append(it)
+        full_loop += t.n_full_loop
+        events += t.n_events
     t1 = time.perf_counter()
     elapsed = t1 - t0
     phase /= max(1, a.steps)
@@ -162,7 +165,9 @@ def main():
                    "particles": int(n_start), "nodes": int(eng.mesh.nnode),
                    "phase_ms": {"deposit_rhs": phase[0], "poisson": phase[1], "field": phase[2],
                                 "flight_total": phase[3], "k_fly": phase[4]},
-                   "pcg_iters_mean": float(np.mean(pcg_it))},
+                   "pcg_iters_mean": float(np.mean(pcg_it)),
+                   "full_event_loop_fraction": full_loop / max(1, n_ps),
+                   "events_per_particle_step": events / max(1, n_ps)},
         "roofline": {"bound": "hbm", "kernel": "k_fly", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": None,

@@ -15,7 +15,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define EMC_HD __host__ __device__
+// forced inlining keeps a struct whose member functions are called in registers
+// instead of in per-lane scratch memory
+#define EMC_INL __attribute__((always_inline))
 #else
+#define EMC_INL
 #include <cmath>
 using std::fabs;
 using std::rint;

@@ -110,6 +110,7 @@ struct Gpu {
   const double* da = nullptr;
   const int* eeff = nullptr;
   int pcg_blocks = 0;
+  long long last_full_loop = 0;       // particles of the last flight that took the full event loop
   hipEvent_t ev[8] = {};
 };
 
@@ -130,6 +131,8 @@ struct emc_ctx {
   Device dev;
   Band band;
   std::vector<TetRec> tetrec;
+  std::vector<SurfRec> surfrec;
+  std::vector<CornerRec> cornerrec;
   std::vector<int> surface_dir;
   HostParticles hp;
   std::unique_ptr<Gpu> gpu;
@@ -156,6 +159,8 @@ static int fail(emc_ctx* c, int code, const std::string& msg) {
 static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
   const Band& b = c->band;
   B.tr = c->tetrec.data();
+  B.sr = c->surfrec.data();
+  B.cr = c->cornerrec.data();
   B.tet = b.tet.data();
   B.xkk = b.xkk.data(); B.ykk = b.ykk.data(); B.zkk = b.zkk.data(); B.eek = b.eek.data();
   B.vgt = b.vgt.data(); B.faclow = b.faclow.data(); B.facmed = b.facmedium.data(); B.fachigh = b.fachigh.data();
@@ -201,6 +206,18 @@ static void build_tetrec(emc_ctx* c) {
     for (int f = 0; f < 4; ++f) r.neib[f] = b.neib[(size_t)f * b.nt + it];
     r.ibt = b.ibt[it];
   }
+  c->surfrec.assign(b.nt, SurfRec{});
+  c->cornerrec.assign(b.nt, CornerRec{});
+  for (int it = 0; it < b.nt; ++it) {
+    SurfRec& s = c->surfrec[it];
+    CornerRec& q = c->cornerrec[it];
+    for (int k = 0; k < 4; ++k) {
+      const int ck = b.tet_corner(k, it);
+      s.e[k] = q.e[k] = b.eek[ck];
+      q.x[k] = b.xkk[ck]; q.y[k] = b.ykk[ck]; q.z[k] = b.zkk[ck];
+    }
+    s.faclow = b.faclow[it]; s.facmed = b.facmedium[it]; s.fachigh = b.fachigh[it]; s.vgt = b.vgt[it];
+  }
 }
 
 // Multigrid hierarchy of the Poisson preconditioner, built once per context.
@@ -227,6 +244,8 @@ static int gpu_setup(emc_ctx* c) {
   BandDev& B = g.B;
   fill_band_dev_host(c, B);  // scalars; pointers replaced below
   B.tr = m.dup(c->tetrec);
+  B.sr = m.dup(c->surfrec);
+  B.cr = m.dup(c->cornerrec);
   B.tet = m.dup(b.tet);
   B.xkk = m.dup(b.xkk); B.ykk = m.dup(b.ykk); B.zkk = m.dup(b.zkk); B.eek = m.dup(b.eek);
   B.vgt = m.dup(b.vgt); B.faclow = m.dup(b.faclow); B.facmed = m.dup(b.facmedium); B.fachigh = m.dup(b.fachigh);
@@ -301,15 +320,25 @@ static int gpu_setup(emc_ctx* c) {
     A.cinv = m.dup(h.cinv);
     A.diag = g.diag; A.cx = g.cx; A.cy = g.cy; A.cz = g.cz;
     A.done = g.done;
+    A.lb = A.nl;
     for (int l = 0; l < A.nl; ++l) {
       const MgLevel& L = h.lv[l];
       MgLevelDev& D = A.L[l];
       D.nx = L.nx; D.ny = L.ny; D.nz = L.nz; D.n = L.n;
-      D.a27 = l > 0 ? m.dup(L.a27) : nullptr;
+      D.c14 = nullptr;
+      if (l > 0) {
+        // upper half of the symmetric stencil (offsets 13..26 of st27) in FP32
+        std::vector<float> c14(14 * L.n);
+        for (int o = 13; o < 27; ++o)
+          for (size_t q = 0; q < L.n; ++q) c14[(size_t)(o - 13) * L.n + q] = (float)L.a27[(size_t)o * L.n + q];
+        D.c14 = m.dup(c14);
+        if (A.lb == A.nl && L.n <= (size_t)MG_BOTTOM_NODES) A.lb = l;
+      }
       D.wdinv = m.dup(L.wdinv);
       D.x = m.alloc<double>(L.n); D.t = m.alloc<double>(L.n); D.r = m.alloc<double>(L.n);
       D.b = l > 0 ? m.alloc<double>(L.n) : g.r;
     }
+    if (A.nl > 1 && A.lb == A.nl) A.lb = A.nl - 1;
   }
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
@@ -727,6 +756,7 @@ static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
   unsigned long long h[4];
   HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
+  g.last_full_loop = (long long)h[3];
   unsigned long long begin = 0, end = std::min(h[0], g.clone_cap);
   int passes = 0;
   while (end > begin) {
@@ -765,6 +795,7 @@ static void read_tallies(emc_ctx* c, emc_fly_tallies* out, int passes) {
   out->n_impurity += (int64_t)cn[CNT_IMPURITY];
   out->n_events += (int64_t)cn[CNT_EVENTS];
   out->n_clone_passes += passes;
+  out->n_full_loop += g.last_full_loop;
 }
 
 static int zero_tallies(emc_ctx* c) {

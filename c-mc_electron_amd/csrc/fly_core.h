@@ -62,55 +62,56 @@ struct Flight {
   EMC_HD Flight(const BandDev& b, const MeshDev& m, const FieldsDev& f, const TallyDev& t)
       : B(b), M(m), F(f), T(t), clones(nullptr), nclones(nullptr), clone_cap(0) {}
 
-  EMC_HD double Random() { return rng.next(); }
-  EMC_HD int ms(int r, int s) const { return B.matsym[r * 48 + s]; }
-  EMC_HD int indmat(int sx, int sy, int sz, int p0, int p1, int p2) const {
+  EMC_HD EMC_INL double Random() { return rng.next(); }
+  EMC_HD EMC_INL int ms(int r, int s) const { return B.matsym[r * 48 + s]; }
+  EMC_HD EMC_INL int indmat(int sx, int sy, int sz, int p0, int p1, int p2) const {
     return B.indmat[((((sx + 1) * 3 + (sy + 1)) * 3 + (sz + 1)) * 3 + p0) * 9 + p1 * 3 + p2];
   }
-  EMC_HD size_t cell(int i, int j, int k) const { return ((size_t)j * M.cnx + i) * M.cnz + k; }
-  EMC_HD size_t node(int i, int j, int k) const { return ((size_t)j * M.pnx + i) * M.pnz + k; }
-  EMC_HD int corner(int c, int it) const { return B.tet[(size_t)c * B.nt + it]; }
-  EMC_HD void count(int slot) { tally_add(&T.counters[slot], 1); }
+  EMC_HD EMC_INL size_t cell(int i, int j, int k) const { return ((size_t)j * M.cnx + i) * M.cnz + k; }
+  EMC_HD EMC_INL size_t node(int i, int j, int k) const { return ((size_t)j * M.pnx + i) * M.pnz + k; }
+  EMC_HD EMC_INL int corner(int c, int it) const { return B.tet[(size_t)c * B.nt + it]; }
+  EMC_HD EMC_INL void count(int slot) { tally_add(&T.counters[slot], 1); }
 
   // in_wedge / OutWedge (src/ParticleUtils.cpp:293-305, src/Band.cpp:555-570)
-  EMC_HD void in_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
+  EMC_HD EMC_INL void in_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
     double v[3];
     v[ms(0, isym) - 1] = xi * ms(3, isym);
     v[ms(1, isym) - 1] = yi * ms(4, isym);
     v[ms(2, isym) - 1] = zi * ms(5, isym);
     xo = v[0]; yo = v[1]; zo = v[2];
   }
-  EMC_HD void out_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
+  EMC_HD EMC_INL void out_wedge(double& xo, double& yo, double& zo, double xi, double yi, double zi) const {
     const double v[3] = {xi, yi, zi};
     xo = v[ms(0, isym) - 1] * ms(3, isym);
     yo = v[ms(1, isym) - 1] * ms(4, isym);
     zo = v[ms(2, isym) - 1] * ms(5, isym);
   }
-  EMC_HD void GetV() {
+  EMC_HD EMC_INL void GetV() {
     const TetRec& r = B.tr[itet];
     out_wedge(vx, vy, vz, r.vt[0], r.vt[1], r.vt[2]);
   }
-  EMC_HD void GetEnergy() {
+  EMC_HD EMC_INL void GetEnergy() {
     double dx, dy, dz;
     in_wedge(dx, dy, dz, kx, ky, kz);
-    const int c0 = corner(0, itet);
-    dx -= B.xkk[c0]; dy -= B.ykk[c0]; dz -= B.zkk[c0];
+    const CornerRec& q = B.cr[itet];
+    dx -= q.x[0]; dy -= q.y[0]; dz -= q.z[0];
     const TetRec& r = B.tr[itet];
     energy = r.vt[3] + r.vt[0] * dx + r.vt[1] * dy + r.vt[2] * dz;
   }
 
   // ---- Band interpolators and areas (src/Band.cpp:194-352, 665-703, 1999-2321)
-  EMC_HD static double fii(double x1, double y1, double z1, double x2, double y2, double z2, double x3,
+  EMC_HD EMC_INL static double fii(double x1, double y1, double z1, double x2, double y2, double z2, double x3,
                            double y3, double z3) {
     const double sbx = (y2 - y1) * (z3 - z1) - (y3 - y1) * (z2 - z1);
     const double sby = (x3 - x1) * (z2 - z1) - (x2 - x1) * (z3 - z1);
     const double sbz = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
     return 0.50 * sqrt(sbx * sbx + sby * sby + sbz * sbz);
   }
-  EMC_HD double surf(double e, int it) const {
-    const int c0 = corner(0, it), c1 = corner(1, it), c2 = corner(2, it), c3 = corner(3, it);
-    const double *X = B.xkk, *Y = B.ykk, *Z = B.zkk;
-    const double e0 = B.eek[c0], e1 = B.eek[c1], e2 = B.eek[c2], e3 = B.eek[c3];
+  EMC_HD EMC_INL double surf(double e, int it) const {
+    const CornerRec& q = B.cr[it];
+    const int c0 = 0, c1 = 1, c2 = 2, c3 = 3;
+    const double *X = q.x, *Y = q.y, *Z = q.z;
+    const double e0 = q.e[0], e1 = q.e[1], e2 = q.e[2], e3 = q.e[3];
     double r;
     if (!(e > e0 && e < e3)) return 0.0;
     if (e <= e1) {
@@ -148,20 +149,20 @@ struct Flight {
     }
     return 0.0;
   }
-  EMC_HD double fastsurf(double e, int it) {
-    const double e0 = B.eek[corner(0, it)], e1 = B.eek[corner(1, it)], e2 = B.eek[corner(2, it)],
-                 e3 = B.eek[corner(3, it)];
+  EMC_HD EMC_INL double fastsurf(double e, int it) { return fastsurf_r(e, it, B.sr[it]); }
+  EMC_HD EMC_INL double fastsurf_r(double e, int it, const SurfRec& s) {
+    const double e0 = s.e[0], e1 = s.e[1], e2 = s.e[2], e3 = s.e[3];
     double r = 0.0;
     if (e > e0 && e < e3) {
       if (e <= e1)
-        r = B.faclow[it] * (e - e0) * (e - e0);
+        r = s.faclow * (e - e0) * (e - e0);
       else if (e < e2) {
         if (e1 - e0 > 1e-10)
-          r = B.faclow[it] * (e - e0) * (e - e0) - B.facmed[it] * (e - e1) * (e - e1);
+          r = s.faclow * (e - e0) * (e - e0) - s.facmed * (e - e1) * (e - e1);
         else
           r = surf(e, it);
       } else
-        r = B.fachigh[it] * (e - e3) * (e - e3);
+        r = s.fachigh * (e - e3) * (e - e3);
     }
     if (r < 0.0) {
       if (T.counters) count(CNT_DROP0 + 6);
@@ -169,7 +170,7 @@ struct Flight {
     }
     return r;
   }
-  EMC_HD double calscattsum(double e, int ib) const {
+  EMC_HD EMC_INL double calscattsum(double e, int ib) const {
     if (e <= 0.0) return B.sumscatt[ib];
     if (e >= B.emax) return B.sumscatt[(size_t)MTAB * MNB + ib];
     const int itab = (int)(e / B.dtable);
@@ -177,14 +178,14 @@ struct Flight {
     return B.sumscatt[(size_t)itab * MNB + ib] +
            intp * (B.sumscatt[(size_t)(itab + 1) * MNB + ib] - B.sumscatt[(size_t)itab * MNB + ib]);
   }
-  EMC_HD double caldostetmax(double e, int ib) const {
+  EMC_HD EMC_INL double caldostetmax(double e, int ib) const {
     const double* d = &B.dostetmax[(size_t)ib * NTAB1];
     if (e <= 0.0) return d[0];
     if (e >= B.emax) return d[MTAB];
     const int itab = (int)(e / B.dtable);
     return d[itab] > d[itab + 1] ? d[itab] : d[itab + 1];
   }
-  EMC_HD void confbz(double& a, double& b, double& c) const {
+  EMC_HD EMC_INL void confbz(double& a, double& b, double& c) const {
     const double q = B.a0pi;
     for (int guard = 0; guard < 1024; ++guard) {  // a finite k folds in a few passes
       const bool out = fabs(a) > q || fabs(b) > q || fabs(c) > q || fabs(a) + fabs(b) + fabs(c) > 1.50 * q;
@@ -196,7 +197,7 @@ struct Flight {
   }
 
   // ---- tetrahedron search (src/ParticleUtils.cpp:331-625)
-  EMC_HD bool contains(int it, double a, double b, double c) const {
+  EMC_HD EMC_INL bool contains(int it, double a, double b, double c) const {
     const TetRec& r = B.tr[it];
     const double d1 = r.n[0][0] * a + r.n[0][1] * b + r.n[0][2] * c - r.n[0][3];
     const double d2 = r.n[1][0] * a + r.n[1][1] * b + r.n[1][2] * c - r.n[1][3];
@@ -204,23 +205,25 @@ struct Flight {
     const double d4 = r.n[3][0] * a + r.n[3][1] * b + r.n[3][2] * c - r.n[3][3];
     return (d1 >= -MY_ZERO) && (d2 >= -MY_ZERO) && (d3 >= -MY_ZERO) && (d4 >= -MY_ZERO);
   }
-  EMC_HD bool scan(const int* list, int p0, int nn, double a, double b, double c) {
+  EMC_HD EMC_INL bool scan(const int* list, int p0, int nn, double a, double b, double c) {
     for (int itl = p0; itl <= p0 + nn - 1; ++itl) {
       const int it = list[itl];
       if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return true; }
     }
     return false;
   }
-  EMC_HD void Getisym() {
-    int pos[3] = {1, 1, 1}, mink, maxk;
+  EMC_HD EMC_INL void Getisym() {
+    int mink, maxk;
     double tmin, tmax;
     if (fabs(kx) > fabs(ky)) { tmin = fabs(ky); tmax = fabs(kx); mink = 1; maxk = 0; }
     else { tmin = fabs(kx); tmax = fabs(ky); mink = 0; maxk = 1; }
     if (fabs(kz) < tmin) mink = 2;
     else if (fabs(kz) > tmax) maxk = 2;
-    pos[mink] = 2;
-    pos[maxk] = 0;
-    isym = indmat(kx < 0 ? -1 : 1, ky < 0 ? -1 : 1, kz < 0 ? -1 : 1, pos[0], pos[1], pos[2]);
+    // position of each axis in the sorted |k| order: 0 largest, 1 middle, 2 smallest
+    const int p0 = maxk == 0 ? 0 : (mink == 0 ? 2 : 1);
+    const int p1 = maxk == 1 ? 0 : (mink == 1 ? 2 : 1);
+    const int p2 = maxk == 2 ? 0 : (mink == 2 ? 2 : 1);
+    isym = indmat(kx < 0 ? -1 : 1, ky < 0 ? -1 : 1, kz < 0 ? -1 : 1, p0, p1, p2);
     double a, b, c;
     in_wedge(a, b, c, kx, ky, kz);
     if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
@@ -241,7 +244,7 @@ struct Flight {
     drop_reason = EMC_DROP_NO_TET_;
   }
   static constexpr int EMC_DROP_NO_TET_ = 2;
-  EMC_HD void GetisymBZ() {
+  EMC_HD EMC_INL void GetisymBZ() {
     double a, b, c;
     in_wedge(a, b, c, kx, ky, kz);
     if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
@@ -253,10 +256,10 @@ struct Flight {
   }
 
   // ---- final state on the iso-energy surface (src/ParticleUtils.cpp:48-284)
-  EMC_HD void getK() {
-    const int it = itet;
-    const int c0 = corner(0, it), c1 = corner(1, it), c2 = corner(2, it), c3 = corner(3, it);
-    const double *X = B.xkk, *Y = B.ykk, *Z = B.zkk, *E = B.eek;
+  EMC_HD EMC_INL void getK() {
+    const CornerRec& q = B.cr[itet];
+    const double *X = q.x, *Y = q.y, *Z = q.z, *E = q.e;
+    const int c0 = 0, c1 = 1, c2 = 2, c3 = 3;
     const double rv1 = 1.0 - sqrt(Random());
     const double rv2 = Random() * (1.0 - rv1);
     if ((energy < E[c0]) || (energy > E[c3])) energy = E[c0] + (E[c3] - E[c0]) * Random();
@@ -308,7 +311,7 @@ struct Flight {
     out_wedge(kx, ky, kz, xkl, ykl, zkl);
   }
 
-  EMC_HD void get_state() {
+  EMC_HD EMC_INL void get_state() {
     const double e = energy;
     if (e <= 0.0) { Flag_SelfScatter = 1; return; }
     const double maxdostet = caldostetmax(e, iband);
@@ -326,8 +329,9 @@ struct Flight {
     for (;;) {
       itet = list[(int)(nted * Random()) + base];
       count2++;
-      if ((count2 < 10000) && !((B.tr[itet].vt[3] < e) && (B.tr[itet].e3 > e))) continue;
-      const double dostet = fastsurf(e, itet) / B.vgt[itet];
+      const SurfRec& sr = B.sr[itet];
+      if ((count2 < 10000) && !((sr.e[0] < e) && (sr.e[3] > e))) continue;
+      const double dostet = fastsurf_r(e, itet, sr) / sr.vgt;
       if (((Random() * maxdostet) > dostet) && (count2 < 10000)) continue;
       break;
     }
@@ -336,7 +340,7 @@ struct Flight {
   }
 
   // TALIND (src/ParticleUtils.cpp:626-748)
-  EMC_HD void TALIND(int ityp) {
+  EMC_HD EMC_INL void TALIND(int ityp) {
     int ital = 1;
     const double ax = fabs(kx), ay = fabs(ky), az = fabs(kz);
     if ((ax >= ay) && (ax >= az)) {
@@ -355,7 +359,7 @@ struct Flight {
   }
 
   // heat_to_point (src/heat.cpp:169-196)
-  EMC_HD void heat_to_point(double e, unsigned long long* heat) {
+  EMC_HD EMC_INL void heat_to_point(double e, unsigned long long* heat) {
     if (!heat) return;
     const int i = icell, j = jcell, k = kcell;
     const double rx = (x - M.lx[i]) / M.dx[i];
@@ -378,11 +382,11 @@ struct Flight {
     double e, kx, ky, kz;
     int iband, itet, isym;
   };
-  EMC_HD Saved save() const { return Saved{energy, kx, ky, kz, iband, itet, isym}; }
-  EMC_HD void restore(const Saved& s) {
+  EMC_HD EMC_INL Saved save() const { return Saved{energy, kx, ky, kz, iband, itet, isym}; }
+  EMC_HD EMC_INL void restore(const Saved& s) {
     energy = s.e; kx = s.kx; ky = s.ky; kz = s.kz; iband = s.iband; itet = s.itet; isym = s.isym;
   }
-  EMC_HD void EPSCAT(int iscat) {
+  EMC_HD EMC_INL void EPSCAT(int iscat) {
     const Saved s0 = save();
     iband = iscat / B.scpre + B.bandofE;
     const int p = iscat % B.scpre;
@@ -397,7 +401,7 @@ struct Flight {
     if (Flag_SelfScatter) restore(s0);
     if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_e);
   }
-  EMC_HD void SCATII(int iscat, int nproc) {
+  EMC_HD EMC_INL void SCATII(int iscat, int nproc) {
     const Saved s0 = save();
     iband = iscat / nproc + (par_type == PELEC ? B.bandofE : B.bandofH);
     energy = (energy - B.sieg) / 3.0;
@@ -405,7 +409,7 @@ struct Flight {
     get_state();
     if (Flag_SelfScatter) restore(s0);
   }
-  EMC_HD void HPSCAT(int iscat) {
+  EMC_HD EMC_INL void HPSCAT(int iscat) {
     const Saved s0 = save();
     iband = iscat / B.scprh + B.bandofH;
     const int p = iscat % B.scprh;
@@ -417,7 +421,7 @@ struct Flight {
     if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_h);
   }
   // rate of process iscat (CALSCATTE / CALSCATTH element, src/Band.cpp:2098-2231)
-  EMC_HD double process_rate(bool elec, int iscat, int itab, double intp, int where) const {
+  EMC_HD EMC_INL double process_rate(bool elec, int iscat, int itab, double intp, int where) const {
     const int np = elec ? B.scpre : B.scprh;
     const int is = iscat % np, f = iscat / np;
     const int ini = iband - (elec ? B.bandofE : B.bandofH);
@@ -438,7 +442,7 @@ struct Flight {
     if (where > 0) return drow[MTAB];
     return drow[itab] + intp * (drow[itab + 1] - drow[itab]);
   }
-  EMC_HD void select_process(double drand) {
+  EMC_HD EMC_INL void select_process(double drand) {
     const bool elec = par_type == PELEC;
     const int np = elec ? B.scpre : B.scprh;
     const int nmax = np * (elec ? B.nbandE : B.nbandH);
@@ -463,7 +467,7 @@ struct Flight {
     else if (elec) EPSCAT(iscat);
     else HPSCAT(iscat);
   }
-  EMC_HD void PhononScatter() {
+  EMC_HD EMC_INL void PhononScatter() {
     const double scasum = calscattsum(energy, iband);
     const double g = B.tr[itet].gam;
     const double drand = scasum > g ? Random() * scasum : Random() * g;
@@ -473,7 +477,7 @@ struct Flight {
   }
 
   // ---- ionised impurities (src/ElectronImpurityScatter.cpp:6-306)
-  EMC_HD double GetImpScRate(double eel) const {
+  EMC_HD EMC_INL double GetImpScRate(double eel) const {
     if ((eel * M.eV0 < 0.120) && (DA * M.conc0 >= 1e22)) {
       const double fr = M.frickel;
       const double eebeta = fr * 4 * PI * Rho / (2.0 * B.meld * M.eps_si);
@@ -483,7 +487,7 @@ struct Flight {
     }
     return 0.0;
   }
-  EMC_HD double GetImpScGamma() const {
+  EMC_HD EMC_INL double GetImpScGamma() const {
     const double e0 = B.tr[itet].vt[3], e3 = B.tr[itet].e3;
     const double eemax = M.frickel * 4 * PI * Rho / (8.0 * B.melt * M.eps_si);
     double gammax;
@@ -493,7 +497,7 @@ struct Flight {
     const double lo = 1.0 / M.scrt0;
     return gammax > lo ? gammax : lo;
   }
-  EMC_HD void EleImpScUpdateK() {
+  EMC_HD EMC_INL void EleImpScUpdateK() {
     double xkl, ykl, zkl;
     in_wedge(xkl, ykl, zkl, kx, ky, kz);
     xkl = xkl - 0.85 * B.a0pi;
@@ -533,26 +537,26 @@ struct Flight {
     } else
       Flag_SelfScatter = 1;
   }
-  EMC_HD void ElectronImpurityScatter() {
+  EMC_HD EMC_INL void ElectronImpurityScatter() {
     const double gamimp = GetImpScRate(energy);
     if (gamimp < Random() * ImpScGamma) Flag_SelfScatter = 1;
     else EleImpScUpdateK();
   }
 
   // ---- boundaries (src/CellHit.cpp)
-  EMC_HD void mirror(int axis) {
+  EMC_HD EMC_INL void mirror(int axis) {
     int a = ms(3, isym), b = ms(4, isym), c = ms(5, isym);
     if (axis == 0) { kx = -kx; a = -a; }
     else if (axis == 1) { ky = -ky; b = -b; }
     else { kz = -kz; c = -c; }
     isym = indmat(a, b, c, ms(0, isym) - 1, ms(1, isym) - 1, ms(2, isym) - 1);
   }
-  EMC_HD void Reflect() {
+  EMC_HD EMC_INL void Reflect() {
     if (idir == UP || idir == DOWN) mirror(0);
     else if (idir == RIGHT || idir == LEFT) mirror(1);
     else if (idir == FRONT || idir == BACK) mirror(2);
   }
-  EMC_HD void Diffuse() {
+  EMC_HD EMC_INL void Diffuse() {
     const Saved s0 = save();
     TALIND(1);
     get_state();
@@ -573,7 +577,7 @@ struct Flight {
     if (v == 0.0) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
     if (flip) mirror(axis);
   }
-  EMC_HD bool neighbour(int& ii, int& jj, int& kk) const {
+  EMC_HD EMC_INL bool neighbour(int& ii, int& jj, int& kk) const {
     ii = icell; jj = jcell; kk = kcell;
     switch (idir) {
       case UP: --ii; break;
@@ -585,12 +589,12 @@ struct Flight {
     }
     return ii >= 0 && ii < M.cnx && jj >= 0 && jj < M.cny && kk >= 0 && kk < M.cnz;
   }
-  EMC_HD void contact_tally(int icont, int col, long long v) {
+  EMC_HD EMC_INL void contact_tally(int icont, int col, long long v) {
     if (icont < 0 || icont >= EMC_MAX_CONTACT_ || icont >= M.ncontact) { count(CNT_DROP0 + 7); return; }
     tally_add(&T.contact[icont * 5 + col], v);
   }
   static constexpr int EMC_MAX_CONTACT_ = 16;
-  EMC_HD void Generate() {
+  EMC_HD EMC_INL void Generate() {
     count(CNT_GEN);
     const int icont = M.attached[cell(icell, jcell, kcell)] - 1;
     if (icont < 0) count(CNT_DROP0 + 7);
@@ -617,7 +621,7 @@ struct Flight {
     c.pk = pk_make(par_type, ii, jj, kk);
     c.itet = itet; c.isym = isym; c.pad = 0;
   }
-  EMC_HD void Catch(bool gate) {
+  EMC_HD EMC_INL void Catch(bool gate) {
     int ii, jj, kk, icont = -1;
     if (neighbour(ii, jj, kk)) icont = M.attached[cell(ii, jj, kk)] - 1;
     if (icont < 0) count(CNT_DROP0 + 7);
@@ -631,7 +635,7 @@ struct Flight {
       contact_tally(icont, 2, fxd(energy, EMC_FX_CONTACT_));  // the reference adds to EnergyGen
     }
   }
-  EMC_HD void load_cell() {
+  EMC_HD EMC_INL void load_cell() {
     const size_t c = cell(icell, jcell, kcell);
     Rho = F.rho[c];
     DA = M.da[c];
@@ -639,7 +643,7 @@ struct Flight {
     if (par_type == 0) { Ex = sg * F.ex[c]; Ey = sg * F.ey[c]; Ez = sg * F.ez[c]; }
     else { Ex = sg * F.hx[c]; Ey = sg * F.hy[c]; Ez = sg * F.hz[c]; }
   }
-  EMC_HD bool HitCell() {
+  EMC_HD EMC_INL bool HitCell() {
     const int imot = M.motion[cell(icell, jcell, kcell) * 6 + idir];
     int ii, jj, kk;
     const bool inside = neighbour(ii, jj, kk);
@@ -691,7 +695,7 @@ struct Flight {
   }
 
   // TetTime (src/TetHit.cpp:4-106)
-  EMC_HD double TetTime() {
+  EMC_HD EMC_INL double TetTime() {
     double xkl, ykl, zkl, xfl, yfl, zfl;
     in_wedge(xkl, ykl, zkl, kx, ky, kz);
     in_wedge(xfl, yfl, zfl, Ex, Ey, Ez);
@@ -710,7 +714,7 @@ struct Flight {
     return best;
   }
   // HitTet (src/TetHit.cpp:109-142)
-  EMC_HD void HitTet() {
+  EMC_HD EMC_INL void HitTet() {
     const int nb = B.tr[itet].neib[itetdir];
     if (nb >= 0) itet = nb;
     else if (nb == -5 || nb == -6) {
@@ -726,7 +730,7 @@ struct Flight {
       isym = B.newisym[(-nb - 2) * 48 + isym];
   }
   // CellTime (src/CellHit.cpp:4-93)
-  EMC_HD double CellTime() {
+  EMC_HD EMC_INL double CellTime() {
     int ix, iy, iz;
     double tx, ty, tz, t;
     if (vx > 0) { ix = DOWN; tx = (M.lx[icell + 1] - x) / vx; }
@@ -751,7 +755,7 @@ struct Flight {
     double ssnl, imprnl, phrnl, TetTf, CellTf;
     int loop;
   };
-  EMC_HD void fly_begin(Loop& L) {
+  EMC_HD EMC_INL void fly_begin(Loop& L) {
     L.getTet = L.getCell = L.getPh = L.getImp = L.getSurf = true;
     L.ssnl = L.imprnl = L.phrnl = 0;
     L.TetTf = L.CellTf = 0;
@@ -763,7 +767,7 @@ struct Flight {
     iband = B.tr[itet].ibt;
   }
   // candidate times of the current iteration; returns the Min(...) flag and Tf
-  EMC_HD int next_event(Loop& L, double& Tf, double& gam, size_t& c) {
+  EMC_HD EMC_INL int next_event(Loop& L, double& Tf, double& gam, size_t& c) {
     if (L.getTet) { L.TetTf = TetTime(); L.getTet = false; }
     if (L.getCell) { L.CellTf = CellTime(); L.getCell = false; }
     if (L.getPh) { L.phrnl = -dm_log(Random()); L.getPh = false; }
@@ -799,7 +803,7 @@ struct Flight {
     return flag;
   }
   // free flight for Tf (src/mcmodel.cpp:2009-2046)
-  EMC_HD void advance(Loop& L, double Tf, double gam) {
+  EMC_HD EMC_INL void advance(Loop& L, double Tf, double gam) {
     L.TetTf -= Tf; L.CellTf -= Tf;
     left_time -= Tf;
     L.phrnl -= gam * Tf;
@@ -810,7 +814,7 @@ struct Flight {
     x += vx * Tf; y += vy * Tf; z += vz * Tf;
   }
 
-  EMC_HD void fly() {
+  EMC_HD EMC_INL void fly() {
     Loop L;
     fly_begin(L);
     while (left_time > 0) {
@@ -865,7 +869,7 @@ struct Flight {
   // The common case of a step: the first event is the end of the time step.  Runs
   // exactly the first loop iteration of fly(); returns false (state untouched in
   // memory) when anything else happens first, so fly() can redo the particle.
-  EMC_HD bool fly_fast() {
+  EMC_HD EMC_INL bool fly_fast() {
     Loop L;
     fly_begin(L);
     GetV();

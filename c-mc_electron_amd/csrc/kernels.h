@@ -99,11 +99,17 @@ struct PcgArgs {
 };
 
 constexpr int MG_MAXL = 10;
+// levels with at most this many nodes run inside one workgroup (k_mg_bottom)
+constexpr int MG_BOTTOM_NODES = 2048;
 
 struct MgLevelDev {
   int nx, ny, nz, pad;
   size_t n;
-  const double* a27;     // [27][n], levels >= 1
+  // levels >= 1: upper half of the symmetric 27-point Galerkin stencil in FP32,
+  // [14][n]: entry 0 is the diagonal, entry o-13 the coupling to offset o of
+  // st27 for o = 14..26; the coupling to offset o < 13 is entry 13-o of the
+  // neighbour.  The preconditioner only needs an SPD operator close to A_l.
+  const float* c14;
   const double* wdinv;   // [n] omega/diag, 0 at fixed nodes
   double *x, *t, *r, *b; // work vectors; b of level 0 is the CG residual
 };
@@ -112,6 +118,8 @@ struct MgLevelDev {
 struct MgArgs {
   MgLevelDev L[MG_MAXL];
   int nl, nc;                          // levels, nodes of the coarsest level
+  int lb;                              // first level handled by k_mg_bottom (nl: none)
+  int pad;
   const double* cinv;                  // [nc*nc] coarsest inverse, row major
   const double *diag, *cx, *cy, *cz;   // level-0 operator
   const int* done;

@@ -609,6 +609,18 @@ __global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, const double* rz
 }
 
 // ------------------------------------------------------------------ multigrid V-cycle
+// The phases of the cycle are written once per node and run either over the whole
+// grid (one launch per phase, large levels) or inside k_mg_bottom, one workgroup
+// that walks all small levels with barriers between phases.
+struct GridLoop {
+  __device__ static size_t start() { return (size_t)blockIdx.x * blockDim.x + threadIdx.x; }
+  __device__ static size_t stride() { return (size_t)gridDim.x * blockDim.x; }
+};
+struct BlockLoop {
+  __device__ static size_t start() { return threadIdx.x; }
+  __device__ static size_t stride() { return blockDim.x; }
+};
+
 __device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, int& k) {
   k = (int)(n % L.nz);
   const size_t t = n / L.nz;
@@ -616,21 +628,22 @@ __device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, in
   j = (int)(t / L.nx);
 }
 
-// (A_l x)(n); level 0 is the 7-point operator, coarser levels the 27-point Galerkin one
-template <bool FINE>
-__device__ inline double mg_apply(const MgArgs& m, const MgLevelDev& L, const double* __restrict__ x, size_t n, int i,
-                                  int j, int k) {
+// (A_l x)(n) with x given by an accessor; level 0 is the 7-point operator in FP64,
+// coarser levels the symmetric half stencil
+template <bool FINE, class XA>
+__device__ inline double mg_Ax(const MgArgs& m, const MgLevelDev& L, const XA& xa, size_t n, int i, int j, int k) {
   if (FINE) {
     const size_t sx = L.nz, sy = (size_t)L.nx * L.nz;
-    double v = m.diag[n] * x[n];
-    if (i + 1 < L.nx) v += m.cx[n] * x[n + sx];
-    if (i > 0) v += m.cx[n - sx] * x[n - sx];
-    if (j + 1 < L.ny) v += m.cy[n] * x[n + sy];
-    if (j > 0) v += m.cy[n - sy] * x[n - sy];
-    if (k + 1 < L.nz) v += m.cz[n] * x[n + 1];
-    if (k > 0) v += m.cz[n - 1] * x[n - 1];
+    double v = m.diag[n] * xa(n);
+    if (i + 1 < L.nx) v += m.cx[n] * xa(n + sx);
+    if (i > 0) v += m.cx[n - sx] * xa(n - sx);
+    if (j + 1 < L.ny) v += m.cy[n] * xa(n + sy);
+    if (j > 0) v += m.cy[n - sy] * xa(n - sy);
+    if (k + 1 < L.nz) v += m.cz[n] * xa(n + 1);
+    if (k > 0) v += m.cz[n - 1] * xa(n - 1);
     return v;
   } else {
+    const long long sx = L.nz, sy = (long long)L.nx * L.nz;
     double v = 0;
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj) {
@@ -638,11 +651,13 @@ __device__ inline double mg_apply(const MgArgs& m, const MgLevelDev& L, const do
 #pragma unroll
       for (int di = -1; di <= 1; ++di) {
         if (i + di < 0 || i + di >= L.nx) continue;
-        const size_t row = ((size_t)(j + dj) * L.nx + (i + di)) * L.nz;
 #pragma unroll
         for (int dk = -1; dk <= 1; ++dk) {
           if (k + dk < 0 || k + dk >= L.nz) continue;
-          v += L.a27[(size_t)(((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1)) * L.n + n] * x[row + (k + dk)];
+          const int o = ((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1);
+          const size_t q = (size_t)((long long)n + dj * sy + di * sx + dk);
+          const float c = o >= 13 ? L.c14[(size_t)(o - 13) * L.n + n] : L.c14[(size_t)(13 - o) * L.n + q];
+          v += (double)c * xa(q);
         }
       }
     }
@@ -650,64 +665,46 @@ __device__ inline double mg_apply(const MgArgs& m, const MgLevelDev& L, const do
   }
 }
 
-// x = wdinv b (one Jacobi sweep from zero)
-__global__ __launch_bounds__(256) void k_mg_first(MgArgs m, int l) {
-  if (*m.done) return;
-  const MgLevelDev& L = m.L[l];
-  GRID_LOOP(n, L.n) L.x[n] = L.wdinv[n] * L.b[n];
+// t = x1 + wdinv (b - A x1) with x1 = wdinv b: two Jacobi sweeps from zero
+template <bool FINE, class LP>
+__device__ inline void ph_pre2(const MgArgs& m, const MgLevelDev& L) {
+  auto x1 = [&](size_t q) { return L.wdinv[q] * L.b[q]; };
+  for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    L.t[n] = x1(n) + L.wdinv[n] * (L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k));
+  }
 }
 
 // dst = src + wdinv (b - A src)
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_mg_sweep(MgArgs m, int l, const double* src, double* dst) {
-  if (*m.done) return;
-  const MgLevelDev& L = m.L[l];
-  GRID_LOOP(n, L.n) {
+template <bool FINE, class LP>
+__device__ inline void ph_sweep(const MgArgs& m, const MgLevelDev& L, const double* src, double* dst) {
+  auto xs = [&](size_t q) { return src[q]; };
+  for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    dst[n] = src[n] + L.wdinv[n] * (L.b[n] - mg_apply<FINE>(m, L, src, n, i, j, k));
+    dst[n] = src[n] + L.wdinv[n] * (L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k));
   }
 }
 
-// last sweep of level 0 into z, with the partial r.z of the CG (r = b of level 0)
-__global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, const double* src, double* z, double* part_rz) {
-  if (*m.done) return;
-  const MgLevelDev& L = m.L[0];
-  double rz = 0;
-  GRID_LOOP(n, L.n) {
+// r = b - A t
+template <bool FINE, class LP>
+__device__ inline void ph_resid(const MgArgs& m, const MgLevelDev& L) {
+  auto xs = [&](size_t q) { return L.t[q]; };
+  for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    const double v = src[n] + L.wdinv[n] * (L.b[n] - mg_apply<true>(m, L, src, n, i, j, k));
-    z[n] = v;
-    rz += L.b[n] * v;
-  }
-  block_reduce3(rz, 0.0, 0.0, part_rz);
-}
-
-// r = b - A src
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m, int l, const double* src) {
-  if (*m.done) return;
-  const MgLevelDev& L = m.L[l];
-  GRID_LOOP(n, L.n) {
-    int i, j, k;
-    lvl_ijk(L, n, i, j, k);
-    L.r[n] = L.b[n] - mg_apply<FINE>(m, L, src, n, i, j, k);
+    L.r[n] = L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k);
   }
 }
 
-__device__ inline double pw1(int f, int C) {
-  const int d = f - 2 * C;
-  return d == 0 ? 1.0 : 0.5;
-}
+__device__ inline double pw1(int f, int C) { return f == 2 * C ? 1.0 : 0.5; }
 
-// b_{l+1} = P^T r_l: coarse node I gathers the fine residuals around fine node 2I
+// b_C = P^T r_F: coarse node I gathers the fine residuals around fine node 2I
 // (residuals of fixed fine nodes are zero, so the zeroed rows of P need no test)
-__global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
-  if (*m.done) return;
-  const MgLevelDev& F = m.L[l];
-  const MgLevelDev& C = m.L[l + 1];
-  GRID_LOOP(I, C.n) {
+template <class LP>
+__device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
+  for (size_t I = LP::start(); I < C.n; I += LP::stride()) {
     int ci, cj, ck;
     lvl_ijk(C, I, ci, cj, ck);
     double v = 0;
@@ -715,19 +712,16 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
       for (int fi = max(2 * ci - 1, 0); fi <= min(2 * ci + 1, F.nx - 1); ++fi) {
         const double w = pw1(fj, cj) * pw1(fi, ci);
         const size_t row = ((size_t)fj * F.nx + fi) * F.nz;
-        for (int fk = max(2 * ck - 1, 0); fk <= min(2 * ck + 1, F.nz - 1); ++fk)
-          v += w * pw1(fk, ck) * F.r[row + fk];
+        for (int fk = max(2 * ck - 1, 0); fk <= min(2 * ck + 1, F.nz - 1); ++fk) v += w * pw1(fk, ck) * F.r[row + fk];
       }
     C.b[I] = v;
   }
 }
 
-// dst_l += P e_{l+1} on the unknown nodes (wdinv > 0)
-__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m, int l, double* dst, const double* e) {
-  if (*m.done) return;
-  const MgLevelDev& F = m.L[l];
-  const MgLevelDev& C = m.L[l + 1];
-  GRID_LOOP(n, F.n) {
+// t_F += P e on the unknown nodes (wdinv > 0)
+template <class LP>
+__device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, const double* e) {
+  for (size_t n = LP::start(); n < F.n; n += LP::stride()) {
     if (F.wdinv[n] == 0.0) continue;
     int i, j, k;
     lvl_ijk(F, n, i, j, k);
@@ -744,25 +738,113 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m, int l, double* dst
         }
       }
     }
-    dst[n] += v;
+    F.t[n] += v;
   }
 }
 
-// coarsest level: x = C^-1 b (dense); with a single level the result is z and the
-// partial r.z is taken here
-__global__ __launch_bounds__(256) void k_mg_coarse(MgArgs m, double* out, double* part_rz) {
-  if (*m.done) return;
-  const MgLevelDev& L = m.L[m.nl - 1];
+// out = C^-1 in (dense, row major), one wave per row; `wave` and `nwaves` number
+// the waves taking part
+__device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out, size_t wave, size_t nwaves) {
   const int nc = m.nc;
-  double rz = 0;
-  GRID_LOOP(r, (size_t)nc) {
+  const int lane = threadIdx.x & 63;
+  for (size_t r = wave; r < (size_t)nc; r += nwaves) {
     const double* row = m.cinv + r * nc;
     double v = 0;
-    for (int c = 0; c < nc; ++c) v += row[c] * L.b[c];
-    out[r] = v;
-    rz += L.b[r] * v;
+    for (int c = lane; c < nc; c += 64) v += row[c] * in[c];
+    v = wave_sum(v);
+    if (lane == 0) out[r] = v;
   }
-  if (part_rz) block_reduce3(rz, 0.0, 0.0, part_rz);
+}
+
+// result vector of level l after its part of the cycle
+__device__ __host__ inline const double* mg_result(const MgArgs& m, int l) {
+  return l == m.nl - 1 ? m.L[l].x : m.L[l].t;
+}
+
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_mg_pre2(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_pre2<FINE, GridLoop>(m, m.L[l]);
+}
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_resid<FINE, GridLoop>(m, m.L[l]);
+}
+__global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_restrict<GridLoop>(m.L[l], m.L[l + 1]);
+}
+__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_prolong<GridLoop>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
+}
+// dst = src + wdinv (b - A src) on level l >= 1: t -> x, then x -> t
+__global__ __launch_bounds__(256) void k_mg_post2a(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
+}
+__global__ __launch_bounds__(256) void k_mg_post2b(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].x, m.L[l].t);
+}
+__global__ __launch_bounds__(256) void k_mg_post0a(MgArgs m) {
+  if (*m.done) return;
+  ph_sweep<true, GridLoop>(m, m.L[0], m.L[0].t, m.L[0].x);
+}
+
+// last sweep of level 0 into z, with the partial r.z of the CG (r = b of level 0)
+__global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, double* z, double* part_rz) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[0];
+  const double* src = L.x;
+  auto xs = [&](size_t q) { return src[q]; };
+  double rz = 0;
+  for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    const double v = src[n] + L.wdinv[n] * (L.b[n] - mg_Ax<true>(m, L, xs, n, i, j, k));
+    z[n] = v;
+    rz += L.b[n] * v;
+  }
+  block_reduce3(rz, 0.0, 0.0, part_rz);
+}
+
+// Levels lb .. nl-1 in one workgroup: down-sweep, dense coarsest solve, up-sweep.
+__global__ __launch_bounds__(1024) void k_mg_bottom(MgArgs m) {
+  if (*m.done) return;
+  const int nl = m.nl, lb = m.lb;
+  for (int l = lb; l < nl - 1; ++l) {
+    ph_pre2<false, BlockLoop>(m, m.L[l]);
+    __syncthreads();
+    ph_resid<false, BlockLoop>(m, m.L[l]);
+    __syncthreads();
+    ph_restrict<BlockLoop>(m.L[l], m.L[l + 1]);
+    __syncthreads();
+  }
+  ph_coarse(m, m.L[nl - 1].b, m.L[nl - 1].x, threadIdx.x >> 6, blockDim.x >> 6);
+  __syncthreads();
+  for (int l = nl - 2; l >= lb; --l) {
+    ph_prolong<BlockLoop>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
+    __syncthreads();
+    ph_sweep<false, BlockLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
+    __syncthreads();
+    ph_sweep<false, BlockLoop>(m, m.L[l], m.L[l].x, m.L[l].t);
+    __syncthreads();
+  }
+}
+
+// single-level hierarchy: z = C^-1 r over the grid, then the partial r.z
+__global__ __launch_bounds__(256) void k_mg_coarse_z(MgArgs m, double* z) {
+  if (*m.done) return;
+  ph_coarse(m, m.L[0].b, z, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
+}
+__global__ __launch_bounds__(256) void k_mg_dot_rz(MgArgs m, const double* z, double* part_rz) {
+  if (*m.done) return;
+  const MgLevelDev& L = m.L[0];
+  double rz = 0;
+  for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) rz += L.b[n] * z[n];
+  block_reduce3(rz, 0.0, 0.0, part_rz);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -832,35 +914,31 @@ int emc_pcg_blocks(size_t n) { return grid_for((long long)n, 256, 1024); }
 static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hipStream_t s) {
   const int nl = m.nl;
   if (nl == 1) {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(a.nblocks), dim3(256), 0, s, m, a.z, part_rz);
+    hipLaunchKernelGGL(k_mg_coarse_z, dim3(grid_for((long long)m.nc * 64, 256)), dim3(256), 0, s, m, a.z);
+    hipLaunchKernelGGL(k_mg_dot_rz, dim3(a.nblocks), dim3(256), 0, s, m, (const double*)a.z, part_rz);
     return;
   }
-  for (int l = 0; l < nl - 1; ++l) {
-    const MgLevelDev& L = m.L[l];
-    const dim3 g(grid_for((long long)L.n, 256));
-    hipLaunchKernelGGL(k_mg_first, g, dim3(256), 0, s, m, l);
+  for (int l = 0; l < m.lb; ++l) {
+    const dim3 g(grid_for((long long)m.L[l].n, 256));
     if (l == 0) {
-      hipLaunchKernelGGL(k_mg_sweep<true>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
-      hipLaunchKernelGGL(k_mg_resid<true>, g, dim3(256), 0, s, m, l, (const double*)L.t);
+      hipLaunchKernelGGL(k_mg_pre2<true>, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_resid<true>, g, dim3(256), 0, s, m, l);
     } else {
-      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
-      hipLaunchKernelGGL(k_mg_resid<false>, g, dim3(256), 0, s, m, l, (const double*)L.t);
+      hipLaunchKernelGGL(k_mg_pre2<false>, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_resid<false>, g, dim3(256), 0, s, m, l);
     }
     hipLaunchKernelGGL(k_mg_restrict, dim3(grid_for((long long)m.L[l + 1].n, 256)), dim3(256), 0, s, m, l);
   }
-  const MgLevelDev& Lc = m.L[nl - 1];
-  hipLaunchKernelGGL(k_mg_coarse, dim3(grid_for((long long)m.nc, 256)), dim3(256), 0, s, m, Lc.x, (double*)nullptr);
-  for (int l = nl - 2; l >= 0; --l) {
-    const MgLevelDev& L = m.L[l];
-    const double* e = (l + 1 == nl - 1) ? m.L[l + 1].x : m.L[l + 1].t;
-    const dim3 g(grid_for((long long)L.n, 256));
-    hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m, l, L.t, e);
+  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(1024), 0, s, m);
+  for (int l = m.lb - 1; l >= 0; --l) {
+    const dim3 g(grid_for((long long)m.L[l].n, 256));
+    hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m, l);
     if (l == 0) {
-      hipLaunchKernelGGL(k_mg_sweep<true>, g, dim3(256), 0, s, m, l, (const double*)L.t, L.x);
-      hipLaunchKernelGGL(k_mg_sweep_last, dim3(a.nblocks), dim3(256), 0, s, m, (const double*)L.x, a.z, part_rz);
+      hipLaunchKernelGGL(k_mg_post0a, g, dim3(256), 0, s, m);
+      hipLaunchKernelGGL(k_mg_sweep_last, dim3(a.nblocks), dim3(256), 0, s, m, a.z, part_rz);
     } else {
-      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.t, L.x);
-      hipLaunchKernelGGL(k_mg_sweep<false>, g, dim3(256), 0, s, m, l, (const double*)L.x, L.t);
+      hipLaunchKernelGGL(k_mg_post2a, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_post2b, g, dim3(256), 0, s, m, l);
     }
   }
 }

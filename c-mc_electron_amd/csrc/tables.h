@@ -23,8 +23,22 @@ struct alignas(16) TetRec {
 };
 static_assert(sizeof(TetRec) == 208, "TetRec layout");
 
+// What one try of get_state's rejection loop reads, in one 64-byte line: corner
+// energies, the FASTSURF area factors and |v| of the tetrahedron.
+struct alignas(64) SurfRec {
+  double e[4];
+  double faclow, facmed, fachigh, vgt;
+};
+
+// Corner coordinates and energies of a tetrahedron for getK, in one 128-byte line.
+struct alignas(128) CornerRec {
+  double x[4], y[4], z[4], e[4];
+};
+
 struct BandDev {
   const TetRec* tr;
+  const SurfRec* sr;
+  const CornerRec* cr;
   const int* tet;  // [4][nt]
   const double *xkk, *ykk, *zkk, *eek;
   const double *vgt, *faclow, *facmed, *fachigh;

@@ -103,7 +103,7 @@ class FlyTallies(C.Structure):
         ("drops", C.c_int64 * NREASON),
         ("n_catch", C.c_int64), ("n_gen", C.c_int64),
         ("n_phonon", C.c_int64), ("n_impurity", C.c_int64),
-        ("n_events", C.c_int64), ("n_clone_passes", C.c_int64),
+        ("n_events", C.c_int64), ("n_clone_passes", C.c_int64), ("n_full_loop", C.c_int64),
     ]
 
 

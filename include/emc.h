@@ -163,6 +163,8 @@ typedef struct emc_fly_tallies {
   int64_t n_phonon, n_impurity; /* real scattering events (sttt counters)    */
   int64_t n_events;          /* event-loop iterations                        */
   int64_t n_clone_passes;
+  int64_t n_full_loop;       /* particles that needed the full event loop (the
+                                rest drifted to the end of the step with no event) */
 } emc_fly_tallies;
 
 /* Options for emc_create. */
