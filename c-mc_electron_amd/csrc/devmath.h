@@ -219,6 +219,20 @@ EMC_HD inline uint64_t init_key(uint64_t seed, uint64_t serial) { return mix64(m
 
 // One particle's stream for one step: draw n is element n%2 of Philox block n/2;
 // 52-bit mantissa -> (m + 0.5) 2^-52, strictly inside (0,1).
+// A zero defined per lane (in a vector register).  The draw counters below start
+// from it: they change only through control flow (how many draws a particle
+// makes), and a value that starts per lane can never be merged into one
+// wave-wide register, whatever the compiler concludes about the branches.
+EMC_HD inline uint32_t lane_zero() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+#else
+  return 0u;
+#endif
+}
+
 struct Rng {
   uint32_t k0, k1, c1, c2, block;
   int have;
@@ -226,7 +240,8 @@ struct Rng {
   EMC_HD void init(uint64_t key, uint64_t step) {
     k0 = (uint32_t)key; k1 = (uint32_t)(key >> 32);
     c1 = (uint32_t)step; c2 = (uint32_t)(step >> 32);
-    block = 0; have = 0; spare = 0.0;
+    const uint32_t z = lane_zero();
+    block = z; have = (int)z; spare = 0.0;
   }
   EMC_HD double next() {
     if (have) { have = 0; return spare; }

@@ -72,6 +72,8 @@ struct Gpu {
   MeshDev M{};
   FieldsDev F{};
   TallyDev T{};
+  FlyConst* cst = nullptr;            // device copy of {B, M, F} for the flight kernels
+  FlyConst cst_host{};                // what cst holds
   PartDev P{};
   PartDev P2{};                       // second particle buffer: the tile sort writes into it
   long long n = 0, cap = 0;
@@ -289,6 +291,7 @@ static int gpu_setup(emc_ctx* c) {
   g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
   g.counters = m.alloc<unsigned long long>(CNT_N);
   g.ctr = m.alloc<unsigned long long>(4);
+  g.cst = m.alloc<FlyConst>(1);
   g.ntx = (d.cnx + TILE_I - 1) / TILE_I;
   g.nty = (d.cny + TILE_J - 1) / TILE_J;
   g.ntz = (d.cnz + TILE_K - 1) / TILE_K;
@@ -730,10 +733,23 @@ int emc_fields_get(emc_ctx* c, emc_cell_fields* f) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- step pieces
+// Brings the device copy of the table/field descriptors up to date (they change
+// only when hole fields get storage of their own or are aliased back).
+static int sync_fly_const(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  FlyConst k{};
+  k.B = g.B; k.M = g.M; k.F = g.F;
+  if (std::memcmp(&k, &g.cst_host, sizeof(k)) == 0) return EMC_OK;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));  // no flight kernel may still read the old copy
+  HIPCHECK(c, hipMemcpy(g.cst, &k, sizeof(k), hipMemcpyHostToDevice));
+  g.cst_host = k;
+  return EMC_OK;
+}
+
 static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
   Gpu& g = *c->gpu;
   FlyArgs a{};
-  a.B = g.B; a.M = g.M; a.F = g.F; a.T = g.T; a.P = g.P;
+  a.C = g.cst; a.T = g.T; a.P = g.P;
   if (!flag_heat) { a.T.heat_e = nullptr; a.T.heat_h = nullptr; }
   a.n = g.n; a.cap = g.cap; a.step = step; a.flag_heat = flag_heat;
   a.clones = g.clones; a.nclones = g.ctr; a.clone_cap = g.clone_cap;
@@ -747,6 +763,8 @@ static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
 static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
   Gpu& g = *c->gpu;
   int st = settle_n(c);
+  if (st) return st;
+  st = sync_fly_const(c);
   if (st) return st;
   HIPCHECK(c, emc_launch_zero(g.ctr, 4, g.stream));
   FlyArgs a = fly_args(c, step, flag_heat);

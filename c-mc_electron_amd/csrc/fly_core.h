@@ -29,6 +29,14 @@ namespace emc {
 
 constexpr int FLY_LOOP_CAP = 65536;
 
+// true if the condition holds for any active lane of the wave (device) / the
+// condition itself (host)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define EMC_ANY(p) (__ballot(p) != 0ull)
+#else
+#define EMC_ANY(p) (p)
+#endif
+
 EMC_HD inline void tally_add(unsigned long long* p, long long v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   atomicAdd(p, (unsigned long long)v);
@@ -326,14 +334,21 @@ struct Flight {
     const int* list = smallfl ? B.tlists : B.tlist;
     if (nted == 0) { Flag_SelfScatter = 1; return; }
     int count2 = 0;
-    for (;;) {
+    // Rejection loop.  On the GPU every lane of the wave leaves it together
+    // (lanes that are done wait without drawing): a loop whose lanes exit on
+    // different iterations, with the random-number state live after it, gave
+    // wrong draws in getK for some lanes on gfx950.  Draws per particle are
+    // the same as in the plain loop.
+    bool again = true;
+    while (EMC_ANY(again)) {
+      if (!again) continue;
       itet = list[(int)(nted * Random()) + base];
       count2++;
       const SurfRec& sr = B.sr[itet];
       if ((count2 < 10000) && !((sr.e[0] < e) && (sr.e[3] > e))) continue;
       const double dostet = fastsurf_r(e, itet, sr) / sr.vgt;
       if (((Random() * maxdostet) > dostet) && (count2 < 10000)) continue;
-      break;
+      again = false;
     }
     if (count2 >= 10000) Flag_SelfScatter = 1;
     else { Flag_SelfScatter = 0; getK(); }
@@ -866,21 +881,86 @@ struct Flight {
     }
   }
 
-  // The common case of a step: the first event is the end of the time step.  Runs
-  // exactly the first loop iteration of fly(); returns false (state untouched in
-  // memory) when anything else happens first, so fly() can redo the particle.
-  EMC_HD EMC_INL bool fly_fast() {
+  // The common case of a step: up to FAST_EVENTS events that are tetrahedron
+  // crossings inside the zone or wedge changes, cell crossings by PASS, REFLECT,
+  // SCATTOX with surface scattering on, or PERIOD, then the end of the step.
+  // Runs exactly the iterations of fly() for those events.  Returns false when
+  // anything else comes first (scattering, a Brillouin-zone face, contacts,
+  // leaving the mesh, a second current tally): the particle in memory is
+  // untouched and fly() redoes it from the start with the same random numbers.
+  // A y-plane current of a PASS crossing is held in (pend_kind, pend_j) and
+  // added by commit_pending() only once the particle has finished here.
+  static constexpr int FAST_EVENTS = 16;
+  EMC_HD EMC_INL bool fast_cell_crossing(int& pend_kind, int& pend_j) {
+    const int imot = M.motion[cell(icell, jcell, kcell) * 6 + idir];
+    if (imot == PASS) {
+      int ii, jj, kk;
+      if (!neighbour(ii, jj, kk)) return false;
+      switch (idir) {
+        case UP: x = M.lx[icell]; icell--; break;
+        case DOWN: x = M.lx[icell + 1]; icell++; break;
+        case RIGHT:
+          if (pend_kind) return false;
+          y = M.ly[jcell + 1];
+          jcell++;
+          pend_kind = 1; pend_j = jcell;
+          break;
+        case LEFT:
+          if (pend_kind) return false;
+          y = M.ly[jcell];
+          pend_kind = 2; pend_j = jcell;
+          jcell--;
+          break;
+        case FRONT: z = M.lz[kcell]; kcell--; break;
+        case BACK: z = M.lz[kcell + 1]; kcell++; break;
+      }
+    } else if (imot == REFLECT || (imot == SCATTOX && M.flag_surface)) {
+      Reflect();
+    } else if (imot == PERIOD) {
+      if (idir == UP) x = M.lx[icell + 1];
+      else if (idir == DOWN) x = M.lx[icell];
+      else if (idir == LEFT) y = M.ly[jcell + 1];
+      else if (idir == RIGHT) y = M.ly[jcell];
+      else if (idir == FRONT) z = M.lz[kcell + 1];
+      else if (idir == BACK) z = M.lz[kcell];
+    } else
+      return false;
+    load_cell();
+    return true;
+  }
+  EMC_HD EMC_INL bool fly_fast(int& pend_kind, int& pend_j) {
     Loop L;
     fly_begin(L);
-    GetV();
-    double Tf, gam;
-    size_t c;
-    const int flag = next_event(L, Tf, gam, c);
-    if (flag != 6 || Tf < MY_ZERO) return false;
-    advance(L, Tf, gam);
-    if (energy < 0) return false;
-    n_events = 1;
+    pend_kind = 0;
+    pend_j = 0;
+    while (left_time > 0) {
+      GetV();
+      L.loop++;
+      n_events++;
+      if (L.loop > FAST_EVENTS) return false;
+      double Tf, gam;
+      size_t c;
+      const int flag = next_event(L, Tf, gam, c);
+      if (Tf < MY_ZERO || (flag >= 3 && flag <= 5)) return false;
+      advance(L, Tf, gam);
+      if (energy < 0) return false;
+      if (flag == 1) {
+        // HitTet without the Brillouin-zone fold
+        const int nb = B.tr[itet].neib[itetdir];
+        if (nb == -5 || nb == -6) return false;
+        if (nb >= 0) itet = nb;
+        else isym = B.newisym[(-nb - 2) * 48 + isym];
+        L.getTet = L.getCell = true;
+      } else if (flag == 2) {
+        if (!fast_cell_crossing(pend_kind, pend_j)) return false;
+        L.getTet = L.getCell = true;
+      }
+    }
     return true;
+  }
+  EMC_HD EMC_INL void commit_pending(int pend_kind, int pend_j) {
+    if (pend_kind == 1) tally_add(&T.rcur[pend_j], fxd(charge, EMC_FX_CONTACT_));
+    else if (pend_kind == 2) tally_add(&T.lcur[pend_j], fxd(charge, EMC_FX_CONTACT_));
   }
 };
 

@@ -9,10 +9,18 @@
 
 namespace emc {
 
-struct FlyArgs {
+// Descriptors of the band tables, the mesh and the cell fields, resident in
+// device memory: the flight kernels read them through one pointer, so the
+// compiler loads each table pointer where it is used instead of keeping some
+// eighty of them in scalar registers across the event loop.
+struct FlyConst {
   BandDev B;
   MeshDev M;
   FieldsDev F;
+};
+
+struct FlyArgs {
+  const FlyConst* C;           // device memory
   TallyDev T;
   PartDev P;
   long long n, cap;            // live range [0,n) and array capacity

@@ -58,17 +58,17 @@ __device__ inline void load_particle(Flight& f, const FlyArgs& a, long long q, u
   f.icell = pk_i(pk); f.jcell = pk_j(pk); f.kcell = pk_k(pk); f.par_type = pk_type(pk);
   f.itet = a.P.itet[q]; f.isym = a.P.isym[q];
   f.key = a.P.key[q]; f.step = a.step;
-  f.left_time = a.M.dt;
+  f.left_time = f.M.dt;
   f.nclone = 0; f.flag_heat = a.flag_heat;
   f.n_events = 0; f.n_phonon = 0; f.n_impurity = 0;
   f.rng.init(f.key, a.step);
 }
 
-// Fast pass: every particle whose first event is the end of the time step drifts
-// and is written back (position, wave vector and energy; cell, tetrahedron and
-// wedge are unchanged).  The others are listed, wave-compacted, for k_fly_slow,
-// which restarts them from the untouched state: the RNG is counter based, so the
-// full event loop draws the same numbers and the result equals one pass of fly().
+// Fast pass: particles whose step has only tetrahedron and plain cell crossings
+// (Flight::fly_fast) finish here.  The others are listed, wave-compacted, for
+// k_fly_slow, which restarts them from the untouched state: the RNG is counter
+// based, so the full event loop draws the same numbers and the result equals one
+// pass of fly().
 __global__ __launch_bounds__(256) void k_fly_fast(FlyArgs a) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   long long ev = 0;
@@ -80,13 +80,18 @@ __global__ __launch_bounds__(256) void k_fly_fast(FlyArgs a) {
     if (q < a.n) {
       const uint32_t pk = a.P.pk[q];
       if (pk != PK_DEAD) {
-        Flight f(a.B, a.M, a.F, a.T);
+        Flight f(a.C->B, a.C->M, a.C->F, a.T);
         load_particle(f, a, q, pk);
-        if (f.fly_fast()) {
+        int pend_kind, pend_j;
+        if (f.fly_fast(pend_kind, pend_j)) {
           ev += f.n_events;
+          f.commit_pending(pend_kind, pend_j);
           a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
           a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
           a.P.energy[q] = f.energy;
+          const uint32_t pk2 = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
+          if (pk2 != pk) a.P.pk[q] = pk2;
+          a.P.itet[q] = f.itet; a.P.isym[q] = f.isym;
         } else
           slow = true;
       }
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ns; e += stride) {
     const long long q = a.slow[e];
     const uint32_t pk = a.P.pk[q];
-    Flight f(a.B, a.M, a.F, a.T);
+    Flight f(a.C->B, a.C->M, a.C->F, a.T);
     load_particle(f, a, q, pk);
     f.fly();
     ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
@@ -139,7 +144,7 @@ __global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long lon
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long q = (long long)begin + blockIdx.x * blockDim.x + threadIdx.x; q < (long long)end; q += stride) {
     CloneRec& c = a.clones[q];
-    Flight f(a.B, a.M, a.F, a.T);
+    Flight f(a.C->B, a.C->M, a.C->F, a.T);
     f.clones = a.clones;
     f.nclones = a.nclones;
     f.clone_cap = a.clone_cap;

@@ -215,3 +215,29 @@ def test_poisson_multilevel_mesh():
         assert np.linalg.norm(x2 - xg) <= 1e-10 * np.linalg.norm(xg)
     finally:
         e.close()
+
+
+def test_flight_parity_large_population():
+    """128 k particles in a strong random field, five steps with heat: many lanes of
+    a wave scatter in the same step, which a flight kernel that mixes per-lane
+    state across lanes gets wrong (the rejection loop of get_state is the case
+    this caught)."""
+    from helpers import deck
+    print("[gpu] flight parity, 128k particles", flush=True)
+    e = emcgpu.Engine(deck("big", refine=1, electrons=100000, holes=20000, vd=0.5, vg=0.7), device=0)
+    try:
+        e.particles_init()
+        _, f = field_like(e, seed=7, scale=60.0)
+        e.fields_set(f)
+        e.tallies_reset()
+        po = e.download()
+        for step in range(5):
+            tg = e.fly_step(50 + step, flag_heat=1)
+            pg = e.download()
+            po, to, _ = oracle_lib.fly_step(e, po, po.n, f, 50 + step, flag_heat=1)
+            assert_same_tallies(tg, to, f"large step {step}")
+            assert_same_population(pg, po, f"large step {step}")
+            po = sort_by_key(po)
+        assert tg.n_full_loop > 0 and tg.n_phonon + tg.n_impurity > 100
+    finally:
+        e.close()
