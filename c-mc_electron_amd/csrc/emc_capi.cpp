@@ -113,6 +113,7 @@ struct Gpu {
   const int* eeff = nullptr;
   int pcg_blocks = 0;
   long long last_full_loop = 0;       // particles of the last flight that took the full event loop
+  long long wave_slots = 1024;        // CUs x 4 SIMDs
   hipEvent_t ev[8] = {};
 };
 
@@ -135,6 +136,7 @@ struct emc_ctx {
   std::vector<TetRec> tetrec;
   std::vector<SurfRec> surfrec;
   std::vector<CornerRec> cornerrec;
+  std::vector<Sphere> tsph, tlist_s, tlists_s, slist_s;
   std::vector<int> surface_dir;
   HostParticles hp;
   std::unique_ptr<Gpu> gpu;
@@ -163,6 +165,8 @@ static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
   B.tr = c->tetrec.data();
   B.sr = c->surfrec.data();
   B.cr = c->cornerrec.data();
+  B.tsph = c->tsph.data();
+  B.tlist_s = c->tlist_s.data(); B.tlists_s = c->tlists_s.data(); B.slist_s = c->slist_s.data();
   B.tet = b.tet.data();
   B.xkk = b.xkk.data(); B.ykk = b.ykk.data(); B.zkk = b.zkk.data(); B.eek = b.eek.data();
   B.vgt = b.vgt.data(); B.faclow = b.faclow.data(); B.facmed = b.facmedium.data(); B.fachigh = b.fachigh.data();
@@ -220,6 +224,55 @@ static void build_tetrec(emc_ctx* c) {
     }
     s.faclow = b.faclow[it]; s.facmed = b.facmedium[it]; s.fachigh = b.fachigh[it]; s.vgt = b.vgt[it];
   }
+  // Enclosing spheres of the tetrahedra grown by the containment tolerance: corner
+  // i of the grown tetrahedron solves n_f . v = d_f - MY_ZERO on the three faces
+  // f != i (face f is opposite corner f).  Centre = centroid, radius = farthest
+  // grown corner, widened by a relative 1e-9 for rounding.  A degenerate
+  // tetrahedron gets an infinite radius (always tested exactly).
+  c->tsph.assign(b.nt, Sphere{});
+  for (int it = 0; it < b.nt; ++it) {
+    const CornerRec& q = c->cornerrec[it];
+    const double cx = 0.25 * (q.x[0] + q.x[1] + q.x[2] + q.x[3]);
+    const double cy = 0.25 * (q.y[0] + q.y[1] + q.y[2] + q.y[3]);
+    const double cz = 0.25 * (q.z[0] + q.z[1] + q.z[2] + q.z[3]);
+    double r2max = 0.0;
+    bool ok = true;
+    for (int i = 0; i < 4 && ok; ++i) {
+      double m[3][4];
+      int row = 0;
+      for (int f = 0; f < 4; ++f) {
+        if (f == i) continue;
+        for (int k = 0; k < 3; ++k) m[row][k] = b.dn(k, f, it);
+        m[row][3] = b.dn(3, f, it) - MY_ZERO;
+        ++row;
+      }
+      const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                         m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+      if (!(std::fabs(det) > 0.0)) { ok = false; break; }
+      auto col = [&](int k, double& v) {
+        double t[3][3];
+        for (int r = 0; r < 3; ++r)
+          for (int cc = 0; cc < 3; ++cc) t[r][cc] = cc == k ? m[r][3] : m[r][cc];
+        v = (t[0][0] * (t[1][1] * t[2][2] - t[1][2] * t[2][1]) - t[0][1] * (t[1][0] * t[2][2] - t[1][2] * t[2][0]) +
+             t[0][2] * (t[1][0] * t[2][1] - t[1][1] * t[2][0])) / det;
+      };
+      double vx, vy, vz;
+      col(0, vx); col(1, vy); col(2, vz);
+      const double r2 = (vx - cx) * (vx - cx) + (vy - cy) * (vy - cy) + (vz - cz) * (vz - cz);
+      if (!std::isfinite(r2)) { ok = false; break; }
+      if (r2 > r2max) r2max = r2;
+    }
+    const double r = ok ? std::sqrt(r2max) * (1.0 + 1e-9) + 1e-12 : HUGE_VAL;
+    c->tsph[it] = Sphere{cx, cy, cz, ok ? r * r : HUGE_VAL};
+  }
+  auto along = [&](const std::vector<int>& list, std::vector<Sphere>& out) {
+    out.resize(list.size());
+    for (size_t q = 0; q < list.size(); ++q) out[q] = c->tsph[list[q]];
+  };
+  along(b.tlist, c->tlist_s);
+  along(b.tlists, c->tlists_s);
+  along(b.slist, c->slist_s);
 }
 
 // Multigrid hierarchy of the Poisson preconditioner, built once per context.
@@ -239,6 +292,11 @@ static int gpu_setup(emc_ctx* c) {
   g.device = c->opts.device;
   HIPCHECK(c, hipSetDevice(g.device));
   HIPCHECK(c, hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g.device) == hipSuccess && ncu > 0)
+      g.wave_slots = 4LL * ncu;
+  }
   for (auto& e : g.ev) HIPCHECK(c, hipEventCreate(&e));
   const Band& b = c->band;
   const Device& d = c->dev;
@@ -248,6 +306,8 @@ static int gpu_setup(emc_ctx* c) {
   B.tr = m.dup(c->tetrec);
   B.sr = m.dup(c->surfrec);
   B.cr = m.dup(c->cornerrec);
+  B.tsph = m.dup(c->tsph);
+  B.tlist_s = m.dup(c->tlist_s); B.tlists_s = m.dup(c->tlists_s); B.slist_s = m.dup(c->slist_s);
   B.tet = m.dup(b.tet);
   B.xkk = m.dup(b.xkk); B.ykk = m.dup(b.ykk); B.zkk = m.dup(b.zkk); B.eek = m.dup(b.eek);
   B.vgt = m.dup(b.vgt); B.faclow = m.dup(b.faclow); B.facmed = m.dup(b.facmedium); B.fachigh = m.dup(b.fachigh);
@@ -559,6 +619,7 @@ int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
   std::memset(v, 0, sizeof(*v));
   v->nk = b.nk; v->nt = b.nt; v->nbt = b.nbt;
   for (int q = 0; q < 3; ++q) { v->nband[q] = b.nband[q]; v->bandof[q] = b.bandof[q]; v->difpr[q] = b.difpr[q]; v->DOSMAX[q] = b.DOSMAX[q]; }
+  v->tsphere = c->tsph.empty() ? nullptr : &c->tsph[0].cx;
   v->scpre = b.scpre; v->scprh = b.scprh;
   v->dtable = b.dtable; v->emax = b.emax; v->dlist = b.dlist; v->dlists = b.dlists; v->a0pi = b.a0pi; v->sieg = b.sieg;
   v->mell = b.mell; v->melt = b.melt; v->meld = b.meld;
@@ -755,6 +816,7 @@ static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
   a.clones = g.clones; a.nclones = g.ctr; a.clone_cap = g.clone_cap;
   a.dead = g.dead; a.ndead = g.ctr + 1; a.dead_cap = g.dead_cap;
   a.slow = g.slow; a.nslow = g.ctr + 3;
+  a.wave_slots = g.wave_slots;
   return a;
 }
 

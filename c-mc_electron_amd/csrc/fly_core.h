@@ -213,8 +213,16 @@ struct Flight {
     const double d4 = r.n[3][0] * a + r.n[3][1] * b + r.n[3][2] * c - r.n[3][3];
     return (d1 >= -MY_ZERO) && (d2 >= -MY_ZERO) && (d3 >= -MY_ZERO) && (d4 >= -MY_ZERO);
   }
-  EMC_HD EMC_INL bool scan(const int* list, int p0, int nn, double a, double b, double c) {
+  EMC_HD EMC_INL static bool in_sphere(const Sphere& s, double a, double b, double c) {
+    const double dx = a - s.cx, dy = b - s.cy, dz = c - s.cz;
+    return dx * dx + dy * dy + dz * dz <= s.r2;
+  }
+  // first tetrahedron of list[p0 .. p0+nn-1] (in list order) that holds the point;
+  // the sphere array runs alongside the list, so most candidates cost 32 bytes
+  // read in order instead of a dependent load of their record
+  EMC_HD EMC_INL bool scan(const int* list, const Sphere* sph, int p0, int nn, double a, double b, double c) {
     for (int itl = p0; itl <= p0 + nn - 1; ++itl) {
+      if (!in_sphere(sph[itl], a, b, c)) continue;
       const int it = list[itl];
       if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return true; }
     }
@@ -237,17 +245,17 @@ struct Flight {
     if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
     int ie = (int)(energy * B.dlists);
     if (ie < 0) ie = 0;
-    if (ie < MWLES && scan(B.tlists, B.ptlists[ie * MNB + iband], B.ntlists[ie * MNB + iband], a, b, c)) return;
+    if (ie < MWLES && scan(B.tlists, B.tlists_s, B.ptlists[ie * MNB + iband], B.ntlists[ie * MNB + iband], a, b, c)) return;
     ie = (int)(energy * B.dlist);
     if (ie < 0) ie = 0;
     if (ie >= MWLE) ie = MWLE - 1;
-    if (scan(B.tlist, B.ptlist[ie * MNB + iband], B.ntlist[ie * MNB + iband], a, b, c)) return;
-    if (ie + 1 < MWLE && scan(B.tlist, B.ptlist[(ie + 1) * MNB + iband], B.ntlist[(ie + 1) * MNB + iband], a, b, c))
+    if (scan(B.tlist, B.tlist_s, B.ptlist[ie * MNB + iband], B.ntlist[ie * MNB + iband], a, b, c)) return;
+    if (ie + 1 < MWLE && scan(B.tlist, B.tlist_s, B.ptlist[(ie + 1) * MNB + iband], B.ntlist[(ie + 1) * MNB + iband], a, b, c))
       return;
-    if (ie - 1 >= 0 && scan(B.tlist, B.ptlist[(ie - 1) * MNB + iband], B.ntlist[(ie - 1) * MNB + iband], a, b, c))
+    if (ie - 1 >= 0 && scan(B.tlist, B.tlist_s, B.ptlist[(ie - 1) * MNB + iband], B.ntlist[(ie - 1) * MNB + iband], a, b, c))
       return;
     for (int it = 0; it < B.nt; ++it)
-      if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return; }
+      if (in_sphere(B.tsph[it], a, b, c) && iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return; }
     Flag_Catch = 1;
     drop_reason = EMC_DROP_NO_TET_;
   }
@@ -259,7 +267,7 @@ struct Flight {
     int ie = (int)(energy * B.dlist);
     if (ie < 0) ie = 0;
     if (ie >= MWLE) ie = MWLE - 1;
-    if (scan(B.slist, B.pslist[ie * MNB + iband], B.nslist[ie * MNB + iband], a, b, c)) return;
+    if (scan(B.slist, B.slist_s, B.pslist[ie * MNB + iband], B.nslist[ie * MNB + iband], a, b, c)) return;
     Getisym();
   }
 
@@ -401,39 +409,74 @@ struct Flight {
   EMC_HD EMC_INL void restore(const Saved& s) {
     energy = s.e; kx = s.kx; ky = s.ky; kz = s.kz; iband = s.iband; itet = s.itet; isym = s.isym;
   }
-  EMC_HD EMC_INL void EPSCAT(int iscat) {
-    const Saved s0 = save();
-    iband = iscat / B.scpre + B.bandofE;
-    const int p = iscat % B.scpre;
-    if (p == 0) TALIND(1);
-    else if (p <= 12) {
-      const double temps[6] = {B.temptag, B.templag, B.templog, B.temptaf, B.templaf, B.temptof};
-      const double hw = temps[(p - 1) >> 1];
-      energy = (p & 1) ? energy + hw : energy - hw;
-      TALIND(p <= 6 ? 2 : 3);
+  // The scattering routines that end in get_state are split in two around it:
+  // prep_* does what comes before get_state and returns which finish() applies
+  // after it, so get_state (with getK) and the process selection are compiled
+  // once in the event loop (a smaller flight kernel).  The order of operations
+  // and random draws is that of the routines of the reference.
+  enum { POST_NONE = 0, POST_EPSCAT = 1, POST_SCATII = 2, POST_HPSCAT = 3, POST_DIFFUSE = 4 };
+
+  // EPSCAT (src/ElectronPhononScatter.cpp:98-195), ESCATII / HSCATII (:406-497,
+  // src/HolePhononScatter.cpp) and HPSCAT (src/HolePhononScatter.cpp), up to get_state
+  EMC_HD EMC_INL int prep_process(int iscat, int np, bool elec, Saved& s0) {
+    s0 = save();
+    if (((iscat % np) + 1) == np) {  // impact ionisation
+      iband = iscat / np + (par_type == PELEC ? B.bandofE : B.bandofH);
+      energy = (energy - B.sieg) / 3.0;
+      isym = ((int)(48 * Random()) % 48);
+      return POST_SCATII;
     }
-    get_state();
-    if (Flag_SelfScatter) restore(s0);
-    if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_e);
-  }
-  EMC_HD EMC_INL void SCATII(int iscat, int nproc) {
-    const Saved s0 = save();
-    iband = iscat / nproc + (par_type == PELEC ? B.bandofE : B.bandofH);
-    energy = (energy - B.sieg) / 3.0;
-    isym = ((int)(48 * Random()) % 48);
-    get_state();
-    if (Flag_SelfScatter) restore(s0);
-  }
-  EMC_HD EMC_INL void HPSCAT(int iscat) {
-    const Saved s0 = save();
+    if (elec) {
+      iband = iscat / B.scpre + B.bandofE;
+      const int p = iscat % B.scpre;
+      if (p == 0) TALIND(1);
+      else if (p <= 12) {
+        const double temps[6] = {B.temptag, B.templag, B.templog, B.temptaf, B.templaf, B.temptof};
+        const double hw = temps[(p - 1) >> 1];
+        energy = (p & 1) ? energy + hw : energy - hw;
+        TALIND(p <= 6 ? 2 : 3);
+      }
+      return POST_EPSCAT;
+    }
     iband = iscat / B.scprh + B.bandofH;
     const int p = iscat % B.scprh;
     if (p == 1) energy = energy + B.temphop;
     else if (p == 2) energy = energy - B.temphop;
     isym = ((int)(48 * Random()) % 48);
-    get_state();
-    if (Flag_SelfScatter) restore(s0);
-    if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), T.heat_h);
+    return POST_HPSCAT;
+  }
+  // Diffuse (src/ParSurfaceScatter.cpp, src/CellHit.cpp), up to get_state
+  EMC_HD EMC_INL int prep_diffuse(Saved& s0) {
+    s0 = save();
+    TALIND(1);
+    return POST_DIFFUSE;
+  }
+  EMC_HD EMC_INL void finish(int post, const Saved& s0) {
+    if (post == POST_EPSCAT || post == POST_HPSCAT) {
+      if (Flag_SelfScatter) restore(s0);
+      if (flag_heat) heat_to_point((energy - s0.e) * fabs(charge), post == POST_EPSCAT ? T.heat_e : T.heat_h);
+    } else if (post == POST_SCATII) {
+      if (Flag_SelfScatter) restore(s0);
+    } else if (post == POST_DIFFUSE) {
+      if (Flag_SelfScatter) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
+      GetV();
+      double v = 0.0;
+      int axis = 0;
+      bool flip = false, known = true;
+      switch (idir) {
+        case UP: v = vx; axis = 0; flip = v < 0.0; break;
+        case RIGHT: v = vy; axis = 1; flip = v > 0.0; break;
+        case DOWN: v = vx; axis = 0; flip = v > 0.0; break;
+        case LEFT: v = vy; axis = 1; flip = v < 0.0; break;
+        case FRONT: v = vz; axis = 2; flip = v < 0.0; break;
+        case BACK: v = vz; axis = 2; flip = v > 0.0; break;
+        default: known = false; break;
+      }
+      if (known) {
+        if (v == 0.0) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
+        if (flip) mirror(axis);
+      }
+    }
   }
   // rate of process iscat (CALSCATTE / CALSCATTH element, src/Band.cpp:2098-2231)
   EMC_HD EMC_INL double process_rate(bool elec, int iscat, int itab, double intp, int where) const {
@@ -457,7 +500,9 @@ struct Flight {
     if (where > 0) return drow[MTAB];
     return drow[itab] + intp * (drow[itab + 1] - drow[itab]);
   }
-  EMC_HD EMC_INL void select_process(double drand) {
+  // process selection of the phonon scatterings (src/ElectronPhononScatter.cpp:5-95,
+  // src/HolePhononScatter.cpp:4-60): cumulative search over the rates
+  EMC_HD EMC_INL int select_prep(double drand, Saved& s0) {
     const bool elec = par_type == PELEC;
     const int np = elec ? B.scpre : B.scprh;
     const int nmax = np * (elec ? B.nbandE : B.nbandH);
@@ -477,18 +522,17 @@ struct Flight {
       if (iscat >= nmax) { Flag_SelfScatter = 1; break; }
       dscat = dscat + process_rate(elec, iscat, itab, intp, where);
     }
-    if (Flag_SelfScatter) return;
-    if (((iscat % np) + 1) == np) SCATII(iscat, np);
-    else if (elec) EPSCAT(iscat);
-    else HPSCAT(iscat);
+    if (Flag_SelfScatter) return POST_NONE;
+    return prep_process(iscat, np, elec, s0);
   }
-  EMC_HD EMC_INL void PhononScatter() {
+  // ElectronPhononScatter / HolePhononScatter: self-scattering test against gamtet
+  EMC_HD EMC_INL int prep_phonon(Saved& s0) {
     const double scasum = calscattsum(energy, iband);
     const double g = B.tr[itet].gam;
     const double drand = scasum > g ? Random() * scasum : Random() * g;
-    if (scasum < drand) { Flag_SelfScatter = 1; return; }
+    if (scasum < drand) { Flag_SelfScatter = 1; return POST_NONE; }
     Flag_SelfScatter = 0;
-    select_process(drand);
+    return select_prep(drand, s0);
   }
 
   // ---- ionised impurities (src/ElectronImpurityScatter.cpp:6-306)
@@ -571,27 +615,6 @@ struct Flight {
     else if (idir == RIGHT || idir == LEFT) mirror(1);
     else if (idir == FRONT || idir == BACK) mirror(2);
   }
-  EMC_HD EMC_INL void Diffuse() {
-    const Saved s0 = save();
-    TALIND(1);
-    get_state();
-    if (Flag_SelfScatter) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
-    GetV();
-    double v;
-    int axis;
-    bool flip;
-    switch (idir) {
-      case UP: v = vx; axis = 0; flip = v < 0.0; break;
-      case RIGHT: v = vy; axis = 1; flip = v > 0.0; break;
-      case DOWN: v = vx; axis = 0; flip = v > 0.0; break;
-      case LEFT: v = vy; axis = 1; flip = v < 0.0; break;
-      case FRONT: v = vz; axis = 2; flip = v < 0.0; break;
-      case BACK: v = vz; axis = 2; flip = v > 0.0; break;
-      default: return;
-    }
-    if (v == 0.0) { iband = s0.iband; itet = s0.itet; isym = s0.isym; kx = s0.kx; ky = s0.ky; kz = s0.kz; }
-    if (flip) mirror(axis);
-  }
   EMC_HD EMC_INL bool neighbour(int& ii, int& jj, int& kk) const {
     ii = icell; jj = jcell; kk = kcell;
     switch (idir) {
@@ -658,7 +681,9 @@ struct Flight {
     if (par_type == 0) { Ex = sg * F.ex[c]; Ey = sg * F.ey[c]; Ez = sg * F.ez[c]; }
     else { Ex = sg * F.hx[c]; Ey = sg * F.hy[c]; Ez = sg * F.hz[c]; }
   }
-  EMC_HD EMC_INL bool HitCell() {
+  // a Diffuse at an oxide face is left to the caller (diffuse = true): it reads no
+  // cell data, so running it after load_cell() changes nothing
+  EMC_HD EMC_INL bool HitCell(bool& diffuse) {
     const int imot = M.motion[cell(icell, jcell, kcell) * 6 + idir];
     int ii, jj, kk;
     const bool inside = neighbour(ii, jj, kk);
@@ -685,7 +710,7 @@ struct Flight {
     } else if (imot == SCATTOX) {
       if (M.flag_surface) Reflect();
       else if (Random() > B.difpr[par_type]) Reflect();
-      else Diffuse();
+      else diffuse = true;
     } else if (imot == PERIOD) {
       if (idir == UP) x = M.lx[icell + 1];
       else if (idir == DOWN) x = M.lx[icell];
@@ -843,19 +868,22 @@ struct Flight {
       if (Tf < MY_ZERO) { Flag_Catch = 1; drop_reason = 0; break; }
       advance(L, Tf, gam);
       if (energy < 0) { Flag_Catch = 1; drop_reason = 1; break; }
+      int post = POST_NONE;
+      Saved s0{};
       switch (flag) {
         case 1:
           HitTet();
           L.getTet = L.getCell = true;
           break;
-        case 2:
-          if (HitCell()) L.getTet = L.getCell = true;
+        case 2: {
+          bool diffuse = false;
+          if (HitCell(diffuse)) L.getTet = L.getCell = true;
           else { Flag_Catch = 1; drop_reason = 4; }
+          if (diffuse) post = prep_diffuse(s0);
           break;
+        }
         case 3:
-          PhononScatter();
-          if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_phonon++; }
-          L.getPh = true;
+          post = prep_phonon(s0);
           break;
         case 4:
           if (par_type == PELEC) ElectronImpurityScatter();
@@ -866,16 +894,25 @@ struct Flight {
           // ParticleSurfaceScatter (src/ParSurfaceScatter.cpp:48-61)
           if (Random() * SurfScGamma < SurfRoughnessScRate) {
             idir = M.surface_dir[M.nearest_surf[c]];
-            Diffuse();
+            post = prep_diffuse(s0);
           } else {
             Flag_SelfScatter = 0;
-            select_process(Random() * calscattsum(energy, iband));
+            post = select_prep(Random() * calscattsum(energy, iband), s0);
           }
-          if (!Flag_SelfScatter) L.getTet = L.getCell = true;
-          L.getSurf = true;
           break;
         default:
           break;
+      }
+      if (post != POST_NONE) {
+        get_state();
+        finish(post, s0);
+      }
+      if (flag == 3) {
+        if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_phonon++; }
+        L.getPh = true;
+      } else if (flag == 5) {
+        if (!Flag_SelfScatter) L.getTet = L.getCell = true;
+        L.getSurf = true;
       }
       if (Flag_Catch) break;
     }

@@ -34,6 +34,7 @@ struct FlyArgs {
   unsigned long long dead_cap;
   uint32_t* slow;              // [cap] particles k_fly_fast hands to k_fly_slow
   unsigned long long* nslow;   // device counter, zero before the flight
+  long long wave_slots;        // waves the GPU runs at once (CUs x 4 SIMDs)
 };
 
 // Deposit tiles: TILE_I x TILE_J x TILE_K cells (i, j, k) per workgroup; a tile's

@@ -109,64 +109,80 @@ __global__ __launch_bounds__(256) void k_fly_fast(FlyArgs a) {
   if (lane == 0 && ev) atomicAdd(&a.T.counters[CNT_EVENTS], (unsigned long long)ev);
 }
 
-// Full event loop for the particles k_fly_fast listed; one lane per entry.
+// Full event loop for the particles k_fly_fast listed.  These histories diverge
+// (scattering, zone faces, contacts), and a wave runs its lanes' different paths
+// one after the other, so the list is spread thinly: each wave takes `lpw` entries
+// (lanes 0..lpw-1), with lpw chosen so that the list covers about a.wave_slots
+// waves, the number the GPU runs at once.
 __global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
   long long ev = 0, ph = 0, im = 0, ca = 0;
   const long long ns = (long long)*a.nslow;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ns; e += stride) {
-    const long long q = a.slow[e];
-    const uint32_t pk = a.P.pk[q];
-    Flight f(a.C->B, a.C->M, a.C->F, a.T);
-    load_particle(f, a, q, pk);
-    f.fly();
-    ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
-    if (f.Flag_Catch) {
-      ++ca;
-      finish_drop(a.T, f);
-      a.P.pk[q] = PK_DEAD;
-      const unsigned long long d = atomicAdd(a.ndead, 1ull);
-      if (d < a.dead_cap) a.dead[d] = q;
-      continue;
+  const long long slots = a.wave_slots > 0 ? a.wave_slots : 1;
+  const long long lpw = ns <= slots ? 1 : (ns + slots - 1) / slots < 64 ? (ns + slots - 1) / slots : 64;
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  if (lane < lpw) {
+    for (long long e = wave * lpw + lane; e < ns; e += nwaves * lpw) {
+      const long long q = a.slow[e];
+      const uint32_t pk = a.P.pk[q];
+      Flight f(a.C->B, a.C->M, a.C->F, a.T);
+      load_particle(f, a, q, pk);
+      f.fly();
+      ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
+      if (f.Flag_Catch) {
+        ++ca;
+        finish_drop(a.T, f);
+        a.P.pk[q] = PK_DEAD;
+        const unsigned long long d = atomicAdd(a.ndead, 1ull);
+        if (d < a.dead_cap) a.dead[d] = q;
+        continue;
+      }
+      a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
+      a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
+      a.P.energy[q] = f.energy;
+      a.P.pk[q] = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
+      a.P.itet[q] = f.itet; a.P.isym[q] = f.isym;
     }
-    a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
-    a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
-    a.P.energy[q] = f.energy;
-    a.P.pk[q] = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
-    a.P.itet[q] = f.itet; a.P.isym[q] = f.isym;
   }
   flush_counters(a.T, ev, ph, im, ca);
 }
 
-// Clones [begin, end) of the clone buffer fly the remaining time of the step.
-__global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long long begin, unsigned long long end) {
+// Clones [begin, end) of the clone buffer fly the remaining time of the step;
+// `lpw` clones per wave, as in k_fly_slow.
+__global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long long begin, unsigned long long end,
+                                                    int lpw) {
   long long ev = 0, ph = 0, im = 0, ca = 0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long q = (long long)begin + blockIdx.x * blockDim.x + threadIdx.x; q < (long long)end; q += stride) {
-    CloneRec& c = a.clones[q];
-    Flight f(a.C->B, a.C->M, a.C->F, a.T);
-    f.clones = a.clones;
-    f.nclones = a.nclones;
-    f.clone_cap = a.clone_cap;
-    f.x = c.x; f.y = c.y; f.z = c.z; f.kx = c.kx; f.ky = c.ky; f.kz = c.kz;
-    f.energy = c.energy; f.charge = c.charge; f.left_time = c.left_time;
-    f.icell = pk_i(c.pk); f.jcell = pk_j(c.pk); f.kcell = pk_k(c.pk); f.par_type = pk_type(c.pk);
-    f.itet = c.itet; f.isym = c.isym; f.key = c.key; f.step = a.step;
-    f.nclone = 0; f.flag_heat = a.flag_heat;
-    f.n_events = 0; f.n_phonon = 0; f.n_impurity = 0;
-    f.rng.init(f.key, a.step);
-    f.fly();
-    ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
-    if (f.Flag_Catch) {
-      ++ca;
-      finish_drop(a.T, f);
-      c.pk = PK_DEAD;
-      continue;
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  if (lane < lpw) {
+    for (long long q = (long long)begin + wave * lpw + lane; q < (long long)end; q += nwaves * lpw) {
+      CloneRec& c = a.clones[q];
+      Flight f(a.C->B, a.C->M, a.C->F, a.T);
+      f.clones = a.clones;
+      f.nclones = a.nclones;
+      f.clone_cap = a.clone_cap;
+      f.x = c.x; f.y = c.y; f.z = c.z; f.kx = c.kx; f.ky = c.ky; f.kz = c.kz;
+      f.energy = c.energy; f.charge = c.charge; f.left_time = c.left_time;
+      f.icell = pk_i(c.pk); f.jcell = pk_j(c.pk); f.kcell = pk_k(c.pk); f.par_type = pk_type(c.pk);
+      f.itet = c.itet; f.isym = c.isym; f.key = c.key; f.step = a.step;
+      f.nclone = 0; f.flag_heat = a.flag_heat;
+      f.n_events = 0; f.n_phonon = 0; f.n_impurity = 0;
+      f.rng.init(f.key, a.step);
+      f.fly();
+      ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
+      if (f.Flag_Catch) {
+        ++ca;
+        finish_drop(a.T, f);
+        c.pk = PK_DEAD;
+        continue;
+      }
+      c.x = f.x; c.y = f.y; c.z = f.z; c.kx = f.kx; c.ky = f.ky; c.kz = f.kz;
+      c.energy = f.energy;
+      c.pk = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
+      c.itet = f.itet; c.isym = f.isym;
     }
-    c.x = f.x; c.y = f.y; c.z = f.z; c.kx = f.kx; c.ky = f.ky; c.kz = f.kz;
-    c.energy = f.energy;
-    c.pk = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
-    c.itet = f.itet; c.isym = f.isym;
   }
   flush_counters(a.T, ev, ph, im, ca);
 }
@@ -869,17 +885,23 @@ static int grid_for(long long n, int bs, int maxg = 65536) {
   return (int)g;
 }
 
-// a.nslow must read zero on entry; the slow grid covers every particle, blocks
-// past the listed count exit at once
+// a.nslow must read zero on entry.  The slow list is short and its length is only
+// known on the device: k_fly_slow runs one wave per SIMD (a.wave_slots waves) and
+// strides over the list, so no empty workgroups of this large kernel are queued.
 hipError_t emc_launch_fly(const FlyArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fly_fast, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_fly_slow, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  const long long slow_blocks = a.wave_slots > 4 ? a.wave_slots / 4 : 1;
+  hipLaunchKernelGGL(k_fly_slow, dim3((unsigned)slow_blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_fly_clones(const FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s) {
   if (e <= b) return hipSuccess;
-  hipLaunchKernelGGL(k_fly_clones, dim3(grid_for((long long)(e - b), 256)), dim3(256), 0, s, a, b, e);
+  const long long n = (long long)(e - b);
+  const long long slots = a.wave_slots > 0 ? a.wave_slots : 1;
+  const int lpw = (int)(n <= slots ? 1 : ((n + slots - 1) / slots < 64 ? (n + slots - 1) / slots : 64));
+  const long long waves = (n + lpw - 1) / lpw;
+  hipLaunchKernelGGL(k_fly_clones, dim3(grid_for(waves * 64, 256)), dim3(256), 0, s, a, b, e, lpw);
   return hipGetLastError();
 }
 hipError_t emc_launch_merge(const FlyArgs& a, unsigned long long nclones, unsigned long long ndead,

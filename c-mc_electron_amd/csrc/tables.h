@@ -35,8 +35,17 @@ struct alignas(128) CornerRec {
   double x[4], y[4], z[4], e[4];
 };
 
+// A sphere enclosing a tetrahedron grown by the containment tolerance (every
+// point that Flight::contains accepts lies inside it): the tetrahedron searches
+// test it first and run the exact plane test only on its hits.
+struct alignas(32) Sphere {
+  double cx, cy, cz, r2;
+};
+
 struct BandDev {
   const TetRec* tr;
+  const Sphere* tsph;                         // [nt]
+  const Sphere *tlist_s, *tlists_s, *slist_s; // aligned with tlist, tlists, slist
   const SurfRec* sr;
   const CornerRec* cr;
   const int* tet;  // [4][nt]

@@ -151,7 +151,7 @@ class Engine:
                      ("energy", 7501), ("dos", 8 * 7501), ("sumdos", 7501 * 3), ("dostetmax", 8 * 7501),
                      ("sumscatt", 7501 * 8), ("scatte", 14 * 16), ("dose", 14 * 4 * 7501),
                      ("scattiie", 16 * 7501), ("scatth", 4 * 9), ("dosh", 4 * 3 * 7501), ("scattiih", 9 * 7501),
-                     ("xbz", 240), ("ybz", 240), ("zbz", 240)):
+                     ("xbz", 240), ("ybz", 240), ("zbz", 240), ("tsphere", 4 * nt)):
             v[f] = _view(getattr(b, f), n, np.float64)
         for f, n in (("tet", 4 * nt), ("neib", 4 * nt), ("ibt", nt), ("matsym", 288), ("indmat", 729),
                      ("newisym", 240), ("ntlist", 325 * 8), ("ptlist", 325 * 8), ("tlist", b.tlist_len),

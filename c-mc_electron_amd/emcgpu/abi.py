@@ -55,6 +55,7 @@ class BandView(C.Structure):
         ("energy", _d), ("dos", _d), ("sumdos", _d), ("dostetmax", _d), ("sumscatt", _d),
         ("scatte", _d), ("dose", _d), ("scattiie", _d), ("scatth", _d), ("dosh", _d), ("scattiih", _d),
         ("DOSMAX", C.c_double * 3),
+        ("tsphere", _d),
     ]
 
 

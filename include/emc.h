@@ -110,6 +110,9 @@ typedef struct emc_band_view {
   const double *dosh;                   /* [4][3][7501]                            */
   const double *scattiih;               /* [3][3][7501]                            */
   double DOSMAX[3];
+  const double *tsphere;               /* [nt][4] centre x y z and radius^2 of the sphere
+                                           enclosing each tetrahedron grown by MY_ZERO
+                                           (prefilter of the tetrahedron searches)   */
 } emc_band_view;
 
 /* Host view of the mesh.  cell c = (j*cnx + i)*cnz + k ; node n = (j*pnx + i)*pnz + k. */

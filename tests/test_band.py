@@ -198,3 +198,38 @@ def test_intrinsic_density(tabs):
     eng = tabs["_eng"]
     ni = tabs["_view"].Ni * eng.mesh.conc0 * 1e-6  # cm^-3
     assert 5e9 < ni < 2e10
+
+
+def test_search_spheres_enclose_grown_tetrahedra(tabs):
+    """The tetrahedron searches skip a candidate whose sphere does not hold the point.
+    That is exact only if every point the containment test accepts (planes
+    n.k - d >= -1e-13) lies inside the sphere: check the corners of every
+    tetrahedron grown by the tolerance, and random points of the grown tetrahedra."""
+    v = tabs["_view"]
+    nt = v.nt
+    sph = tabs["tsphere"].reshape(nt, 4)
+    dn = tabs["dnt"].reshape(4, 4, nt)          # [comp][face][nt]
+    P, _ = corners(tabs, nt)
+    assert np.all(np.isfinite(sph[:, 3])) and np.all(sph[:, 3] > 0)
+    rng = np.random.default_rng(5)
+    sel = rng.integers(0, nt, 4000)
+    for it in sel:
+        n = dn[:3, :, it].T                      # [face][3]
+        d = dn[3, :, it] - 1e-13
+        grown = []
+        for i in range(4):
+            f = [q for q in range(4) if q != i]
+            grown.append(np.linalg.solve(n[f], d[f]))
+        grown = np.array(grown)
+        c = sph[it, :3]
+        assert np.all(((grown - c) ** 2).sum(axis=1) <= sph[it, 3]), it
+        # random convex combinations of the grown corners satisfy the planes
+        w = rng.dirichlet(np.ones(4), 8)
+        pts = w @ grown
+        assert np.all(((pts - c) ** 2).sum(axis=1) <= sph[it, 3])
+        inside = np.all(pts @ n.T - dn[3, :, it] >= -1e-13 * (1 + 1e-6), axis=1)
+        assert inside.all()
+    # the ungrown corners are strictly inside
+    for it in sel[:200]:
+        cpts = np.array([P[k][it] for k in range(4)])
+        assert np.all(((cpts - sph[it, :3]) ** 2).sum(axis=1) < sph[it, 3])
