@@ -221,7 +221,21 @@ struct Flight {
   // the sphere array runs alongside the list, so most candidates cost 32 bytes
   // read in order instead of a dependent load of their record
   EMC_HD EMC_INL bool scan(const int* list, const Sphere* sph, int p0, int nn, double a, double b, double c) {
-    for (int itl = p0; itl <= p0 + nn - 1; ++itl) {
+    const int end = p0 + nn;
+    int itl = p0;
+    // eight sphere tests per round, their loads independent of each other
+    for (; itl + 8 <= end; itl += 8) {
+      bool h[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h[u] = in_sphere(sph[itl + u], a, b, c);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (h[u]) {
+          const int it = list[itl + u];
+          if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return true; }
+        }
+    }
+    for (; itl < end; ++itl) {
       if (!in_sphere(sph[itl], a, b, c)) continue;
       const int it = list[itl];
       if (iband == B.tr[it].ibt && contains(it, a, b, c)) { itet = it; return true; }
