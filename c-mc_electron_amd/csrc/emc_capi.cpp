@@ -383,7 +383,6 @@ static int gpu_setup(emc_ctx* c) {
     A.cinv = m.dup(h.cinv);
     A.diag = g.diag; A.cx = g.cx; A.cy = g.cy; A.cz = g.cz;
     A.done = g.done;
-    A.lb = A.nl;
     for (int l = 0; l < A.nl; ++l) {
       const MgLevel& L = h.lv[l];
       MgLevelDev& D = A.L[l];
@@ -395,13 +394,11 @@ static int gpu_setup(emc_ctx* c) {
         for (int o = 13; o < 27; ++o)
           for (size_t q = 0; q < L.n; ++q) c14[(size_t)(o - 13) * L.n + q] = (float)L.a27[(size_t)o * L.n + q];
         D.c14 = m.dup(c14);
-        if (A.lb == A.nl && L.n <= (size_t)MG_BOTTOM_NODES) A.lb = l;
       }
       D.wdinv = m.dup(L.wdinv);
       D.x = m.alloc<double>(L.n); D.t = m.alloc<double>(L.n); D.r = m.alloc<double>(L.n);
       D.b = l > 0 ? m.alloc<double>(L.n) : g.r;
     }
-    if (A.nl > 1 && A.lb == A.nl) A.lb = A.nl - 1;
   }
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));

@@ -108,8 +108,6 @@ struct PcgArgs {
 };
 
 constexpr int MG_MAXL = 10;
-// levels with at most this many nodes run inside one workgroup (k_mg_bottom)
-constexpr int MG_BOTTOM_NODES = 2048;
 
 struct MgLevelDev {
   int nx, ny, nz, pad;
@@ -127,8 +125,6 @@ struct MgLevelDev {
 struct MgArgs {
   MgLevelDev L[MG_MAXL];
   int nl, nc;                          // levels, nodes of the coarsest level
-  int lb;                              // first level handled by k_mg_bottom (nl: none)
-  int pad;
   const double* cinv;                  // [nc*nc] coarsest inverse, row major
   const double *diag, *cx, *cy, *cz;   // level-0 operator
   const int* done;

@@ -630,16 +630,12 @@ __global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, const double* rz
 }
 
 // ------------------------------------------------------------------ multigrid V-cycle
-// The phases of the cycle are written once per node and run either over the whole
-// grid (one launch per phase, large levels) or inside k_mg_bottom, one workgroup
-// that walks all small levels with barriers between phases.
+// The phases of the cycle are written once per node (grid-stride loops, one
+// launch per phase).  A single workgroup walking the small levels was measured
+// slower (latency bound) than one launch per phase.
 struct GridLoop {
   __device__ static size_t start() { return (size_t)blockIdx.x * blockDim.x + threadIdx.x; }
   __device__ static size_t stride() { return (size_t)gridDim.x * blockDim.x; }
-};
-struct BlockLoop {
-  __device__ static size_t start() { return threadIdx.x; }
-  __device__ static size_t stride() { return blockDim.x; }
 };
 
 __device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, int& k) {
@@ -777,7 +773,8 @@ __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out,
   }
 }
 
-// result vector of level l after its part of the cycle
+// result vector of level l after its part of the cycle (the coarsest level is
+// solved directly into x, the others end their post-smoothing in t)
 __device__ __host__ inline const double* mg_result(const MgArgs& m, int l) {
   return l == m.nl - 1 ? m.L[l].x : m.L[l].t;
 }
@@ -831,28 +828,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, double* z, doub
   block_reduce3(rz, 0.0, 0.0, part_rz);
 }
 
-// Levels lb .. nl-1 in one workgroup: down-sweep, dense coarsest solve, up-sweep.
-__global__ __launch_bounds__(1024) void k_mg_bottom(MgArgs m) {
+// coarsest level: x = C^-1 b, one wave per row
+__global__ __launch_bounds__(256) void k_mg_coarse(MgArgs m) {
   if (*m.done) return;
-  const int nl = m.nl, lb = m.lb;
-  for (int l = lb; l < nl - 1; ++l) {
-    ph_pre2<false, BlockLoop>(m, m.L[l]);
-    __syncthreads();
-    ph_resid<false, BlockLoop>(m, m.L[l]);
-    __syncthreads();
-    ph_restrict<BlockLoop>(m.L[l], m.L[l + 1]);
-    __syncthreads();
-  }
-  ph_coarse(m, m.L[nl - 1].b, m.L[nl - 1].x, threadIdx.x >> 6, blockDim.x >> 6);
-  __syncthreads();
-  for (int l = nl - 2; l >= lb; --l) {
-    ph_prolong<BlockLoop>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
-    __syncthreads();
-    ph_sweep<false, BlockLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
-    __syncthreads();
-    ph_sweep<false, BlockLoop>(m, m.L[l], m.L[l].x, m.L[l].t);
-    __syncthreads();
-  }
+  const MgLevelDev& L = m.L[m.nl - 1];
+  ph_coarse(m, L.b, L.x, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
 
 // single-level hierarchy: z = C^-1 r over the grid, then the partial r.z
@@ -945,7 +925,7 @@ static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hi
     hipLaunchKernelGGL(k_mg_dot_rz, dim3(a.nblocks), dim3(256), 0, s, m, (const double*)a.z, part_rz);
     return;
   }
-  for (int l = 0; l < m.lb; ++l) {
+  for (int l = 0; l < nl - 1; ++l) {
     const dim3 g(grid_for((long long)m.L[l].n, 256));
     if (l == 0) {
       hipLaunchKernelGGL(k_mg_pre2<true>, g, dim3(256), 0, s, m, l);
@@ -956,8 +936,8 @@ static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hi
     }
     hipLaunchKernelGGL(k_mg_restrict, dim3(grid_for((long long)m.L[l + 1].n, 256)), dim3(256), 0, s, m, l);
   }
-  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(1024), 0, s, m);
-  for (int l = m.lb - 1; l >= 0; --l) {
+  hipLaunchKernelGGL(k_mg_coarse, dim3(grid_for((long long)m.nc * 64, 256)), dim3(256), 0, s, m);
+  for (int l = nl - 2; l >= 0; --l) {
     const dim3 g(grid_for((long long)m.L[l].n, 256));
     hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m, l);
     if (l == 0) {
