@@ -69,7 +69,7 @@ __device__ inline void load_particle(Flight& f, const FlyArgs& a, long long q, u
 // k_fly_slow, which restarts them from the untouched state: the RNG is counter
 // based, so the full event loop draws the same numbers and the result equals one
 // pass of fly().
-__global__ __launch_bounds__(256) void k_fly_fast(FlyArgs a) {
+__global__ __launch_bounds__(256, 3) void k_fly_fast(FlyArgs a) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   long long ev = 0;
   const int lane = threadIdx.x & 63;
