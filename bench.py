@@ -72,6 +72,71 @@ def cpu_baseline(eng, steps):
     return n_ps / dt, dt, n_ps
 
 
+FLY_KERNELS = ("k_fly_fast", "k_fly_slow")
+
+
+def _pmc_pass(counter, args, outdir):
+    """One rocprofv3 counter pass over a child run (this process has not touched
+    the GPU yet).  Returns {kernel name: [values per dispatch]} or None."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    cmd = ["timeout", "-s", "KILL", "150", rp, "--pmc", counter, "--output-format", "csv", "-d", outdir,
+           "-o", "run", "--", sys.executable, os.path.abspath(__file__)] + args
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=env, cwd="/tmp")
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        print(f"[bench] {counter} pass gave no counters (rc {r.returncode})", file=sys.stderr, flush=True)
+        return None
+    out = {}
+    for row in csv.DictReader(open(files[0])):
+        if row["Counter_Name"] == counter:
+            out.setdefault(row["Kernel_Name"].split("(")[0].split("::")[-1], []).append(float(row["Counter_Value"]))
+    return out
+
+
+def measure_traffic(bench_args):
+    """HBM bytes of one flight (k_fly_fast + k_fly_slow) from FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reports for a 512 MiB
+    device-to-device copy (the calibration)."""
+    tmp = tempfile.mkdtemp(prefix="emc_pmc_")
+    mib = 512
+    cf = _pmc_pass("FETCH_SIZE", ["--copy-probe", str(mib)], os.path.join(tmp, "cf"))
+    cw = _pmc_pass("WRITE_SIZE", ["--copy-probe", str(mib)], os.path.join(tmp, "cw"))
+    sf = _pmc_pass("FETCH_SIZE", bench_args, os.path.join(tmp, "sf"))
+    sw = _pmc_pass("WRITE_SIZE", bench_args, os.path.join(tmp, "sw"))
+    if not (cf and cw and sf and sw):
+        return None
+
+    def copy_kib(d):
+        v = [x for k, vals in d.items() if "copyBuffer" in k for x in vals if x > 1024.0]
+        return max(v) if v else None
+
+    kib = mib * 1024.0
+    f_copy, w_copy = copy_kib(cf), copy_kib(cw)
+    if not f_copy or not w_copy:
+        return None
+    fscale, wscale = kib / f_copy, kib / w_copy
+
+    def per_step(d):
+        tot = 0.0
+        for k in FLY_KERNELS:
+            vals = d.get(k, [])
+            if not vals:
+                return None
+            tot += vals[-1]  # the last (timed) step
+        return tot
+
+    f_fly, w_fly = per_step(sf), per_step(sw)
+    if f_fly is None or w_fly is None:
+        return None
+    return {"bytes": (f_fly * fscale + w_fly * wscale) * 1024.0, "fetch_kib": f_fly, "write_kib": w_fly,
+            "fetch_scale": fscale, "write_scale": wscale}
+
+
 def copy_probe(mib):
     """A device-to-device copy of known size: the bytes the HBM counters should report for
     it calibrate the traffic counted for the engine's kernels."""
@@ -94,6 +159,8 @@ def main():
     ap.add_argument("--holes", type=int, default=500000)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes (roofline.traffic = null)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -103,6 +170,13 @@ def main():
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    traffic = None
+    if not a.no_traffic:
+        # before this process touches the GPU: the counter passes run as children
+        traffic = measure_traffic(["--steps", "1", "--warmup", "1", "--no-cpu", "--no-traffic",
+                                   "--refine", str(a.refine), "--electrons", str(a.electrons),
+                                   "--holes", str(a.holes)])
+        print(f"[bench] traffic {traffic}", file=sys.stderr, flush=True)
     tmp = tempfile.mkdtemp(prefix="emc_bench_")
     inp = make_workload(a.refine, a.electrons, a.holes, tmp)
     print(f"[bench] deck {inp}", file=sys.stderr, flush=True)
@@ -168,10 +242,12 @@ def main():
                    "pcg_iters_mean": float(np.mean(pcg_it)),
                    "full_event_loop_fraction": full_loop / max(1, n_ps),
                    "events_per_particle_step": events / max(1, n_ps)},
-        "roofline": {"bound": "hbm", "kernel": "k_fly", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_fly_fast + k_fly_slow", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": None,
-                     "bytes_per_particle": FLY_BYTES_PER_PARTICLE},
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_counters": traffic,
+                     "bytes_per_particle": FLY_BYTES_PER_PARTICLE,
+                     "algorithmic_bytes": FLY_BYTES_PER_PARTICLE * n_avg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

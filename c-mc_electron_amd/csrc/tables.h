@@ -8,20 +8,22 @@
 
 namespace emc {
 
-// One tetrahedron's data read on every flight event, contiguous (208 B):
-// the four inward face planes (nx, ny, nz, d) of Band::HESSTET, the group
-// velocity and base energy of Band::VELOTET, the variable-Gamma bound gamtet,
-// the top-corner energy and the neighbour codes NeibrTet.
-struct alignas(16) TetRec {
+// One tetrahedron's data read on every flight event, contiguous: the four
+// inward face planes (nx, ny, nz, d) of Band::HESSTET, the group velocity and
+// base energy of Band::VELOTET, the variable-Gamma bound gamtet, the top-corner
+// energy and the neighbour codes NeibrTet.  Padded to 256 B and aligned to the
+// 128 B cache line, so a record is exactly two lines (208 B unpadded straddle
+// two or three).
+struct alignas(128) TetRec {
   double n[4][4];
   double vt[4];
   double gam;
   double e3;
   int neib[4];
   int ibt;
-  int pad[3];
+  int pad[15];
 };
-static_assert(sizeof(TetRec) == 208, "TetRec layout");
+static_assert(sizeof(TetRec) == 256, "TetRec layout");
 
 // What one try of get_state's rejection loop reads, in one 64-byte line: corner
 // energies, the FASTSURF area factors and |v| of the tetrahedron.
