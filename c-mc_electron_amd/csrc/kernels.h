@@ -121,7 +121,8 @@ struct MgLevelDev {
   double *x, *t, *r, *b; // work vectors; b of level 0 is the CG residual
 };
 
-// One V(2,2) cycle of the hierarchy built by mg_build (multigrid.h).
+// One multigrid cycle (V(2,2) on level 0, V(1,1) below) of the hierarchy built
+// by mg_build (multigrid.h).
 struct MgArgs {
   MgLevelDev L[MG_MAXL];
   int nl, nc;                          // levels, nodes of the coarsest level

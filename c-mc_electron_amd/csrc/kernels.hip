@@ -11,7 +11,8 @@
 //   k_rhs        node density and Poisson right-hand side (eliminated system)
 //   k_field      cell fields from node potential, Rho clamp, surface rates
 //   k_pcg_*      CG on the 7-point operator, preconditioned by
-//   k_mg_*       one multigrid V(2,2) cycle (damped Jacobi, Galerkin coarse levels)
+//   k_mg_*       one multigrid cycle: V(2,2) on the finest level, V(1,1) below
+//                (damped Jacobi, Galerkin coarse levels, dense coarsest solve)
 //
 // All HBM-bound integer/FP64 work: no MFMA.  Compiled with -ffp-contract=off so
 // the FP64 results match the CPU restatement bit for bit.
@@ -715,6 +716,17 @@ __device__ inline void ph_resid(const MgArgs& m, const MgLevelDev& L) {
   }
 }
 
+// r = b - A x1 with x1 = wdinv b: one Jacobi sweep from zero and its residual
+template <bool FINE, class LP>
+__device__ inline void ph_resid1(const MgArgs& m, const MgLevelDev& L) {
+  auto x1 = [&](size_t q) { return L.wdinv[q] * L.b[q]; };
+  for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
+    int i, j, k;
+    lvl_ijk(L, n, i, j, k);
+    L.r[n] = L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k);
+  }
+}
+
 __device__ inline double pw1(int f, int C) { return f == 2 * C ? 1.0 : 0.5; }
 
 // b_C = P^T r_F: coarse node I gathers the fine residuals around fine node 2I
@@ -735,11 +747,16 @@ __device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
   }
 }
 
-// t_F += P e on the unknown nodes (wdinv > 0)
-template <class LP>
+// t_F += P e on the unknown nodes (wdinv > 0); with FROM_ZERO the smoothed value
+// before the correction is the sweep from zero, t_F = wdinv b + P e (fixed
+// nodes 0)
+template <class LP, bool FROM_ZERO = false>
 __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, const double* e) {
   for (size_t n = LP::start(); n < F.n; n += LP::stride()) {
-    if (F.wdinv[n] == 0.0) continue;
+    if (F.wdinv[n] == 0.0) {
+      if (FROM_ZERO) F.t[n] = 0.0;
+      continue;
+    }
     int i, j, k;
     lvl_ijk(F, n, i, j, k);
     double v = 0;
@@ -755,7 +772,8 @@ __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, cons
         }
       }
     }
-    F.t[n] += v;
+    if (FROM_ZERO) F.t[n] = F.wdinv[n] * F.b[n] + v;
+    else F.t[n] += v;
   }
 }
 
@@ -773,38 +791,40 @@ __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out,
   }
 }
 
-// result vector of level l after its part of the cycle (the coarsest level is
-// solved directly into x, the others end their post-smoothing in t)
-__device__ __host__ inline const double* mg_result(const MgArgs& m, int l) {
-  return l == m.nl - 1 ? m.L[l].x : m.L[l].t;
-}
+// Cycle shape: V(2,2) on level 0, V(1,1) on the coarser levels (the same CG
+// iteration count on the FinFET meshes, two launches fewer per level).  The
+// result of every level >= 1 ends in x (the coarsest is solved into x).
+__device__ __host__ inline const double* mg_result(const MgArgs& m, int l) { return m.L[l].x; }
 
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_mg_pre2(MgArgs m, int l) {
+__global__ __launch_bounds__(256) void k_mg_pre2(MgArgs m) {
   if (*m.done) return;
-  ph_pre2<FINE, GridLoop>(m, m.L[l]);
+  ph_pre2<true, GridLoop>(m, m.L[0]);
 }
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m, int l) {
+__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m) {
   if (*m.done) return;
-  ph_resid<FINE, GridLoop>(m, m.L[l]);
+  ph_resid<true, GridLoop>(m, m.L[0]);
+}
+// levels >= 1: sweep from zero and residual in one pass
+__global__ __launch_bounds__(256) void k_mg_resid1(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_resid1<false, GridLoop>(m, m.L[l]);
 }
 __global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
   if (*m.done) return;
   ph_restrict<GridLoop>(m.L[l], m.L[l + 1]);
 }
-__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m, int l) {
+__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m) {
   if (*m.done) return;
-  ph_prolong<GridLoop>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
+  ph_prolong<GridLoop>(m.L[0], m.L[1], mg_result(m, 1));
 }
-// dst = src + wdinv (b - A src) on level l >= 1: t -> x, then x -> t
-__global__ __launch_bounds__(256) void k_mg_post2a(MgArgs m, int l) {
+__global__ __launch_bounds__(256) void k_mg_prolong1(MgArgs m, int l) {
+  if (*m.done) return;
+  ph_prolong<GridLoop, true>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
+}
+// levels >= 1: one sweep t -> x
+__global__ __launch_bounds__(256) void k_mg_post1(MgArgs m, int l) {
   if (*m.done) return;
   ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
-}
-__global__ __launch_bounds__(256) void k_mg_post2b(MgArgs m, int l) {
-  if (*m.done) return;
-  ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].x, m.L[l].t);
 }
 __global__ __launch_bounds__(256) void k_mg_post0a(MgArgs m) {
   if (*m.done) return;
@@ -928,24 +948,23 @@ static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hi
   for (int l = 0; l < nl - 1; ++l) {
     const dim3 g(grid_for((long long)m.L[l].n, 256));
     if (l == 0) {
-      hipLaunchKernelGGL(k_mg_pre2<true>, g, dim3(256), 0, s, m, l);
-      hipLaunchKernelGGL(k_mg_resid<true>, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_pre2, g, dim3(256), 0, s, m);
+      hipLaunchKernelGGL(k_mg_resid, g, dim3(256), 0, s, m);
     } else {
-      hipLaunchKernelGGL(k_mg_pre2<false>, g, dim3(256), 0, s, m, l);
-      hipLaunchKernelGGL(k_mg_resid<false>, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_resid1, g, dim3(256), 0, s, m, l);
     }
     hipLaunchKernelGGL(k_mg_restrict, dim3(grid_for((long long)m.L[l + 1].n, 256)), dim3(256), 0, s, m, l);
   }
   hipLaunchKernelGGL(k_mg_coarse, dim3(grid_for((long long)m.nc * 64, 256)), dim3(256), 0, s, m);
   for (int l = nl - 2; l >= 0; --l) {
     const dim3 g(grid_for((long long)m.L[l].n, 256));
-    hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m, l);
     if (l == 0) {
+      hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m);
       hipLaunchKernelGGL(k_mg_post0a, g, dim3(256), 0, s, m);
       hipLaunchKernelGGL(k_mg_sweep_last, dim3(a.nblocks), dim3(256), 0, s, m, a.z, part_rz);
     } else {
-      hipLaunchKernelGGL(k_mg_post2a, g, dim3(256), 0, s, m, l);
-      hipLaunchKernelGGL(k_mg_post2b, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_prolong1, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_post1, g, dim3(256), 0, s, m, l);
     }
   }
 }

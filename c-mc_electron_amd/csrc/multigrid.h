@@ -2,7 +2,8 @@
 //
 // The reference factorises the Poisson matrix once with KLU (src/PoissonSolver.cpp:
 // 48-78) and back-solves every step.  This build solves the same SPD system (contact
-// rows eliminated) with CG preconditioned by one V(2,2) cycle:
+// rows eliminated) with CG preconditioned by one multigrid cycle, V(2,2) on
+// level 0 and V(1,1) on the coarser levels:
 //   - level 0 is the 7-point operator (diag, cx, cy, cz) of the node grid;
 //   - level l+1 has floor(n_l/2)+1 nodes per axis, coarse node I sitting on fine
 //     node 2I; P is trilinear interpolation with the rows of fixed (Dirichlet or

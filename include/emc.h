@@ -219,7 +219,8 @@ int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell
 
 /* Poisson solve on the eliminated operator: rhs, sol of length nnode.  rhs holds
  * the reference right-hand side (compute_rhs); sol is the initial guess on entry.
- * CG preconditioned by one geometric multigrid V(2,2) cycle, to ||r|| <= rtol ||b||. */
+ * CG preconditioned by one geometric multigrid cycle (V(2,2) on the mesh, V(1,1)
+ * on the coarser levels), to ||r|| <= rtol ||b||. */
 int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
                       int *iters, double *relres);
 

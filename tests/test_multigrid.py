@@ -105,12 +105,14 @@ def vcycle(levels, ops, fixed, Ps, cinv, level, b):
         return cinv @ b
     D = A.diagonal()
     w = np.where(fixed[level], 0.0, levels[level][2] / D)
+    nu = 2 if level == 0 else 1      # the library's cycle: V(2,2) on level 0, V(1,1) below
     x = w * b
-    x = x + w * (b - A @ x)
+    for _ in range(nu - 1):
+        x = x + w * (b - A @ x)
     P = Ps[level]
     x = x + P @ vcycle(levels, ops, fixed, Ps, cinv, level + 1, P.T @ (b - A @ x))
-    x = x + w * (b - A @ x)
-    x = x + w * (b - A @ x)
+    for _ in range(nu):
+        x = x + w * (b - A @ x)
     return x
 
 
