@@ -40,7 +40,7 @@ struct FlyArgs {
 // Deposit tiles: TILE_I x TILE_J x TILE_K cells (i, j, k) per workgroup; a tile's
 // node block spans (TILE_I+1)(TILE_J+1)(TILE_K+1) nodes.  Tile index
 // t = (tj*ntx + ti)*ntz + tk follows the node order (k fastest, then i, then j).
-constexpr int TILE_I = 8, TILE_J = 8, TILE_K = 4;
+constexpr int TILE_I = 4, TILE_J = 8, TILE_K = 4;
 constexpr int TILE_NODES = (TILE_I + 1) * (TILE_J + 1) * (TILE_K + 1);
 constexpr int TILE_CELLS = TILE_I * TILE_J * TILE_K;
 constexpr int DEP_NQ = 11;  // node charge + 5 statistics x 2 carrier types

@@ -5,7 +5,7 @@
 //   k_fly_slow   one lane per listed particle, full-band event loop (fly_core.h)
 //   k_fly_clones same for GENERATE clones of this step
 //   k_merge      surviving clones -> holes left by caught particles, then append
-//   k_tile_*     counting sort of the particles by 8x8x4-cell tile (drops empty slots)
+//   k_tile_*     counting sort of the particles by 4x8x4-cell tile (drops empty slots)
 //   k_deposit_*  CIC charge, |q|-weighted node statistics, NGP cell sums: LDS per
 //                tile, then a per-node gather of the tile blocks (int64, no global atomics)
 //   k_rhs        node density and Poisson right-hand side (eliminated system)

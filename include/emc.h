@@ -212,7 +212,7 @@ int emc_fly_step(emc_ctx *ctx, uint64_t step, int flag_heat, emc_fly_tallies *ou
  *   node_stat[10][nnode] vxE vyE vzE |q|E E|q|E, then the hole five (EMC_FX_STAT)
  *   cell_q[3][ncell]    sum q electrons, sum q holes, sum |q| (EMC_FX_Q)
  *   cell_n[ncell]       particle count per cell (stat_enum)
- * The deposit first sorts the population on the device by 8x8x4-cell tile and
+ * The deposit first sorts the population on the device by 4x8x4-cell tile and
  * drops empty slots, so the order of emc_particles_download changes (the sums do
  * not: they are integer).                                                          */
 int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell_q, int64_t *cell_n);

@@ -105,8 +105,9 @@ def test_deposit_parity(eng):
     p2 = eng.download()
     assert p2.n == live.sum() and np.all(p2.i[:p2.n] >= 0)
     assert_same_population(p2, p.subset(np.nonzero(live)[0]), "tile sort")
-    tile = ((p2.j[:p2.n] // 8) * ((eng.mesh.cnx + 7) // 8) + p2.i[:p2.n] // 8) * ((eng.mesh.cnz + 3) // 4) + \
-        p2.k[:p2.n] // 4
+    TI, TJ, TK = 4, 8, 4   # deposit tile, cells along i, j, k
+    tile = ((p2.j[:p2.n] // TJ) * ((eng.mesh.cnx + TI - 1) // TI) + p2.i[:p2.n] // TI) * \
+        ((eng.mesh.cnz + TK - 1) // TK) + p2.k[:p2.n] // TK
     assert np.all(np.diff(tile) >= 0), "population not in tile order"
 
 
