@@ -18,6 +18,7 @@
 #include "fly_core.h"
 #include "kernels.h"
 #include "multigrid.h"
+#include "tetindex.h"
 
 using namespace emc;
 
@@ -137,6 +138,8 @@ struct emc_ctx {
   std::vector<SurfRec> surfrec;
   std::vector<CornerRec> cornerrec;
   std::vector<Sphere> tsph, tlist_s, tlists_s, slist_s;
+  TetIndex tix;         // empty when build_tet_index refused the tables
+  std::string tix_why;  // why it refused
   std::vector<int> surface_dir;
   HostParticles hp;
   std::unique_ptr<Gpu> gpu;
@@ -166,6 +169,9 @@ static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
   B.sr = c->surfrec.data();
   B.cr = c->cornerrec.data();
   B.tsph = c->tsph.data();
+  B.troot = c->tix.nodes.empty() ? nullptr : c->tix.roots.data();
+  B.tnode = c->tix.nodes.empty() ? nullptr : c->tix.nodes.data();
+  B.tent = c->tix.nodes.empty() ? nullptr : c->tix.ent.data();
   B.tlist_s = c->tlist_s.data(); B.tlists_s = c->tlists_s.data(); B.slist_s = c->slist_s.data();
   B.tet = b.tet.data();
   B.xkk = b.xkk.data(); B.ykk = b.ykk.data(); B.zkk = b.zkk.data(); B.eek = b.eek.data();
@@ -229,8 +235,12 @@ static void build_tetrec(emc_ctx* c) {
   // f != i (face f is opposite corner f).  Centre = centroid, radius = farthest
   // grown corner, widened by a relative 1e-9 for rounding.  A degenerate
   // tetrahedron gets an infinite radius (always tested exactly).
+  // The box of the grown corners, widened the same way, bounds the tetrahedron for
+  // the point-location index.
   c->tsph.assign(b.nt, Sphere{});
+  std::vector<TetBox> tbox(b.nt);
   for (int it = 0; it < b.nt; ++it) {
+    double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     const CornerRec& q = c->cornerrec[it];
     const double cx = 0.25 * (q.x[0] + q.x[1] + q.x[2] + q.x[3]);
     const double cy = 0.25 * (q.y[0] + q.y[1] + q.y[2] + q.y[3]);
@@ -262,9 +272,19 @@ static void build_tetrec(emc_ctx* c) {
       const double r2 = (vx - cx) * (vx - cx) + (vy - cy) * (vy - cy) + (vz - cz) * (vz - cz);
       if (!std::isfinite(r2)) { ok = false; break; }
       if (r2 > r2max) r2max = r2;
+      const double v[3] = {vx, vy, vz};
+      for (int a = 0; a < 3; ++a) {
+        blo[a] = std::min(blo[a], v[a]);
+        bhi[a] = std::max(bhi[a], v[a]);
+      }
     }
     const double r = ok ? std::sqrt(r2max) * (1.0 + 1e-9) + 1e-12 : HUGE_VAL;
     c->tsph[it] = Sphere{cx, cy, cz, ok ? r * r : HUGE_VAL};
+    for (int a = 0; a < 3; ++a) {
+      const double w = 1e-9 * (std::fabs(blo[a]) + std::fabs(bhi[a]) + (bhi[a] - blo[a])) + 1e-12;
+      tbox[it].lo[a] = ok ? blo[a] - w : -HUGE_VAL;
+      tbox[it].hi[a] = ok ? bhi[a] + w : HUGE_VAL;
+    }
   }
   auto along = [&](const std::vector<int>& list, std::vector<Sphere>& out) {
     out.resize(list.size());
@@ -273,6 +293,8 @@ static void build_tetrec(emc_ctx* c) {
   along(b.tlist, c->tlist_s);
   along(b.tlists, c->tlists_s);
   along(b.slist, c->slist_s);
+  c->tix_why = build_tet_index(b, tbox, c->tix);
+  if (!c->tix_why.empty()) c->tix = TetIndex{};
 }
 
 // Multigrid hierarchy of the Poisson preconditioner, built once per context.
@@ -307,6 +329,11 @@ static int gpu_setup(emc_ctx* c) {
   B.sr = m.dup(c->surfrec);
   B.cr = m.dup(c->cornerrec);
   B.tsph = m.dup(c->tsph);
+  if (!c->tix.nodes.empty()) {
+    B.troot = m.dup(c->tix.roots);
+    B.tnode = m.dup(c->tix.nodes);
+    B.tent = m.dup(c->tix.ent);
+  }
   B.tlist_s = m.dup(c->tlist_s); B.tlists_s = m.dup(c->tlists_s); B.slist_s = m.dup(c->slist_s);
   B.tet = m.dup(b.tet);
   B.xkk = m.dup(b.xkk); B.ykk = m.dup(b.ykk); B.zkk = m.dup(b.zkk); B.eek = m.dup(b.eek);
@@ -617,6 +644,13 @@ int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
   v->nk = b.nk; v->nt = b.nt; v->nbt = b.nbt;
   for (int q = 0; q < 3; ++q) { v->nband[q] = b.nband[q]; v->bandof[q] = b.bandof[q]; v->difpr[q] = b.difpr[q]; v->DOSMAX[q] = b.DOSMAX[q]; }
   v->tsphere = c->tsph.empty() ? nullptr : &c->tsph[0].cx;
+  v->tindex_nodes = (int)c->tix.nodes.size();
+  v->tindex_entries = (int)c->tix.ent.size();
+  if (!c->tix.nodes.empty()) {
+    v->tindex_root = c->tix.roots.data();
+    v->tindex_node = &c->tix.nodes[0].first;
+    v->tindex_entry = c->tix.ent.data();
+  }
   v->scpre = b.scpre; v->scprh = b.scprh;
   v->dtable = b.dtable; v->emax = b.emax; v->dlist = b.dlist; v->dlists = b.dlists; v->a0pi = b.a0pi; v->sieg = b.sieg;
   v->mell = b.mell; v->melt = b.melt; v->meld = b.meld;
@@ -636,6 +670,34 @@ int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
   v->energy = b.energy.data(); v->dos = b.dos.data(); v->sumdos = b.sumdos.data(); v->dostetmax = b.dostetmax.data();
   v->sumscatt = b.sumscatt.data(); v->scatte = b.scatte.data(); v->dose = b.dose.data(); v->scattiie = b.scattiie.data();
   v->scatth = b.scatth.data(); v->dosh = b.dosh.data(); v->scattiih = b.scattiih.data();
+  return EMC_OK;
+}
+
+int emc_band_locate(emc_ctx* c, int64_t n, const double* k, const double* energy, const int* iband, int* itet,
+                    int* isym) {
+  if (!c) return EMC_ERR_ARG;
+  if (n < 0 || (n > 0 && (!k || !energy || !iband || !itet || !isym))) return fail(c, EMC_ERR_ARG, "band_locate: bad arguments");
+  const Band& b = c->band;
+  for (int64_t q = 0; q < n; ++q)
+    if (iband[q] < 0 || iband[q] >= MNB || itet[q] < 0 || itet[q] >= b.nt)
+      return fail(c, EMC_ERR_ARG, "band_locate: band or tetrahedron index out of range");
+  BandDev B{};
+  fill_band_dev_host(c, B);
+  MeshDev M{};
+  fill_mesh_dev_scalars(c, M);
+  FieldsDev F{};
+  TallyDev T{};
+  Flight f(B, M, F, T);
+  for (int64_t q = 0; q < n; ++q) {
+    f.kx = k[3 * q]; f.ky = k[3 * q + 1]; f.kz = k[3 * q + 2];
+    f.energy = energy[q];
+    f.iband = iband[q];
+    f.itet = itet[q];
+    f.Flag_Catch = 0;
+    f.Getisym();
+    itet[q] = f.Flag_Catch ? -1 : f.itet;
+    isym[q] = f.isym;
+  }
   return EMC_OK;
 }
 

@@ -242,6 +242,53 @@ struct Flight {
     }
     return false;
   }
+  // The answer of Getisym's list scans from the band's point-location octree: the
+  // leaf holding (a, b, c) lists, in descending tetrahedron index, every tetrahedron
+  // of the band that can hold the point, with its bucket ranges.  The first holding
+  // entry in a bucket is what that bucket's scan returns (buckets run in descending
+  // index), the last holding entry what the scan over all tetrahedra returns; the
+  // buckets are tried in the scans' order: tlists bin, tlist bins ie, ie+1, ie-1.
+  EMC_HD EMC_INL bool locate(double a, double b, double c) {
+    const TetIdxRoot& R = B.troot[iband];
+    if (!(a >= R.lo[0] && a <= R.hi[0] && b >= R.lo[1] && b <= R.hi[1] && c >= R.lo[2] && c <= R.hi[2]))
+      return false;  // outside every tetrahedron of the band
+    double l0 = R.lo[0], l1 = R.lo[1], l2 = R.lo[2], h0 = R.hi[0], h1 = R.hi[1], h2 = R.hi[2];
+    TetIdxNode nd = B.tnode[R.node];
+    while (nd.count < 0) {
+      const double m0 = 0.5 * (l0 + h0), m1 = 0.5 * (l1 + h1), m2 = 0.5 * (l2 + h2);
+      int ch = 0;
+      if (a >= m0) { ch |= 1; l0 = m0; } else h0 = m0;
+      if (b >= m1) { ch |= 2; l1 = m1; } else h1 = m1;
+      if (c >= m2) { ch |= 4; l2 = m2; } else h2 = m2;
+      nd = B.tnode[nd.first + ch];
+    }
+    int ies = (int)(energy * B.dlists);
+    if (ies < 0) ies = 0;
+    const bool use_s = ies < MWLES;
+    int ie = (int)(energy * B.dlist);
+    if (ie < 0) ie = 0;
+    if (ie >= MWLE) ie = MWLE - 1;
+    int bs = -1, b0 = -1, bup = -1, bdn = -1, ball = -1;
+    for (int q = nd.first; q < nd.first + nd.count; ++q) {
+      const TetIdxEntry& E = B.tent[q];
+      if (!(a >= (double)E.lo[0] && a <= (double)E.hi[0] && b >= (double)E.lo[1] && b <= (double)E.hi[1] &&
+            c >= (double)E.lo[2] && c <= (double)E.hi[2]))
+        continue;
+      if (!contains(E.t, a, b, c)) continue;
+      const int lo = (int)(E.bins & 511u), hi = (int)((E.bins >> 9) & 511u);
+      const int slo = (int)((E.bins >> 18) & 63u), shi = (int)((E.bins >> 24) & 63u);
+      if (use_s && bs < 0 && slo <= ies && ies <= shi) bs = E.t;
+      if (b0 < 0 && lo <= ie && ie <= hi) b0 = E.t;
+      if (bup < 0 && ie + 1 < MWLE && lo <= ie + 1 && ie + 1 <= hi) bup = E.t;
+      if (bdn < 0 && ie - 1 >= 0 && lo <= ie - 1 && ie - 1 <= hi) bdn = E.t;
+      ball = E.t;
+      if (bs >= 0 || (!use_s && b0 >= 0)) break;  // nothing later can take precedence
+    }
+    const int r = bs >= 0 ? bs : b0 >= 0 ? b0 : bup >= 0 ? bup : bdn >= 0 ? bdn : ball;
+    if (r < 0) return false;
+    itet = r;
+    return true;
+  }
   EMC_HD EMC_INL void Getisym() {
     int mink, maxk;
     double tmin, tmax;
@@ -257,6 +304,13 @@ struct Flight {
     double a, b, c;
     in_wedge(a, b, c, kx, ky, kz);
     if (iband == B.tr[itet].ibt && contains(itet, a, b, c)) return;
+    if (B.tnode) {
+      if (!locate(a, b, c)) {
+        Flag_Catch = 1;
+        drop_reason = EMC_DROP_NO_TET_;
+      }
+      return;
+    }
     int ie = (int)(energy * B.dlists);
     if (ie < 0) ie = 0;
     if (ie < MWLES && scan(B.tlists, B.tlists_s, B.ptlists[ie * MNB + iband], B.ntlists[ie * MNB + iband], a, b, c)) return;

@@ -44,8 +44,32 @@ struct alignas(32) Sphere {
   double cx, cy, cz, r2;
 };
 
+// Point-location octree of one band's tetrahedra (built by build_tet_index).
+struct TetIdxRoot {
+  double lo[3], hi[3];  // closed box holding the band's grown tetrahedra
+  int node, pad;        // root node
+};
+// count >= 0: leaf, entries first .. first+count-1; count < 0: inner node, children
+// first .. first+7 (child bit 0: x >= mid, bit 1: y >= mid, bit 2: z >= mid)
+struct TetIdxNode {
+  int first, count;
+};
+// A tetrahedron of a leaf: the box of its grown tetrahedron widened to float (every
+// point Flight::contains accepts lies in it) and its bucket ranges, packed: tlist bins
+// lo = bits 0-8, hi = bits 9-17, tlists bins slo = bits 18-23, shi = bits 24-29 (an
+// empty range has lo > hi).
+struct alignas(32) TetIdxEntry {
+  float lo[3], hi[3];
+  int t;
+  uint32_t bins;
+};
+static_assert(sizeof(TetIdxEntry) == 32, "TetIdxEntry layout");
+
 struct BandDev {
   const TetRec* tr;
+  const TetIdxRoot* troot;  // [MNB] or null: no index, Getisym scans the lists
+  const TetIdxNode* tnode;
+  const TetIdxEntry* tent;
   const Sphere* tsph;                         // [nt]
   const Sphere *tlist_s, *tlists_s, *slist_s; // aligned with tlist, tlists, slist
   const SurfRec* sr;

@@ -242,6 +242,31 @@ class Engine:
                                                C.byref(it), C.byref(rr)))
         return sol, it.value, rr.value
 
+    def tet_index(self):
+        """The point-location octree: (roots, nodes, entries) numpy copies, or None."""
+        b = self.band
+        if b.tindex_nodes <= 0:
+            return None
+        rdt = np.dtype([("lo", "<f8", 3), ("hi", "<f8", 3), ("node", "<i4"), ("pad", "<i4")])
+        edt = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("t", "<i4"), ("bins", "<u4")])
+        roots = np.frombuffer((C.c_char * (8 * rdt.itemsize)).from_address(b.tindex_root), rdt).copy()
+        nodes = _view(b.tindex_node, 2 * b.tindex_nodes, np.int32).reshape(-1, 2).copy()
+        ent = np.frombuffer((C.c_char * (b.tindex_entries * edt.itemsize)).from_address(b.tindex_entry), edt).copy()
+        return roots, nodes, ent
+
+    def band_locate(self, k, energy, iband, itet_hint):
+        """Getisym on the host tables for points k [n, 3]: returns (itet, isym),
+        itet -1 where no tetrahedron holds the point."""
+        k = np.ascontiguousarray(k, np.float64).reshape(-1, 3)
+        n = len(k)
+        e = np.ascontiguousarray(energy, np.float64).reshape(n)
+        ib = np.ascontiguousarray(iband, np.int32).reshape(n)
+        it = np.array(itet_hint, np.int32).reshape(n).copy()
+        sy = np.zeros(n, np.int32)
+        self._check(self.lib.emc_band_locate(self.ctx, n, _ptr(k, C.c_double), _ptr(e, C.c_double), _ptr(ib, C.c_int),
+                                             _ptr(it, C.c_int), _ptr(sy, C.c_int)))
+        return it, sy
+
     def mg_levels(self):
         """The Poisson preconditioner's multigrid hierarchy: list of (dims, a27 or
         None, omega) per level; a27 has shape (27, n)."""

@@ -56,6 +56,8 @@ class BandView(C.Structure):
         ("scatte", _d), ("dose", _d), ("scattiie", _d), ("scatth", _d), ("dosh", _d), ("scattiih", _d),
         ("DOSMAX", C.c_double * 3),
         ("tsphere", _d),
+        ("tindex_nodes", C.c_int), ("tindex_entries", C.c_int),
+        ("tindex_root", C.c_void_p), ("tindex_node", C.POINTER(C.c_int)), ("tindex_entry", C.c_void_p),
     ]
 
 
@@ -131,6 +133,7 @@ def declare(lib):
         "emc_last_error": (C.c_char_p, [ctx]),
         "emc_band_view_get": (C.c_int, [ctx, P(BandView)]),
         "emc_mesh_view_get": (C.c_int, [ctx, P(MeshView)]),
+        "emc_band_locate": (C.c_int, [ctx, C.c_int64, _d, _d, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "emc_particles_init": (C.c_int, [ctx, _i64]),
         "emc_particles_count": (C.c_int, [ctx, _i64]),
         "emc_particles_upload": (C.c_int, [ctx, P(Particles), C.c_int64]),

@@ -113,6 +113,14 @@ typedef struct emc_band_view {
   const double *tsphere;               /* [nt][4] centre x y z and radius^2 of the sphere
                                            enclosing each tetrahedron grown by MY_ZERO
                                            (prefilter of the tetrahedron searches)   */
+  int tindex_nodes, tindex_entries;    /* point-location octree answering Getisym's list
+                                           scans (0 nodes: not built, the scans run)  */
+  const void *tindex_root;             /* [8 bands] {double lo[3], hi[3]; int32 node, pad} */
+  const int *tindex_node;              /* [nodes][2] {first, count}; count < 0: children
+                                           first..first+7 (bit0 x, bit1 y, bit2 z upper) */
+  const void *tindex_entry;            /* [entries] {float lo[3], hi[3]; int32 tet;
+                                           uint32 bins: tlist lo 0-8, hi 9-17, tlists lo
+                                           18-23, hi 24-29}                           */
 } emc_band_view;
 
 /* Host view of the mesh.  cell c = (j*cnx + i)*cnz + k ; node n = (j*pnx + i)*pnz + k. */
@@ -189,6 +197,16 @@ const char *emc_last_error(emc_ctx *ctx);
 
 int emc_band_view_get(emc_ctx *ctx, emc_band_view *out);
 int emc_mesh_view_get(emc_ctx *ctx, emc_mesh_view *out);
+
+/* Getisym (src/ParticleUtils.cpp:395-625) on the host tables, the code the flight
+ * kernels run: for each of n points k[q][0..2] (wave vector, normalised) of energy
+ * energy[q] in band iband[q], isym[q] is the symmetry operation mapping k into the
+ * irreducible wedge and itet[q] the tetrahedron holding it.  itet[q] is read first as
+ * the tetrahedron to test before the search (the particle's current one).  itet[q]
+ * is -1 where no tetrahedron holds the point (the reference's Flag_Catch).  Needs no
+ * GPU. */
+int emc_band_locate(emc_ctx *ctx, int64_t n, const double *k, const double *energy, const int *iband, int *itet,
+                    int *isym);
 
 /* Initial population (host sampling as init_particle_data), uploaded to the GPU
  * when one is attached.  n_out receives the particle count. */
