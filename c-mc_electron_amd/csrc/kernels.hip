@@ -646,39 +646,60 @@ __device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, in
   j = (int)(t / L.nx);
 }
 
+// Keeps the compiler from moving memory accesses across this point: the loads
+// above are all issued before the arithmetic below waits for the first of them
+// (otherwise the scheduler interleaves load, wait, multiply-add per neighbour).
+__device__ inline void mem_fence_compiler() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // (A_l x)(n) with x given by an accessor; level 0 is the 7-point operator in FP64,
-// coarser levels the symmetric half stencil
+// coarser levels the symmetric half stencil.  Every neighbour is loaded from a
+// clamped address and a missing one adds 0.0, so the loads carry no branches and
+// issue together; the sum is the same, term for term, as over the present
+// neighbours only.
 template <bool FINE, class XA>
 __device__ inline double mg_Ax(const MgArgs& m, const MgLevelDev& L, const XA& xa, size_t n, int i, int j, int k) {
   if (FINE) {
     const size_t sx = L.nz, sy = (size_t)L.nx * L.nz;
-    double v = m.diag[n] * xa(n);
-    if (i + 1 < L.nx) v += m.cx[n] * xa(n + sx);
-    if (i > 0) v += m.cx[n - sx] * xa(n - sx);
-    if (j + 1 < L.ny) v += m.cy[n] * xa(n + sy);
-    if (j > 0) v += m.cy[n - sy] * xa(n - sy);
-    if (k + 1 < L.nz) v += m.cz[n] * xa(n + 1);
-    if (k > 0) v += m.cz[n - 1] * xa(n - 1);
+    const bool ip = i + 1 < L.nx, im = i > 0, jp = j + 1 < L.ny, jm = j > 0, kp = k + 1 < L.nz, km = k > 0;
+    const size_t nip = ip ? n + sx : n, nim = im ? n - sx : n, njp = jp ? n + sy : n, njm = jm ? n - sy : n;
+    const size_t nkp = kp ? n + 1 : n, nkm = km ? n - 1 : n;
+    const double c_ip = m.cx[n], c_im = m.cx[nim], c_jp = m.cy[n], c_jm = m.cy[njm], c_kp = m.cz[n], c_km = m.cz[nkm];
+    const double x0 = xa(n), x_ip = xa(nip), x_im = xa(nim), x_jp = xa(njp), x_jm = xa(njm), x_kp = xa(nkp),
+                 x_km = xa(nkm), dg = m.diag[n];
+    mem_fence_compiler();
+    double v = dg * x0;
+    v += ip ? c_ip * x_ip : 0.0;
+    v += im ? c_im * x_im : 0.0;
+    v += jp ? c_jp * x_jp : 0.0;
+    v += jm ? c_jm * x_jm : 0.0;
+    v += kp ? c_kp * x_kp : 0.0;
+    v += km ? c_km * x_km : 0.0;
     return v;
   } else {
     const long long sx = L.nz, sy = (long long)L.nx * L.nz;
-    double v = 0;
+    float c[27];
+    double x[27];
+    bool ok[27];
 #pragma unroll
-    for (int dj = -1; dj <= 1; ++dj) {
-      if (j + dj < 0 || j + dj >= L.ny) continue;
+    for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
-      for (int di = -1; di <= 1; ++di) {
-        if (i + di < 0 || i + di >= L.nx) continue;
+      for (int di = -1; di <= 1; ++di)
 #pragma unroll
         for (int dk = -1; dk <= 1; ++dk) {
-          if (k + dk < 0 || k + dk >= L.nz) continue;
           const int o = ((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1);
-          const size_t q = (size_t)((long long)n + dj * sy + di * sx + dk);
-          const float c = o >= 13 ? L.c14[(size_t)(o - 13) * L.n + n] : L.c14[(size_t)(13 - o) * L.n + q];
-          v += (double)c * xa(q);
+          ok[o] = (unsigned)(j + dj) < (unsigned)L.ny && (unsigned)(i + di) < (unsigned)L.nx &&
+                  (unsigned)(k + dk) < (unsigned)L.nz;
+          const size_t q = ok[o] ? (size_t)((long long)n + dj * sy + di * sx + dk) : n;
+          c[o] = o >= 13 ? L.c14[(size_t)(o - 13) * L.n + n] : L.c14[(size_t)(13 - o) * L.n + q];
+          x[o] = xa(q);
         }
-      }
-    }
+    mem_fence_compiler();  // all 54 loads in flight before the first use
+    double v = 0;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) v += ok[o] ? (double)c[o] * x[o] : 0.0;
     return v;
   }
 }
@@ -717,9 +738,10 @@ __device__ inline void ph_resid(const MgArgs& m, const MgLevelDev& L) {
 }
 
 // r = b - A x1 with x1 = wdinv b: one Jacobi sweep from zero and its residual
+// (levels >= 1: the restriction stored x1 in x)
 template <bool FINE, class LP>
 __device__ inline void ph_resid1(const MgArgs& m, const MgLevelDev& L) {
-  auto x1 = [&](size_t q) { return L.wdinv[q] * L.b[q]; };
+  auto x1 = [&](size_t q) { return L.x[q]; };
   for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
@@ -730,49 +752,84 @@ __device__ inline void ph_resid1(const MgArgs& m, const MgLevelDev& L) {
 __device__ inline double pw1(int f, int C) { return f == 2 * C ? 1.0 : 0.5; }
 
 // b_C = P^T r_F: coarse node I gathers the fine residuals around fine node 2I
-// (residuals of fixed fine nodes are zero, so the zeroed rows of P need no test)
+// (residuals of fixed fine nodes are zero, so the zeroed rows of P need no test);
+// also x_C = wdinv_C b_C, the first sweep from zero on the coarse level.  Fine
+// nodes off the mesh are read at a clamped address and add 0.0.
 template <class LP>
 __device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
   for (size_t I = LP::start(); I < C.n; I += LP::stride()) {
     int ci, cj, ck;
     lvl_ijk(C, I, ci, cj, ck);
+    double r[27];
+    bool ok[27];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+          const int o = ((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1);
+          const int fj = 2 * cj + dj, fi = 2 * ci + di, fk = 2 * ck + dk;
+          ok[o] = (unsigned)fj < (unsigned)F.ny && (unsigned)fi < (unsigned)F.nx && (unsigned)fk < (unsigned)F.nz;
+          r[o] = F.r[ok[o] ? ((size_t)fj * F.nx + fi) * F.nz + fk : 0];
+        }
+    mem_fence_compiler();
     double v = 0;
-    for (int fj = max(2 * cj - 1, 0); fj <= min(2 * cj + 1, F.ny - 1); ++fj)
-      for (int fi = max(2 * ci - 1, 0); fi <= min(2 * ci + 1, F.nx - 1); ++fi) {
-        const double w = pw1(fj, cj) * pw1(fi, ci);
-        const size_t row = ((size_t)fj * F.nx + fi) * F.nz;
-        for (int fk = max(2 * ck - 1, 0); fk <= min(2 * ck + 1, F.nz - 1); ++fk) v += w * pw1(fk, ck) * F.r[row + fk];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        const double w = (dj == 0 ? 1.0 : 0.5) * (di == 0 ? 1.0 : 0.5);
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+          const int o = ((dj + 1) * 3 + (di + 1)) * 3 + (dk + 1);
+          v += ok[o] ? w * (dk == 0 ? 1.0 : 0.5) * r[o] : 0.0;
+        }
       }
     C.b[I] = v;
+    C.x[I] = C.wdinv[I] * v;
   }
 }
 
 // t_F += P e on the unknown nodes (wdinv > 0); with FROM_ZERO the smoothed value
 // before the correction is the sweep from zero, t_F = wdinv b + P e (fixed
-// nodes 0)
+// nodes 0).  Fine node f takes coarse node f/2 when f is even, (f-1)/2 and
+// (f+1)/2 with weight 1/2 each when odd; absent terms add 0.0.
 template <class LP, bool FROM_ZERO = false>
 __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, const double* e) {
   for (size_t n = LP::start(); n < F.n; n += LP::stride()) {
-    if (F.wdinv[n] == 0.0) {
+    const double wd = F.wdinv[n];
+    if (wd == 0.0) {
       if (FROM_ZERO) F.t[n] = 0.0;
       continue;
     }
     int i, j, k;
     lvl_ijk(F, n, i, j, k);
-    double v = 0;
-    for (int cj = j >> 1; cj <= (j + 1) >> 1; ++cj) {
-      if (cj >= C.ny) continue;
-      for (int ci = i >> 1; ci <= (i + 1) >> 1; ++ci) {
-        if (ci >= C.nx) continue;
-        const double w = pw1(j, cj) * pw1(i, ci);
-        const size_t row = ((size_t)cj * C.nx + ci) * C.nz;
-        for (int ck = k >> 1; ck <= (k + 1) >> 1; ++ck) {
-          if (ck >= C.nz) continue;
-          v += w * pw1(k, ck) * e[row + ck];
+    int cj[2] = {j >> 1, (j + 1) >> 1}, ci[2] = {i >> 1, (i + 1) >> 1}, ck[2] = {k >> 1, (k + 1) >> 1};
+    bool okj[2] = {cj[0] < C.ny, (j & 1) && cj[1] < C.ny}, oki[2] = {ci[0] < C.nx, (i & 1) && ci[1] < C.nx},
+         okk[2] = {ck[0] < C.nz, (k & 1) && ck[1] < C.nz};
+    double ev[8];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const bool ok = okj[a] && oki[b] && okk[c];
+          ev[(a * 2 + b) * 2 + c] = e[ok ? ((size_t)cj[a] * C.nx + ci[b]) * C.nz + ck[c] : 0];
         }
+    mem_fence_compiler();
+    double v = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const double w = pw1(j, cj[a]) * pw1(i, ci[b]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          v += (okj[a] && oki[b] && okk[c]) ? w * pw1(k, ck[c]) * ev[(a * 2 + b) * 2 + c] : 0.0;
       }
-    }
-    if (FROM_ZERO) F.t[n] = F.wdinv[n] * F.b[n] + v;
+    if (FROM_ZERO) F.t[n] = wd * F.b[n] + v;
     else F.t[n] += v;
   }
 }
