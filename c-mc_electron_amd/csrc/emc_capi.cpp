@@ -99,7 +99,7 @@ struct Gpu {
   // Poisson
   double *diag = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *dirichlet = nullptr, *corr = nullptr;
   double *pvol = nullptr, *pdop = nullptr, *rhs = nullptr, *pot = nullptr;
-  double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *scal = nullptr;
+  double *r = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *z = nullptr, *scal = nullptr;
   double *part_a = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_rz0 = nullptr, *part_rz1 = nullptr;
   int* done = nullptr;
   MgArgs mg{};                        // device view of the multigrid hierarchy
@@ -394,7 +394,8 @@ static int gpu_setup(emc_ctx* c) {
   g.dirichlet = m.dup(d.dirichlet_val); g.corr = m.dup(d.rhs_corr); g.kind = m.dup(d.node_kind);
   g.pvol = m.dup(d.pvol); g.pdop = m.dup(d.pdop);
   g.rhs = m.alloc<double>(NN); g.pot = m.alloc<double>(NN);
-  g.r = m.alloc<double>(NN); g.p = m.alloc<double>(NN); g.q = m.alloc<double>(NN); g.z = m.alloc<double>(NN);
+  g.r = m.alloc<double>(NN); g.p = m.alloc<double>(NN); g.p2 = m.alloc<double>(NN);
+  g.q = m.alloc<double>(NN); g.z = m.alloc<double>(NN);
   g.pcg_blocks = emc_pcg_blocks(NN);
   for (double** pp : {&g.part_a, &g.part_pq, &g.part_rr, &g.part_rz0, &g.part_rz1}) *pp = m.alloc<double>(3 * g.pcg_blocks);
   g.scal = m.alloc<double>(8);
@@ -1013,7 +1014,7 @@ static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
   PcgArgs a{};
   a.n = d.nnode; a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz; a.nblocks = g.pcg_blocks;
   a.diag = g.diag; a.cx = g.cx; a.cy = g.cy; a.cz = g.cz; a.b = b; a.kind = g.kind;
-  a.x = x; a.r = g.r; a.p = g.p; a.q = g.q; a.z = g.z;
+  a.x = x; a.r = g.r; a.p = g.p; a.p2 = g.p2; a.q = g.q; a.z = g.z;
   a.part_a = g.part_a; a.part_pq = g.part_pq; a.part_rr = g.part_rr; a.part_rz0 = g.part_rz0; a.part_rz1 = g.part_rz1;
   a.scal = g.scal; a.done = g.done; a.rtol = rtol;
   return a;
@@ -1027,34 +1028,28 @@ static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
 static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres) {
   Gpu& g = *c->gpu;
   PcgArgs a = pcg_args(c, b, x, rtol);
+  // k_pcg_begin sets done when the start already meets the tolerance: the
+  // queued iterations are then no-ops and the first check below stops the loop
   HIPCHECK(c, emc_launch_pcg_start(a, g.mg, g.stream));
   double* sh = g.scal_host;
-  HIPCHECK(c, hipMemcpyAsync(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-  HIPCHECK(c, hipStreamSynchronize(g.stream));
-  const double bb = sh[2];
-  const double bn = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
-  double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
-  if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
-  int it = 0;
-  if (rel > rtol) {
-    for (int k = 0; k < maxit; ++k) {
-      HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
-      double* slot = sh + 8 * (k & 1);
-      HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-      HIPCHECK(c, hipEventRecord(g.scal_ready[k & 1], g.stream));
-      if (k == 0) continue;
-      // residual after iteration k-1 (iteration k is already queued)
-      HIPCHECK(c, hipEventSynchronize(g.scal_ready[(k - 1) & 1]));
-      const double* prev = sh + 8 * ((k - 1) & 1);
-      if (!std::isfinite(prev[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
-      if (prev[0] <= prev[4]) break;
-    }
-    HIPCHECK(c, hipStreamSynchronize(g.stream));
-    HIPCHECK(c, hipMemcpy(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost));
-    it = (int)sh[3];
-    rel = std::sqrt(std::max(0.0, sh[0])) / bn;
-    if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
+  for (int k = 0; k < maxit; ++k) {
+    HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
+    double* slot = sh + 8 * (k & 1);
+    HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipEventRecord(g.scal_ready[k & 1], g.stream));
+    if (k == 0) continue;
+    // residual after iteration k-1 (iteration k is already queued)
+    HIPCHECK(c, hipEventSynchronize(g.scal_ready[(k - 1) & 1]));
+    const double* prev = sh + 8 * ((k - 1) & 1);
+    if (!std::isfinite(prev[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
+    if (prev[0] <= prev[4]) break;
   }
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost));
+  const double bn = std::sqrt(sh[2]) > 0 ? std::sqrt(sh[2]) : 1.0;
+  const int it = (int)sh[3];
+  const double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
+  if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
   if (iters) *iters = it;
   if (relres) *relres = rel;
   c->last_pcg_iters = it;

@@ -989,9 +989,9 @@ struct Flight {
   // The common case of a step: up to FAST_EVENTS events that are tetrahedron
   // crossings inside the zone or wedge changes, cell crossings by PASS, REFLECT,
   // SCATTOX with surface scattering on, or PERIOD, then the end of the step.
-  // Runs exactly the iterations of fly() for those events.  Returns false when
-  // anything else comes first (scattering, a Brillouin-zone face, contacts,
-  // leaving the mesh, a second current tally): the particle in memory is
+  // fast_step() runs exactly the iterations of fly() for those events.  It bails
+  // out when anything else comes first (scattering, a Brillouin-zone face,
+  // contacts, leaving the mesh, a second current tally): the particle in memory is
   // untouched and fly() redoes it from the start with the same random numbers.
   // A y-plane current of a PASS crossing is held in (pend_kind, pend_j) and
   // added by commit_pending() only once the particle has finished here.
@@ -1033,35 +1033,34 @@ struct Flight {
     load_cell();
     return true;
   }
-  EMC_HD EMC_INL bool fly_fast(int& pend_kind, int& pend_j) {
-    Loop L;
-    fly_begin(L);
-    pend_kind = 0;
-    pend_j = 0;
-    while (left_time > 0) {
-      GetV();
-      L.loop++;
-      n_events++;
-      if (L.loop > FAST_EVENTS) return false;
-      double Tf, gam;
-      size_t c;
-      const int flag = next_event(L, Tf, gam, c);
-      if (Tf < MY_ZERO || (flag >= 3 && flag <= 5)) return false;
-      advance(L, Tf, gam);
-      if (energy < 0) return false;
-      if (flag == 1) {
-        // HitTet without the Brillouin-zone fold
-        const int nb = B.tr[itet].neib[itetdir];
-        if (nb == -5 || nb == -6) return false;
-        if (nb >= 0) itet = nb;
-        else isym = B.newisym[(-nb - 2) * 48 + isym];
-        L.getTet = L.getCell = true;
-      } else if (flag == 2) {
-        if (!fast_cell_crossing(pend_kind, pend_j)) return false;
-        L.getTet = L.getCell = true;
-      }
+  // One iteration of fly()'s event loop on the fast path, after fly_begin() and
+  // with pend_kind = pend_j = 0 at the start of the particle, entered while
+  // left_time > 0: FAST_MORE (left_time still > 0), FAST_DONE (the step ended
+  // here) or FAST_BAIL (fly() must redo the particle).
+  enum { FAST_MORE = 0, FAST_DONE = 1, FAST_BAIL = 2 };
+  EMC_HD EMC_INL int fast_step(Loop& L, int& pend_kind, int& pend_j) {
+    GetV();
+    L.loop++;
+    n_events++;
+    if (L.loop > FAST_EVENTS) return FAST_BAIL;
+    double Tf, gam;
+    size_t c;
+    const int flag = next_event(L, Tf, gam, c);
+    if (Tf < MY_ZERO || (flag >= 3 && flag <= 5)) return FAST_BAIL;
+    advance(L, Tf, gam);
+    if (energy < 0) return FAST_BAIL;
+    if (flag == 1) {
+      // HitTet without the Brillouin-zone fold
+      const int nb = B.tr[itet].neib[itetdir];
+      if (nb == -5 || nb == -6) return FAST_BAIL;
+      if (nb >= 0) itet = nb;
+      else isym = B.newisym[(-nb - 2) * 48 + isym];
+      L.getTet = L.getCell = true;
+    } else if (flag == 2) {
+      if (!fast_cell_crossing(pend_kind, pend_j)) return FAST_BAIL;
+      L.getTet = L.getCell = true;
     }
-    return true;
+    return left_time > 0 ? FAST_MORE : FAST_DONE;
   }
   EMC_HD EMC_INL void commit_pending(int pend_kind, int pend_j) {
     if (pend_kind == 1) tally_add(&T.rcur[pend_j], fxd(charge, EMC_FX_CONTACT_));

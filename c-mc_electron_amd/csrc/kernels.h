@@ -99,7 +99,7 @@ struct PcgArgs {
   int pnx, pny, pnz, nblocks;
   const double *diag, *cx, *cy, *cz, *b;
   const uint8_t* kind;
-  double *x, *r, *p, *q, *z;
+  double *x, *r, *p, *p2, *q, *z;  // p and p2: search direction of even and odd iterations
   double *part_a, *part_pq, *part_rr, *part_rz0, *part_rz1;  // [3*nblocks] block partial sums
   double* scal;  // [0] r.r [1] r.z [2] b.b [3] iterations done [4] r.r at the tolerance
   int* done;     // set once r.r reaches scal[4]: every later CG / V-cycle kernel returns at once
@@ -128,7 +128,13 @@ struct MgArgs {
   int nl, nc;                          // levels, nodes of the coarsest level
   const double* cinv;                  // [nc*nc] coarsest inverse, row major
   const double *diag, *cx, *cy, *cz;   // level-0 operator
-  const int* done;
+  int* done;
+  // CG iterations: the first kernel of the cycle also books the residual of the
+  // iteration (r.r from part_rr, npart block partials) into scal and sets done
+  // once r.r <= scal[4]; null in the cycle that starts the solve
+  const double* part_rr;
+  double* scal;
+  int npart, pad;
 };
 
 }  // namespace emc

@@ -1,7 +1,7 @@
 // HIP kernels of the MI355X ensemble Monte Carlo step (gfx950, wave64).
 //
-//   k_fly_fast   one lane per particle: drift of those with no event this step,
-//                the rest listed for
+//   k_fly_fast   one lane per particle: those whose step has only crossings
+//                finish here, the rest are listed for
 //   k_fly_slow   one lane per listed particle, full-band event loop (fly_core.h)
 //   k_fly_clones same for GENERATE clones of this step
 //   k_merge      surviving clones -> holes left by caught particles, then append
@@ -66,10 +66,13 @@ __device__ inline void load_particle(Flight& f, const FlyArgs& a, long long q, u
 }
 
 // Fast pass: particles whose step has only tetrahedron and plain cell crossings
-// (Flight::fly_fast) finish here.  The others are listed, wave-compacted, for
+// (Flight::fast_step) finish here.  The others are listed, wave-compacted, for
 // k_fly_slow, which restarts them from the untouched state: the RNG is counter
 // based, so the full event loop draws the same numbers and the result equals one
-// pass of fly().
+// pass of fly().  (Refilling finished lanes from a per-wave chunk, so that a
+// wave does not wait for its longest history, was measured slower: most
+// particles end after one or two events, so nearly every iteration paid for a
+// refill.)
 __global__ __launch_bounds__(256, 3) void k_fly_fast(FlyArgs a) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   long long ev = 0;
@@ -83,8 +86,14 @@ __global__ __launch_bounds__(256, 3) void k_fly_fast(FlyArgs a) {
       if (pk != PK_DEAD) {
         Flight f(a.C->B, a.C->M, a.C->F, a.T);
         load_particle(f, a, q, pk);
-        int pend_kind, pend_j;
-        if (f.fly_fast(pend_kind, pend_j)) {
+        Flight::Loop L;
+        f.fly_begin(L);
+        int pend_kind = 0, pend_j = 0, r = Flight::FAST_DONE;
+        while (f.left_time > 0) {
+          r = f.fast_step(L, pend_kind, pend_j);
+          if (r != Flight::FAST_MORE) break;
+        }
+        if (r != Flight::FAST_BAIL) {
           ev += f.n_events;
           f.commit_pending(pend_kind, pend_j);
           a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
@@ -570,34 +579,57 @@ __global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a) {
   block_reduce3(rr, bb, 0.0, a.part_a);
 }
 
-// p = z ; scal = (r.r, r.z, b.b, 0, rtol^2 b.b)
+// scal = (r.r, r.z, b.b, 0, rtol^2 b.b); done when the start meets the tolerance
+// (one block)
 __global__ __launch_bounds__(256) void k_pcg_begin(PcgArgs a) {
   double ta[3], tz[3];
   total3(a.part_a, a.nblocks, ta);
   total3(a.part_rz0, a.nblocks, tz);
-  GRID_LOOP(n, a.n) a.p[n] = a.z[n];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
     a.scal[0] = ta[0]; a.scal[1] = tz[0]; a.scal[2] = ta[1];
     a.scal[4] = a.rtol * a.rtol * ta[1];
+    if (ta[0] <= a.scal[4]) *a.done = 1;
   }
 }
 
-// q = A p ; partial p.q
-__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a) {
+// p = z + beta p_in (p = z in the first iteration), beta = r.z(new) / r.z(old),
+// then q = A p and the partial p.q.  p at the six neighbours is recomputed from
+// z and p_in with the same operations, so p_out is written once and p_in never
+// changes during the pass.
+__global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_old, const double* rz_new,
+                                                   const double* p_in, double* p_out, int first) {
   if (*a.done) return;
+  double beta = 0.0;
+  if (!first) {
+    double to[3], tn[3];
+    total3(rz_old, a.nblocks, to);
+    total3(rz_new, a.nblocks, tn);
+    beta = (to[0] != 0.0) ? tn[0] / to[0] : 0.0;
+  }
+  const double* __restrict__ z = a.z;
+  auto pv = [&](size_t m) { return first ? z[m] : z[m] + beta * p_in[m]; };
+  const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
   double pq = 0;
   GRID_LOOP(n, a.n) {
     int i, j, k;
     node_ijk(a, n, i, j, k);
-    const double q = apply_op(a, a.p, n, i, j, k);
+    const double pn = pv(n);
+    double q = a.diag[n] * pn;
+    if (i + 1 < a.pnx) q += a.cx[n] * pv(n + sx);
+    if (i > 0) q += a.cx[n - sx] * pv(n - sx);
+    if (j + 1 < a.pny) q += a.cy[n] * pv(n + sy);
+    if (j > 0) q += a.cy[n - sy] * pv(n - sy);
+    if (k + 1 < a.pnz) q += a.cz[n] * pv(n + 1);
+    if (k > 0) q += a.cz[n - 1] * pv(n - 1);
+    p_out[n] = pn;
     a.q[n] = q;
-    pq += a.p[n] * q;
+    pq += pn * q;
   }
   block_reduce3(pq, 0.0, 0.0, a.part_pq);
 }
 
 // alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
-__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_old) {
+__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_old, const double* p) {
   if (*a.done) return;
   double ta[3], tb[3];
   total3(rz_old, a.nblocks, ta);
@@ -605,29 +637,12 @@ __global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_
   const double alpha = (tb[0] != 0.0) ? ta[0] / tb[0] : 0.0;
   double rr = 0;
   GRID_LOOP(n, a.n) {
-    a.x[n] += alpha * a.p[n];
+    a.x[n] += alpha * p[n];
     const double r = a.r[n] - alpha * a.q[n];
     a.r[n] = r;
     rr += r * r;
   }
   block_reduce3(rr, 0.0, 0.0, a.part_rr);
-}
-
-// beta = r.z(new) / r.z(old) ; p = z + beta p ; convergence flag
-__global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, const double* rz_old, const double* rz_new) {
-  if (*a.done) return;
-  double to[3], tn[3], tr[3];
-  total3(rz_old, a.nblocks, to);
-  total3(rz_new, a.nblocks, tn);
-  total3(a.part_rr, a.nblocks, tr);
-  const double beta = (to[0] != 0.0) ? tn[0] / to[0] : 0.0;
-  GRID_LOOP(n, a.n) a.p[n] = a.z[n] + beta * a.p[n];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.scal[0] = tr[0];
-    a.scal[1] = tn[0];
-    a.scal[3] += 1.0;
-    if (tr[0] <= a.scal[4]) *a.done = 1;
-  }
 }
 
 // ------------------------------------------------------------------ multigrid V-cycle
@@ -853,8 +868,24 @@ __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out,
 // result of every level >= 1 ends in x (the coarsest is solved into x).
 __device__ __host__ inline const double* mg_result(const MgArgs& m, int l) { return m.L[l].x; }
 
+// In a CG iteration the first kernel of the cycle books the residual of the
+// iteration that has just updated r (= b of level 0): block 0 writes r.r and the
+// iteration count into scal and sets done at the tolerance (the rest of this
+// launch still runs; every later kernel returns at once).
+__device__ inline void book_residual(const MgArgs& m) {
+  if (!m.part_rr || blockIdx.x != 0) return;
+  double tr[3];
+  total3(m.part_rr, m.npart, tr);
+  if (threadIdx.x == 0) {
+    m.scal[0] = tr[0];
+    m.scal[3] += 1.0;
+    if (tr[0] <= m.scal[4]) *m.done = 1;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_mg_pre2(MgArgs m) {
   if (*m.done) return;
+  book_residual(m);
   ph_pre2<true, GridLoop>(m, m.L[0]);
 }
 __global__ __launch_bounds__(256) void k_mg_resid(MgArgs m) {
@@ -915,6 +946,7 @@ __global__ __launch_bounds__(256) void k_mg_coarse(MgArgs m) {
 // single-level hierarchy: z = C^-1 r over the grid, then the partial r.z
 __global__ __launch_bounds__(256) void k_mg_coarse_z(MgArgs m, double* z) {
   if (*m.done) return;
+  book_residual(m);
   ph_coarse(m, m.L[0].b, z, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
 __global__ __launch_bounds__(256) void k_mg_dot_rz(MgArgs m, const double* z, double* part_rz) {
@@ -1030,19 +1062,30 @@ hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s
   const int g = a.nblocks;
   hipLaunchKernelGGL(k_pcg_fix, dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_pcg_resid, dim3(g), dim3(256), 0, s, a);
-  launch_vcycle(a, m, a.part_rz0, s);
-  hipLaunchKernelGGL(k_pcg_begin, dim3(g), dim3(256), 0, s, a);
+  MgArgs m0 = m;
+  m0.part_rr = nullptr;
+  launch_vcycle(a, m0, a.part_rz0, s);
+  hipLaunchKernelGGL(k_pcg_begin, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-// iteration `it`: r.z of the previous iteration is in part_rz0 (even it) or part_rz1
+// iteration `it`: r.z of the previous iteration is in part_rz0 (even it) or
+// part_rz1; the search direction alternates between p (even it) and p2
 hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStream_t s) {
   const int g = a.nblocks;
   double* rz_old = (it & 1) ? a.part_rz1 : a.part_rz0;
   double* rz_new = (it & 1) ? a.part_rz0 : a.part_rz1;
-  hipLaunchKernelGGL(k_pcg_spmv, dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(256), 0, s, a, (const double*)rz_old);
-  launch_vcycle(a, m, rz_new, s);
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(g), dim3(256), 0, s, a, (const double*)rz_old, (const double*)rz_new);
+  double* p_out = (it & 1) ? a.p2 : a.p;
+  const double* p_in = (it & 1) ? a.p : a.p2;
+  // the previous iteration's cycle left r.z in rz_old; before that, rz_new held
+  // the r.z of the iteration before (beta = rz_old / rz_new of the previous pair)
+  hipLaunchKernelGGL(k_pcg_spmvp, dim3(g), dim3(256), 0, s, a, (const double*)rz_new, (const double*)rz_old, p_in,
+                     p_out, it == 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(256), 0, s, a, (const double*)rz_old, (const double*)p_out);
+  MgArgs mi = m;
+  mi.part_rr = a.part_rr;
+  mi.scal = a.scal;
+  mi.npart = a.nblocks;
+  launch_vcycle(a, mi, rz_new, s);
   return hipGetLastError();
 }
