@@ -331,7 +331,14 @@ __global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
   for (int e = threadIdx.x; e < 4 * TILE_CELLS; e += blockDim.x) sc[e] = 0;
   __syncthreads();
   const bool want_stat = a.node_stat != nullptr;
-  for (unsigned q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+  // Lane l of wave w takes entry 4l + w of each run of 256: the particles of one
+  // cell tend to sit together in the tile, and spreading them over the waves
+  // keeps the lanes of a wave off the same LDS words (same-address atomics
+  // serialise).
+  const unsigned spread = ((threadIdx.x & 63u) << 2) | (threadIdx.x >> 6);
+  for (unsigned q0 = p0; q0 < p1; q0 += blockDim.x) {
+    const unsigned q = q0 + (blockDim.x == 256 ? spread : threadIdx.x);
+    if (q >= p1) continue;
     const uint32_t pk = a.P.pk[q];
     const int i = pk_i(pk), j = pk_j(pk), k = pk_k(pk), pt = pk_type(pk);
     const double x = a.P.x[q], y = a.P.y[q], z = a.P.z[q], charge = a.P.charge[q], energy = a.P.energy[q];
