@@ -90,6 +90,10 @@ struct Gpu {
   long long* dead = nullptr;
   unsigned long long dead_cap = 0;
   unsigned long long* ctr = nullptr;  // [0] nclones [1] ndead [2] ticket [3] slow-list length
+                                      // [4] [5] clone range of the next pass [6] clone passes with work
+  unsigned long long* ctr_host = nullptr;    // pinned copy of ctr after the flight
+  unsigned long long* tally_host = nullptr;  // pinned copy of contact + counters after the flight
+  bool pcg_pending = false;           // the step's PCG result is still to be read (scal_host[16..23])
   uint32_t* slow = nullptr;           // [cap] flight slow list
   // fields / mesh work arrays
   double *ex = nullptr, *ey = nullptr, *ez = nullptr, *rho = nullptr, *rates = nullptr;
@@ -377,7 +381,7 @@ static int gpu_setup(emc_ctx* c) {
   g.lcur = m.alloc<unsigned long long>(d.pny);
   g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
   g.counters = m.alloc<unsigned long long>(CNT_N);
-  g.ctr = m.alloc<unsigned long long>(4);
+  g.ctr = m.alloc<unsigned long long>(8);
   g.cst = m.alloc<FlyConst>(1);
   g.ntx = (d.cnx + TILE_I - 1) / TILE_I;
   g.nty = (d.cny + TILE_J - 1) / TILE_J;
@@ -432,7 +436,10 @@ static int gpu_setup(emc_ctx* c) {
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
   HIPCHECK(c, hipMemset(g.tcount, 0, g.ntile * sizeof(unsigned)));
   HIPCHECK(c, hipHostMalloc((void**)&g.n_host, sizeof(unsigned), hipHostMallocDefault));
-  HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 16 * sizeof(double), hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 24 * sizeof(double), hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.ctr_host, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.tally_host, (EMC_MAX_CONTACT * EMC_CT_NCOL + CNT_N) * sizeof(unsigned long long),
+                            hipHostMallocDefault));
   for (auto& e : g.scal_ready) HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHECK(c, hipEventCreateWithFlags(&g.n_ready, hipEventDisableTiming));
   HIPCHECK(c, hipMemset(g.ex, 0, NC * sizeof(double)));
@@ -628,6 +635,8 @@ void emc_destroy(emc_ctx* c) {
     if (c->gpu->n_ready) (void)hipEventDestroy(c->gpu->n_ready);
     if (c->gpu->n_host) (void)hipHostFree(c->gpu->n_host);
     if (c->gpu->scal_host) (void)hipHostFree(c->gpu->scal_host);
+    if (c->gpu->ctr_host) (void)hipHostFree(c->gpu->ctr_host);
+    if (c->gpu->tally_host) (void)hipHostFree(c->gpu->tally_host);
     for (auto e : c->gpu->scal_ready)
       if (e) (void)hipEventDestroy(e);
     if (c->gpu->stream) (void)hipStreamDestroy(c->gpu->stream);
@@ -880,51 +889,77 @@ static FlyArgs fly_args(emc_ctx* c, uint64_t step, int flag_heat) {
   return a;
 }
 
-// One flight step on the stream: fly, clone passes, merge.  Synchronises to size
-// the clone passes; updates g.n.
-static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
+// Queues one flight step on the stream: fly, two clone passes and the merge,
+// all sized on the device, then pinned copies of the counters and tallies.  No
+// host round trip; finish_fly() completes the step once the stream has passed.
+static int launch_fly(emc_ctx* c, uint64_t step, int flag_heat, FlyArgs& a) {
   Gpu& g = *c->gpu;
   int st = settle_n(c);
   if (st) return st;
   st = sync_fly_const(c);
   if (st) return st;
-  HIPCHECK(c, emc_launch_zero(g.ctr, 4, g.stream));
-  FlyArgs a = fly_args(c, step, flag_heat);
+  HIPCHECK(c, emc_launch_zero(g.ctr, 8, g.stream));
+  a = fly_args(c, step, flag_heat);
   HIPCHECK(c, hipEventRecord(g.ev[5], g.stream));
   HIPCHECK(c, emc_launch_fly(a, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[6], g.stream));
-  unsigned long long h[4];
-  HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
-  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, emc_launch_clones_dev(a, g.ctr, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(g.ctr_host, g.ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL, g.counters, CNT_N * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, g.stream));
+  return EMC_OK;
+}
+
+// After the stream has passed launch_fly's copies: clones of a third generation
+// (rare) fly in host-driven passes and are merged; g.n takes the appended
+// survivors.  Returns the number of clone passes that had clones, or an error.
+static int finish_fly(emc_ctx* c, const FlyArgs& a) {
+  Gpu& g = *c->gpu;
+  unsigned long long* h = g.ctr_host;
   g.last_full_loop = (long long)h[3];
-  unsigned long long begin = 0, end = std::min(h[0], g.clone_cap);
-  int passes = 0;
-  while (end > begin) {
-    HIPCHECK(c, emc_launch_fly_clones(a, begin, end, g.stream));
-    ++passes;
-    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
+  int passes = (int)h[6];
+  if (h[5] > h[4]) {
+    const unsigned long long first = h[4];
+    unsigned long long begin = h[4], end = h[5];
+    while (end > begin) {
+      HIPCHECK(c, emc_launch_fly_clones(a, begin, end, g.stream));
+      HIPCHECK(c, hipMemcpyAsync(h, g.ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
+      HIPCHECK(c, hipStreamSynchronize(g.stream));
+      begin = end;
+      end = std::min(h[0], g.clone_cap);
+      if (end > begin && ++passes > 64) return fail(c, EMC_ERR_STATE, "clone passes do not terminate");
+    }
+    HIPCHECK(c, emc_launch_merge(a, first, end, g.ctr + 1, g.ctr + 2, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL, g.counters,
+                               CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
     HIPCHECK(c, hipStreamSynchronize(g.stream));
-    begin = end;
-    end = std::min(h[0], g.clone_cap);
-    if (passes > 64) return fail(c, EMC_ERR_STATE, "clone passes do not terminate");
   }
-  const unsigned long long ndead = h[1];
-  if (end > 0) {
-    HIPCHECK(c, emc_launch_merge(a, end, ndead, g.ctr + 2, g.stream));
-    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, sizeof(h), hipMemcpyDeviceToHost, g.stream));
-    HIPCHECK(c, hipStreamSynchronize(g.stream));
-    const unsigned long long surv = h[2];
-    const unsigned long long used = std::min(ndead, g.dead_cap);
-    if (surv > used) g.n = std::min<long long>(g.cap, g.n + (long long)(surv - used));
-  }
+  const unsigned long long surv = h[2];
+  const unsigned long long used = std::min(h[1], g.dead_cap);
+  if (surv > used) g.n = std::min<long long>(g.cap, g.n + (long long)(surv - used));
   return passes;
 }
 
+// One flight step, synchronous (the emc_fly_step entry point).
+static int run_fly(emc_ctx* c, uint64_t step, int flag_heat) {
+  Gpu& g = *c->gpu;
+  FlyArgs a{};
+  int st = launch_fly(c, step, flag_heat, a);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  return finish_fly(c, a);
+}
+
+// the flight's tallies from the pinned copies finish_fly() left
 static void read_tallies(emc_ctx* c, emc_fly_tallies* out, int passes) {
   Gpu& g = *c->gpu;
-  unsigned long long ct[EMC_MAX_CONTACT * EMC_CT_NCOL], cn[CNT_N];
-  (void)hipMemcpy(ct, g.contact, sizeof(ct), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(cn, g.counters, sizeof(cn), hipMemcpyDeviceToHost);
+  const unsigned long long* ct = g.tally_host;
+  const unsigned long long* cn = g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL;
   if (!out) return;
   for (int i = 0; i < EMC_MAX_CONTACT; ++i)
     for (int j = 0; j < EMC_CT_NCOL; ++j) out->contact[i][j] += (int64_t)ct[i * EMC_CT_NCOL + j];
@@ -1025,7 +1060,11 @@ static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
 // system).  Iteration k+1 is queued before the host reads the residual of
 // iteration k, so the GPU never waits for the check; once converged the device
 // flag turns the queued kernels into no-ops.
-static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres) {
+// With `defer` the solve is only queued past its last check: the result (iteration
+// count, residual) is read by finish_pcg() once the stream has passed it.
+static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres);
+static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres,
+                   bool defer = false) {
   Gpu& g = *c->gpu;
   PcgArgs a = pcg_args(c, b, x, rtol);
   // k_pcg_begin sets done when the start already meets the tolerance: the
@@ -1044,8 +1083,18 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
     if (!std::isfinite(prev[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
     if (prev[0] <= prev[4]) break;
   }
+  HIPCHECK(c, hipMemcpyAsync(sh + 16, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+  g.pcg_pending = true;
+  if (defer) return EMC_OK;
   HIPCHECK(c, hipStreamSynchronize(g.stream));
-  HIPCHECK(c, hipMemcpy(sh, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost));
+  return finish_pcg(c, rtol, iters, relres);
+}
+
+static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres) {
+  Gpu& g = *c->gpu;
+  if (!g.pcg_pending) return EMC_OK;
+  g.pcg_pending = false;
+  const double* sh = g.scal_host + 16;
   const double bn = std::sqrt(sh[2]) > 0 ? std::sqrt(sh[2]) : 1.0;
   const int it = (int)sh[3];
   const double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
@@ -1199,17 +1248,23 @@ int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
     RhsArgs ra{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
     HIPCHECK(c, emc_launch_rhs(ra, g.stream));
     HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
-    st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr);
+    st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr, true);
     if (st) return st;
     HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
     HIPCHECK(c, emc_launch_field(field_args(c, g.pot, true), g.stream));
     HIPCHECK(c, hipEventRecord(g.ev[3], g.stream));
     st = zero_tallies(c);
     if (st) return st;
-    int passes = run_fly(c, step, c->dev.cfg.flag_heat ? 1 : 0);
-    if (passes < 0) return passes;
+    FlyArgs fa{};
+    st = launch_fly(c, step, c->dev.cfg.flag_heat ? 1 : 0, fa);
+    if (st) return st;
     HIPCHECK(c, hipEventRecord(g.ev[4], g.stream));
     HIPCHECK(c, hipEventSynchronize(g.ev[4]));
+    // the one wait of the step: the solve's result, then the flight's counts
+    st = finish_pcg(c, c->opts.poisson_rtol, nullptr, nullptr);
+    if (st) return st;
+    const int passes = finish_fly(c, fa);
+    if (passes < 0) return passes;
     float ms;
     for (int q = 0; q < 4; ++q) {
       HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[q], g.ev[q + 1]));

@@ -141,8 +141,9 @@ struct MgArgs {
 
 hipError_t emc_launch_fly(const emc::FlyArgs& a, hipStream_t s);
 hipError_t emc_launch_fly_clones(const emc::FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s);
-hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long nclones, unsigned long long ndead,
-                            unsigned long long* ticket, hipStream_t s);
+hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long begin, unsigned long long end,
+                            const unsigned long long* ndead, unsigned long long* ticket, hipStream_t s);
+hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, unsigned long long* ctr, hipStream_t s);
 hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);

@@ -160,8 +160,7 @@ __global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
 
 // Clones [begin, end) of the clone buffer fly the remaining time of the step;
 // `lpw` clones per wave, as in k_fly_slow.
-__global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long long begin, unsigned long long end,
-                                                    int lpw) {
+__device__ inline void fly_clone_range(const FlyArgs& a, unsigned long long begin, unsigned long long end, int lpw) {
   long long ev = 0, ph = 0, im = 0, ca = 0;
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -197,14 +196,65 @@ __global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long lon
   flush_counters(a.T, ev, ph, im, ca);
 }
 
-// Surviving clones take the slots of caught particles first, then go past n.
-__global__ __launch_bounds__(256) void k_merge(FlyArgs a, unsigned long long nclones, unsigned long long ndead,
-                                               unsigned long long* ticket) {
+// clones [begin, end): the host-driven pass
+__global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long long begin, unsigned long long end,
+                                                    int lpw) {
+  fly_clone_range(a, begin, end, lpw);
+}
+
+// Clone passes without the host: ctr[4] .. ctr[5] is the range the next pass
+// flies.  k_clone_range moves it on to the clones created since and counts the
+// non-empty ranges in ctr[6] (one thread).
+__global__ void k_clone_range(unsigned long long* ctr, unsigned long long cap) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr[4] = ctr[5];
+    ctr[5] = ctr[0] < cap ? ctr[0] : cap;
+    if (ctr[5] > ctr[4]) ctr[6] += 1;
+  }
+}
+
+// k_fly_clones over ctr[4] .. ctr[5], spread as k_fly_slow spreads its list
+__global__ __launch_bounds__(256) void k_fly_clones_dev(FlyArgs a, const unsigned long long* ctr) {
+  const long long begin = (long long)ctr[4], n = (long long)ctr[5] - begin;
+  if (n <= 0) return;
+  const long long slots = a.wave_slots > 0 ? a.wave_slots : 1;
+  const int lpw = (int)(n <= slots ? 1 : ((n + slots - 1) / slots < 64 ? (n + slots - 1) / slots : 64));
+  fly_clone_range(a, (unsigned long long)begin, (unsigned long long)(begin + n), lpw);
+}
+
+// Surviving clones take the slots of caught particles first, then go past n:
+// survivors of clones [begin, end), ticket counts the survivors placed so far
+// (ndead: *ndead_p)
+__global__ __launch_bounds__(256) void k_merge(FlyArgs a, unsigned long long begin, unsigned long long end,
+                                               const unsigned long long* ndead_p, unsigned long long* ticket) {
+  const unsigned long long ndead = *ndead_p;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < (long long)nclones; q += stride) {
+  for (long long q = (long long)begin + (long long)blockIdx.x * blockDim.x + threadIdx.x; q < (long long)end; q += stride) {
     const CloneRec& c = a.clones[q];
     if (c.pk == PK_DEAD) continue;
     const unsigned long long m = atomicAdd(ticket, 1ull);
+    long long slot;
+    if (m < ndead && m < a.dead_cap) slot = a.dead[m];
+    else slot = a.n + (long long)(m - (ndead < a.dead_cap ? ndead : a.dead_cap));
+    if (slot >= a.cap) {
+      atomicAdd(&a.T.counters[CNT_DROP0 + 8], 1ull);
+      continue;
+    }
+    a.P.x[slot] = c.x; a.P.y[slot] = c.y; a.P.z[slot] = c.z;
+    a.P.kx[slot] = c.kx; a.P.ky[slot] = c.ky; a.P.kz[slot] = c.kz;
+    a.P.energy[slot] = c.energy; a.P.charge[slot] = c.charge;
+    a.P.pk[slot] = c.pk; a.P.itet[slot] = c.itet; a.P.isym[slot] = c.isym; a.P.key[slot] = c.key;
+  }
+}
+
+// k_merge over the clones flown by the device-driven passes: [0, ctr[4])
+__global__ __launch_bounds__(256) void k_merge_dev(FlyArgs a, unsigned long long* ctr) {
+  const unsigned long long end = ctr[4], ndead = ctr[1];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < (long long)end; q += stride) {
+    const CloneRec& c = a.clones[q];
+    if (c.pk == PK_DEAD) continue;
+    const unsigned long long m = atomicAdd(&ctr[2], 1ull);
     long long slot;
     if (m < ndead && m < a.dead_cap) slot = a.dead[m];
     else slot = a.n + (long long)(m - (ndead < a.dead_cap ? ndead : a.dead_cap));
@@ -1000,10 +1050,25 @@ hipError_t emc_launch_fly_clones(const FlyArgs& a, unsigned long long b, unsigne
   hipLaunchKernelGGL(k_fly_clones, dim3(grid_for(waves * 64, 256)), dim3(256), 0, s, a, b, e, lpw);
   return hipGetLastError();
 }
-hipError_t emc_launch_merge(const FlyArgs& a, unsigned long long nclones, unsigned long long ndead,
-                            unsigned long long* ticket, hipStream_t s) {
-  if (nclones == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_merge, dim3(grid_for((long long)nclones, 256)), dim3(256), 0, s, a, nclones, ndead, ticket);
+hipError_t emc_launch_merge(const FlyArgs& a, unsigned long long begin, unsigned long long end,
+                            const unsigned long long* ndead, unsigned long long* ticket, hipStream_t s) {
+  if (end <= begin) return hipSuccess;
+  hipLaunchKernelGGL(k_merge, dim3(grid_for((long long)(end - begin), 256)), dim3(256), 0, s, a, begin, end, ndead,
+                     ticket);
+  return hipGetLastError();
+}
+// Two clone passes (clones, then the clones they made) and the merge of what
+// they flew, with every range read on the device: no host round trip.  ctr[4]
+// and ctr[5] must read zero on entry; afterwards ctr[4] .. ctr[5] holds clones
+// of a third generation still to fly (almost always empty).
+hipError_t emc_launch_clones_dev(const FlyArgs& a, unsigned long long* ctr, hipStream_t s) {
+  const long long blocks = a.wave_slots > 4 ? a.wave_slots / 4 : 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(k_clone_range, dim3(1), dim3(64), 0, s, ctr, a.clone_cap);
+    hipLaunchKernelGGL(k_fly_clones_dev, dim3((unsigned)blocks), dim3(256), 0, s, a, (const unsigned long long*)ctr);
+  }
+  hipLaunchKernelGGL(k_clone_range, dim3(1), dim3(64), 0, s, ctr, a.clone_cap);
+  hipLaunchKernelGGL(k_merge_dev, dim3((unsigned)blocks), dim3(256), 0, s, a, ctr);
   return hipGetLastError();
 }
 // sort P into Q by tile; toff[ntile] is the live count afterwards

@@ -182,6 +182,33 @@ def test_full_step_composition(eng):
     assert_same_population(pg, po, "full step")
 
 
+def test_consecutive_steps_deposit(eng):
+    """Twenty emc_step calls in a row: GENERATE clones fill holes and are appended,
+    and each step's tile sort and deposit start from the previous step's order.
+    Every step's cell densities equal the oracle's deposit of the population the
+    step started from (bit-exact), and its potential equals a tight solve of the
+    oracle's node charges."""
+    import ctypes as C
+    lib = oracle_lib.load()
+    eng.particles_init()
+    m = eng.mesh
+    gen = 0
+    for s in range(20):
+        p = eng.download()
+        t = eng.step(1)
+        gen += t.n_gen
+        print(f"[gpu] consecutive step {s}: {p.n} slots", flush=True)
+        fg = eng.fields_get()
+        node_q, _, cell_q, _ = oracle_lib.deposit(eng, p, p.n)
+        rho = np.zeros(m.ncell)
+        lib.orc_cell_rho(C.byref(m), oracle_lib.i64ptr(cell_q), oracle_lib.dptr(rho))
+        np.testing.assert_array_equal(fg.rho, rho, err_msg=f"step {s}: cell densities")
+        pg = eng.potential()
+        x, it, rel = eng.poisson_solve(eng.rhs_from_charge(node_q), x0=pg, rtol=1e-12)
+        assert np.max(np.abs(pg - x)) <= 1e-7 * np.max(np.abs(x)), f"step {s}: potential"
+    assert gen > 0, "no GENERATE clones: the merge and append paths were not exercised"
+
+
 def test_poisson_multilevel_mesh():
     """60x120x36 cells (273 k nodes): the hierarchy has two grid-wide levels before
     the one-workgroup bottom, so every V-cycle kernel runs.  Random charge on the
