@@ -305,7 +305,13 @@ static void build_tetrec(emc_ctx* c) {
 static std::string ensure_mg(emc_ctx* c) {
   if (!c->mg.lv.empty()) return "";
   const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  return mg_build(c->dev, c->mg, 600, nth);
+  // The coarsest level is solved by a dense inverse on the GPU (one wave per row):
+  // about n0/300 nodes, between 600 and 2500, trades the launches of small levels
+  // against the size of that product (1617 nodes and four coarse levels at the
+  // 639 k-node FinFET mesh).
+  const long long n0 = (long long)c->dev.nnode;
+  const int min_nodes = (int)std::max(600LL, std::min(2500LL, n0 / 300));
+  return mg_build(c->dev, c->mg, min_nodes, nth);
 }
 
 // ---------------------------------------------------------------- GPU setup

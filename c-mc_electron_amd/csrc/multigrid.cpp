@@ -132,34 +132,53 @@ double lambda_max(const Device& d, const MgHierarchy& h, int l) {
 }
 
 // dense inverse of an SPD matrix by Cholesky (n small)
-bool spd_inverse(std::vector<double>& a, int n) {
+bool spd_inverse(std::vector<double>& a, int n, int threads) {
+  // Cholesky A = L L^T, row by row (rows of L contiguous)
   std::vector<double> L((size_t)n * n, 0.0);
   for (int j = 0; j < n; ++j) {
+    const double* lj = &L[(size_t)j * n];
     double s = a[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    for (int k = 0; k < j; ++k) s -= lj[k] * lj[k];
     if (!(s > 0)) return false;
     const double ljj = std::sqrt(s);
     L[(size_t)j * n + j] = ljj;
     for (int i = j + 1; i < n; ++i) {
+      const double* li = &L[(size_t)i * n];
       double t = a[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      for (int k = 0; k < j; ++k) t -= li[k] * lj[k];
       L[(size_t)i * n + j] = t / ljj;
     }
   }
-  // inverse columns: solve L L^T x = e_c
-  std::vector<double> inv((size_t)n * n), y(n);
-  for (int c = 0; c < n; ++c) {
-    for (int i = 0; i < n; ++i) {
-      double t = i == c ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) t -= L[(size_t)i * n + k] * y[k];
-      y[i] = t / L[(size_t)i * n + i];
+  // U = L^T stored by rows, so the back substitution also reads rows
+  std::vector<double> U((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k <= i; ++k) U[(size_t)k * n + i] = L[(size_t)i * n + k];
+  // column c of the inverse: L y = e_c (y_i = 0 for i < c), then L^T x = y;
+  // columns are independent and split over the threads
+  std::vector<double> inv((size_t)n * n);
+  auto work = [&](int c0, int c1) {
+    std::vector<double> y(n), x(n);
+    for (int c = c0; c < c1; ++c) {
+      for (int i = 0; i < c; ++i) y[i] = 0.0;
+      for (int i = c; i < n; ++i) {
+        const double* li = &L[(size_t)i * n];
+        double t = i == c ? 1.0 : 0.0;
+        for (int k = c; k < i; ++k) t -= li[k] * y[k];
+        y[i] = t / li[i];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        const double* ui = &U[(size_t)i * n];
+        double t = y[i];
+        for (int k = i + 1; k < n; ++k) t -= ui[k] * x[k];
+        x[i] = t / ui[i];
+      }
+      for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = x[i];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double t = y[i];
-      for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
-      inv[(size_t)i * n + c] = t / L[(size_t)i * n + i];
-    }
-  }
+  };
+  const int nth = std::max(1, std::min(threads, n / 64 + 1));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t) th.emplace_back(work, (int)((long long)n * t / nth), (int)((long long)n * (t + 1) / nth));
+  for (auto& t : th) t.join();
   a.swap(inv);
   return true;
 }
@@ -245,7 +264,7 @@ std::string mg_build(const Device& d, MgHierarchy& h, int min_nodes, int threads
               a[n * nc + ((size_t)gj * Lc.nx + gi) * Lc.nz + gk] = op.at(n, i, j, k, di, dj, dk);
             }
       }
-  if (!spd_inverse(a, nc)) return "coarsest multigrid operator is not positive definite";
+  if (!spd_inverse(a, nc, threads)) return "coarsest multigrid operator is not positive definite";
   h.cinv.swap(a);
   return "";
 }
