@@ -77,7 +77,9 @@ FLY_KERNELS = ("k_fly_fast", "k_fly_slow")
 
 def _pmc_pass(counter, args, outdir):
     """One rocprofv3 counter pass over a child run (this process has not touched
-    the GPU yet).  Returns {kernel name: [values per dispatch]} or None."""
+    the GPU yet).  `counter` names one counter or several separated by spaces.
+    Returns {kernel name: [values per dispatch]} (one counter) or {counter: {kernel
+    name: [values]}} (several), or None."""
     import csv
     import glob
     import shutil
@@ -91,11 +93,13 @@ def _pmc_pass(counter, args, outdir):
     if not files:
         print(f"[bench] {counter} pass gave no counters (rc {r.returncode})", file=sys.stderr, flush=True)
         return None
-    out = {}
+    names = counter.split()
+    out = {c: {} for c in names}
     for row in csv.DictReader(open(files[0])):
-        if row["Counter_Name"] == counter:
-            out.setdefault(row["Kernel_Name"].split("(")[0].split("::")[-1], []).append(float(row["Counter_Value"]))
-    return out
+        if row["Counter_Name"] in out:
+            out[row["Counter_Name"]].setdefault(row["Kernel_Name"].split("(")[0].split("::")[-1], []).append(
+                float(row["Counter_Value"]))
+    return out[names[0]] if len(names) == 1 else out
 
 
 def measure_traffic(bench_args):
@@ -133,8 +137,15 @@ def measure_traffic(bench_args):
     f_fly, w_fly = per_step(sf), per_step(sw)
     if f_fly is None or w_fly is None:
         return None
-    return {"bytes": (f_fly * fscale + w_fly * wscale) * 1024.0, "fetch_kib": f_fly, "write_kib": w_fly,
-            "fetch_scale": fscale, "write_scale": wscale}
+    out = {"bytes": (f_fly * fscale + w_fly * wscale) * 1024.0, "fetch_kib": f_fly, "write_kib": w_fly,
+           "fetch_scale": fscale, "write_scale": wscale}
+    # the flight is latency bound: its L2 hit rate says where the table reads are served
+    hm = _pmc_pass("TCC_HIT_sum TCC_MISS_sum", bench_args, os.path.join(tmp, "hm"))
+    if hm:
+        hit, miss = per_step(hm["TCC_HIT_sum"]), per_step(hm["TCC_MISS_sum"])
+        if hit is not None and miss is not None and hit + miss > 0:
+            out["l2_hit_rate"] = hit / (hit + miss)
+    return out
 
 
 def copy_probe(mib):
@@ -246,6 +257,7 @@ def main():
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_counters": traffic,
+                     "l2_hit_rate": traffic.get("l2_hit_rate") if traffic else None,
                      "bytes_per_particle": FLY_BYTES_PER_PARTICLE,
                      "algorithmic_bytes": FLY_BYTES_PER_PARTICLE * n_avg},
         "cpu_baseline": cpu,

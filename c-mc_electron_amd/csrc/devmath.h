@@ -216,6 +216,14 @@ EMC_HD inline uint64_t clone_key(uint64_t parent, uint64_t step, int ordinal) {
   return mix64(parent ^ mix64(step + 0x632BE59BD9B4E019ull * (uint64_t)(ordinal + 1)));
 }
 EMC_HD inline uint64_t init_key(uint64_t seed, uint64_t serial) { return mix64(mix64(seed) + serial); }
+// MultipleRefresh: the draw stream of (cell, carrier type) and the key of new
+// particle i of that cell and type, copied from `parent`
+EMC_HD inline uint64_t mr_stream_key(long long cell, int type) {
+  return mix64(0x4D554C5449524546ull ^ mix64((uint64_t)(2 * cell + type) + 1u));
+}
+EMC_HD inline uint64_t mr_child_key(uint64_t parent, uint64_t step, long long i) {
+  return mix64(parent ^ mix64(step + 0xD1B54A32D192ED03ull * (uint64_t)(i + 1)));
+}
 
 // One particle's stream for one step: draw n is element n%2 of Philox block n/2;
 // 52-bit mantissa -> (m + 0.5) 2^-52, strictly inside (0,1).

@@ -94,6 +94,9 @@ struct Gpu {
   unsigned long long* ctr_host = nullptr;    // pinned copy of ctr after the flight
   unsigned long long* tally_host = nullptr;  // pinned copy of contact + counters after the flight
   bool pcg_pending = false;           // the step's PCG result is still to be read (scal_host[16..23])
+  // MultipleRefresh work arrays, allocated on first use
+  MrArgs mr{};
+  bool mr_ready = false;
   uint32_t* slow = nullptr;           // [cap] flight slow list
   // fields / mesh work arrays
   double *ex = nullptr, *ey = nullptr, *ez = nullptr, *rho = nullptr, *rates = nullptr;
@@ -142,6 +145,7 @@ struct emc_ctx {
   std::vector<SurfRec> surfrec;
   std::vector<CornerRec> cornerrec;
   std::vector<Sphere> tsph, tlist_s, tlists_s, slist_s;
+  std::vector<int> mr_desired_e, mr_desired_h;  // [ncell] MultipleRefresh target counts
   TetIndex tix;         // empty when build_tet_index refused the tables
   std::string tix_why;  // why it refused
   std::vector<int> surface_dir;
@@ -481,6 +485,27 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   return EMC_OK;
 }
 
+// MultipleRefresh target counts per cell (src/particle_init.cpp:44-70): the
+// deck's per-cell numbers, raised in silicon cells to the initial particle
+// count of the cell and to the default_par_number floor.
+static void mr_targets(emc_ctx* c) {
+  const Device& d = c->dev;
+  c->mr_desired_e = d.desired_e;
+  c->mr_desired_h = d.desired_h;
+  if (d.sum_charge[0] == 0.0 || d.sum_charge[1] == 0.0) return;
+  for (size_t cc = 0; cc < (size_t)d.ncell; ++cc) {
+    if (d.material[cc] != SILICON) continue;
+    const int ne = (int)(1e-10 + d.init_e_charge[cc] / d.sum_charge[0] * d.cfg.electron_number) + 1;
+    const int nh = (int)(1e-10 + d.init_h_charge[cc] / d.sum_charge[1] * d.cfg.hole_number) + 1;
+    int& de = c->mr_desired_e[cc];
+    int& dh = c->mr_desired_h[cc];
+    if (ne > de) de = ne;
+    if (nh > dh) dh = nh;
+    if (de < d.default_e) de = d.default_e;
+    if (dh < d.default_h) dh = d.default_h;
+  }
+}
+
 // ---------------------------------------------------------------- initial population
 // init_particle_data / compute_par_num / select_kstate (src/particle_init.cpp:17-219,
 // src/mcmodel.cpp:3006-3057), with the per-particle Philox stream of the build
@@ -624,6 +649,7 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
   c->surface_dir.assign(std::max(1, c->dev.nsurf), 0);
   for (int s = 0; s < c->dev.nsurf; ++s) c->surface_dir[s] = c->dev.surf_dir[s];
   build_tetrec(c);
+  mr_targets(c);
   if (c->opts.device >= 0) {
     int st = gpu_setup(c);
     if (st) return done(st);
@@ -744,6 +770,8 @@ int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
   v->node_kind = d.node_kind.data(); v->dirichlet = d.dirichlet_val.data();
   v->diag = d.diag.data(); v->cx = d.cx.data(); v->cy = d.cy.data(); v->cz = d.cz.data();
   v->rhs_corr = d.rhs_corr.data(); v->stencil = d.stencil.data();
+  v->mr_desired_e = c->mr_desired_e.empty() ? nullptr : c->mr_desired_e.data();
+  v->mr_desired_h = c->mr_desired_h.empty() ? nullptr : c->mr_desired_h.data();
   return EMC_OK;
 }
 
@@ -1113,7 +1141,69 @@ static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres) {
   return EMC_OK;
 }
 
+// MultipleRefresh (src/MultipleRefresh.cpp:61-143) of the device population:
+// resampled into the second particle buffer, which becomes the current one.
+static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after) {
+  Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
+  const Device& d = c->dev;
+  MrArgs& a = g.mr;
+  if (!g.mr_ready) {
+    DevBuf& m = g.mem;
+    const long long cap = g.cap, ng = 2 * (long long)d.ncell;
+    a.gk = m.alloc<unsigned>(cap); a.gk2 = m.alloc<unsigned>(cap);
+    a.idx = m.alloc<unsigned>(cap); a.idx2 = m.alloc<unsigned>(cap);
+    a.sk = m.alloc<unsigned long long>(cap); a.sk2 = m.alloc<unsigned long long>(cap);
+    a.gstart = m.alloc<unsigned>(ng); a.gend = m.alloc<unsigned>(ng);
+    a.cnt = m.alloc<unsigned long long>(ng); a.off = m.alloc<unsigned long long>(ng);
+    a.tot = m.alloc<double>(ng); a.keep = m.alloc<int>(ng);
+    a.nlive = m.alloc<unsigned long long>(1);
+    a.desired_e = m.dup(c->mr_desired_e);
+    a.desired_h = m.dup(c->mr_desired_h);
+    a.temp_bytes = emc_mr_temp_bytes(cap, ng);
+    a.temp = m.alloc<unsigned char>(a.temp_bytes > 0 ? a.temp_bytes : 1);
+    if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("MultipleRefresh buffers: ") + hipGetErrorString(m.last));
+    g.mr_ready = true;
+  }
+  a.P = g.P; a.Q = g.P2; a.n = g.n;
+  a.ngroup = 2 * (long long)d.ncell; a.cnx = d.cnx; a.cnz = d.cnz; a.step = step;
+  if (a.n <= 0) {
+    if (before) *before = 0;
+    if (after) *after = 0;
+    return EMC_OK;
+  }
+  HIPCHECK(c, emc_launch_zero(a.nlive, 1, g.stream));
+  HIPCHECK(c, emc_launch_mr_plan(a, g.stream));
+  unsigned long long h[3];
+  HIPCHECK(c, hipMemcpyAsync(&h[0], a.nlive, 8, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(&h[1], a.off + (a.ngroup - 1), 8, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(&h[2], a.cnt + (a.ngroup - 1), 8, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  const long long total = (long long)(h[1] + h[2]);
+  if (total > g.cap) return fail(c, EMC_ERR_CAPACITY, "MultipleRefresh: the refreshed population exceeds the device capacity");
+  HIPCHECK(c, emc_launch_mr_write(a, g.stream));
+  std::swap(g.P, g.P2);
+  g.n = total;
+  if (before) *before = (long long)h[0];
+  if (after) *after = total;
+  return EMC_OK;
+}
+
 extern "C" {
+
+int emc_multiple_refresh(emc_ctx* c, uint64_t step, int64_t* n_before, int64_t* n_after) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: MultipleRefresh runs on the GPU only");
+  if (c->gpu->cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  long long b = 0, a = 0;
+  const int st = run_mr(c, step, &b, &a);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(c->gpu->stream));
+  if (n_before) *n_before = b;
+  if (n_after) *n_after = a;
+  return EMC_OK;
+}
 
 int emc_fly_step(emc_ctx* c, uint64_t step, int flag_heat, emc_fly_tallies* out) {
   if (!c) return EMC_ERR_ARG;
@@ -1271,6 +1361,14 @@ int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
     if (st) return st;
     const int passes = finish_fly(c, fa);
     if (passes < 0) return passes;
+    // MultipleRefresh after the flight of every mr_step-th step (src/mcmodel.cpp:535-547)
+    const Config& cf = c->dev.cfg;
+    if (cf.flag_mr && cf.mr_step > 0 && step % (uint64_t)cf.mr_step == (uint64_t)cf.mr_step - 1) {
+      long long nb = 0, na = 0;
+      st = run_mr(c, step, &nb, &na);
+      if (st) return st;
+      if (acc) acc->n_mr_gen += na - nb;
+    }
     float ms;
     for (int q = 0; q < 4; ++q) {
       HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[q], g.ev[q + 1]));

@@ -137,8 +137,31 @@ struct MgArgs {
   int npart, pad;
 };
 
+// MultipleRefresh: the population P is resampled per (cell, carrier type) into Q.
+struct MrArgs {
+  PartDev P, Q;
+  long long n;                 // slots of P
+  long long ngroup;            // 2 * cells: group g = 2*cell + type
+  int cnx, cnz;
+  const int *desired_e, *desired_h;  // [ncell] target counts
+  unsigned long long step;
+  // work arrays: [n] sort keys, slots; [ngroup] per-group data
+  unsigned *gk, *gk2, *idx, *idx2;
+  unsigned long long *sk, *sk2;
+  unsigned *gstart, *gend;
+  unsigned long long *cnt, *off;
+  double* tot;
+  int* keep;
+  unsigned long long* nlive;   // live slots of P (zero on entry)
+  void* temp;                  // hipCUB storage
+  size_t temp_bytes;
+};
+
 }  // namespace emc
 
+size_t emc_mr_temp_bytes(long long n, long long ngroup);
+hipError_t emc_launch_mr_plan(const emc::MrArgs& a, hipStream_t s);
+hipError_t emc_launch_mr_write(const emc::MrArgs& a, hipStream_t s);
 hipError_t emc_launch_fly(const emc::FlyArgs& a, hipStream_t s);
 hipError_t emc_launch_fly_clones(const emc::FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s);
 hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long begin, unsigned long long end,

@@ -169,7 +169,7 @@ class Engine:
                      ("diag", nn), ("cx", nn), ("cy", nn), ("cz", nn), ("rhs_corr", nn), ("stencil", 7 * nn)):
             v[f] = _view(getattr(m, f), n, np.float64)
         for f, n in (("material", nc), ("attached", nc), ("nearest_surf", nc), ("eeff_dir", nc),
-                     ("surface_dir", max(1, m.nsurf))):
+                     ("surface_dir", max(1, m.nsurf)), ("mr_desired_e", nc), ("mr_desired_h", nc)):
             v[f] = _view(getattr(m, f), n, np.int32)
         for f, n in (("motion", 6 * nc), ("insurf", nc), ("node_kind", nn)):
             v[f] = _view(getattr(m, f), n, np.uint8)
@@ -253,6 +253,12 @@ class Engine:
         nodes = _view(b.tindex_node, 2 * b.tindex_nodes, np.int32).reshape(-1, 2).copy()
         ent = np.frombuffer((C.c_char * (b.tindex_entries * edt.itemsize)).from_address(b.tindex_entry), edt).copy()
         return roots, nodes, ent
+
+    def multiple_refresh(self, step):
+        """MultipleRefresh of the device population; returns (live before, after)."""
+        b, a = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.emc_multiple_refresh(self.ctx, step, C.byref(b), C.byref(a)))
+        return b.value, a.value
 
     def band_locate(self, k, energy, iband, itet_hint):
         """Getisym on the host tables for points k [n, 3]: returns (itet, isym),

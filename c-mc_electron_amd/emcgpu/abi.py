@@ -87,6 +87,7 @@ class MeshView(C.Structure):
         ("node_kind", _u8), ("dirichlet", _d),
         ("diag", _d), ("cx", _d), ("cy", _d), ("cz", _d),
         ("rhs_corr", _d), ("stencil", _d),
+        ("mr_desired_e", C.POINTER(C.c_int32)), ("mr_desired_h", C.POINTER(C.c_int32)),
     ]
 
 
@@ -107,6 +108,7 @@ class FlyTallies(C.Structure):
         ("n_catch", C.c_int64), ("n_gen", C.c_int64),
         ("n_phonon", C.c_int64), ("n_impurity", C.c_int64),
         ("n_events", C.c_int64), ("n_clone_passes", C.c_int64), ("n_full_loop", C.c_int64),
+        ("n_mr_gen", C.c_int64),
     ]
 
 
@@ -134,6 +136,7 @@ def declare(lib):
         "emc_band_view_get": (C.c_int, [ctx, P(BandView)]),
         "emc_mesh_view_get": (C.c_int, [ctx, P(MeshView)]),
         "emc_band_locate": (C.c_int, [ctx, C.c_int64, _d, _d, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "emc_multiple_refresh": (C.c_int, [ctx, C.c_uint64, P(C.c_int64), P(C.c_int64)]),
         "emc_particles_init": (C.c_int, [ctx, _i64]),
         "emc_particles_count": (C.c_int, [ctx, _i64]),
         "emc_particles_upload": (C.c_int, [ctx, P(Particles), C.c_int64]),

@@ -147,6 +147,10 @@ typedef struct emc_mesh_view {
   const double *diag, *cx, *cy, *cz;/* [nnode] eliminated SPD 7-point operator */
   const double *rhs_corr;           /* [nnode]                                */
   const double *stencil;            /* [nnode][7] operator as assembled        */
+  const int32_t *mr_desired_e, *mr_desired_h; /* [ncell] MultipleRefresh targets:
+                                       the deck's counts, raised in silicon to the
+                                       initial count and the default_par_number
+                                       floor (src/particle_init.cpp:44-70)       */
 } emc_mesh_view;
 
 /* Structure-of-arrays particle buffers (caller-owned on the host). */
@@ -176,6 +180,8 @@ typedef struct emc_fly_tallies {
   int64_t n_clone_passes;
   int64_t n_full_loop;       /* particles that needed the full event loop (the
                                 rest drifted to the end of the step with no event) */
+  int64_t n_mr_gen;          /* mr_gen_num: particles added (+) or removed (-) by
+                                MultipleRefresh (emc_step, Flag_MultipleRefresh) */
 } emc_fly_tallies;
 
 /* Options for emc_create. */
@@ -197,6 +203,19 @@ const char *emc_last_error(emc_ctx *ctx);
 
 int emc_band_view_get(emc_ctx *ctx, emc_band_view *out);
 int emc_mesh_view_get(emc_ctx *ctx, emc_mesh_view *out);
+
+/* MultipleRefresh (src/MultipleRefresh.cpp:9-143): per cell and carrier type, the
+ * particles with |q| >= 1e-30 are kept when their count is within a factor 2 of
+ * the cell's target (emc_mesh_view.mr_desired_*) and sum q^2 < 2 (sum q)^2 / n;
+ * otherwise the target number of copies is drawn with probability |q| / sum|q|,
+ * each with charge sum q / target.  Draws and keys are defined (the reference
+ * uses its global ran1): the draws of cell c and type t at `step` are the Philox
+ * stream of key mix64(0x4D554C5449524546 ^ mix64(2c + t + 1)), a cell's particles
+ * are taken in ascending key order, copy i of parent p gets key
+ * mix64(p ^ mix64(step + 0xD1B54A32D192ED03 (i + 1))).  The population is left in
+ * cell order (electrons, then holes, of each cell).  emc_step calls it after the
+ * flight of every mr_step-th step when the deck sets Flag_MultipleRefresh. */
+int emc_multiple_refresh(emc_ctx *ctx, uint64_t step, int64_t *n_before, int64_t *n_after);
 
 /* Getisym (src/ParticleUtils.cpp:395-625) on the host tables, the code the flight
  * kernels run: for each of n points k[q][0..2] (wave vector, normalised) of energy

@@ -17,36 +17,7 @@ enum { PELEC = 0, PHOLE = 1 };
 #define PI 3.141592653589793238462643e0
 #define LOOP_CAP 65536
 
-typedef struct {
-  uint32_t k0, k1, c1, c2;
-  uint32_t block;
-  int have;
-  double spare;
-} Rng;
-
-static double rng_next(Rng *r) {
-  if (r->have) {
-    r->have = 0;
-    return r->spare;
-  }
-  uint32_t c[4] = {r->block, r->c1, r->c2, 0u}, k[2] = {r->k0, r->k1}, o[4];
-  orc_philox4x32(c, k, o);
-  r->block++;
-  uint64_t a = (((uint64_t)o[1] << 32) | o[0]) >> 12, b = (((uint64_t)o[3] << 32) | o[2]) >> 12;
-  r->spare = ((double)b + 0.5) * 2.220446049250313080847263336181640625e-16;
-  r->have = 1;
-  return ((double)a + 0.5) * 2.220446049250313080847263336181640625e-16;
-}
-
-static void rng_init(Rng *r, uint64_t key, uint64_t step) {
-  r->k0 = (uint32_t)key;
-  r->k1 = (uint32_t)(key >> 32);
-  r->c1 = (uint32_t)step;
-  r->c2 = (uint32_t)(step >> 32);
-  r->block = 0;
-  r->have = 0;
-  r->spare = 0;
-}
+#include "orng.h"
 
 typedef struct {
   double x, y, z, kx, ky, kz, energy, charge, left_time;

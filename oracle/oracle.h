@@ -86,6 +86,14 @@ void orc_surface_rates(const emc_mesh_view *m, const double *ex, const double *e
 /* ---- FP64 Jacobi-PCG on the eliminated operator; returns iterations */
 int orc_pcg(const emc_mesh_view *m, const double *rhs_elim, double *x, double rtol, int maxit,
             double *relres);
+/* ---- MultipleRefresh (src/MultipleRefresh.cpp:9-143) of a host population into `out`
+ * (capacity cap); returns the new count or -1 when cap is too small.  Order and
+ * draws as defined in omr.c (the same as the GPU path). */
+int64_t orc_multiple_refresh(const emc_mesh_view *m, const emc_particles *in, int64_t n, const int32_t *desired_e,
+                             const int32_t *desired_h, uint64_t step, emc_particles *out, int64_t cap);
+uint64_t orc_mr_stream_key(int64_t cell, int type);
+uint64_t orc_mr_child_key(uint64_t parent, uint64_t step, int64_t i);
+
 /* y = A x with the operator as assembled by init_matrix (contact rows identity) */
 void orc_apply_full(const emc_mesh_view *m, const double *x, double *y);
 

@@ -41,6 +41,10 @@ def load():
         "orc_get_state": (C.c_int, [P(abi.BandView), C.c_uint64, C.c_uint64, C.c_double, C.c_int, C.c_int, d,
                                     P(C.c_int), P(C.c_int)]),
         "orc_getisym": (C.c_int, [P(abi.BandView), d, C.c_double, C.c_int, C.c_int, P(C.c_int), P(C.c_int)]),
+        "orc_multiple_refresh": (C.c_int64, [P(abi.MeshView), P(abi.Particles), C.c_int64, P(C.c_int32),
+                                             P(C.c_int32), C.c_uint64, P(abi.Particles), C.c_int64]),
+        "orc_mr_stream_key": (C.c_uint64, [C.c_int64, C.c_int]),
+        "orc_mr_child_key": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int64]),
         "orc_deposit": (C.c_int, [P(abi.BandView), P(abi.MeshView), P(abi.Particles), C.c_int64, i64, i64, i64, i64]),
         "orc_rhs": (None, [P(abi.MeshView), i64, d, d, d]),
         "orc_cell_rho": (None, [P(abi.MeshView), i64, d]),
@@ -98,3 +102,19 @@ def deposit(eng, parts, n):
     lib.orc_deposit(C.byref(eng.band), C.byref(eng.mesh), C.byref(ps), n, i64ptr(node_q), i64ptr(node_stat),
                     i64ptr(cell_q), i64ptr(cell_n))
     return node_q, node_stat, cell_q, cell_n
+
+
+def multiple_refresh(eng, parts, n, step, desired_e=None, desired_h=None, cap=None):
+    """The oracle's MultipleRefresh of `parts` (first n slots): a new Particles."""
+    lib = load()
+    m = eng.mesh_arrays()
+    de = np.ascontiguousarray(m["mr_desired_e"] if desired_e is None else desired_e, np.int32)
+    dh = np.ascontiguousarray(m["mr_desired_h"] if desired_h is None else desired_h, np.int32)
+    cap = cap if cap is not None else int(n + de.sum() + dh.sum() + 16)
+    out = emcgpu.ParticleArrays(cap)
+    ps, po = parts.struct(), out.struct()
+    k = lib.orc_multiple_refresh(C.byref(eng.mesh), C.byref(ps), n, de.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 dh.ctypes.data_as(C.POINTER(C.c_int32)), step, C.byref(po), cap)
+    assert k >= 0, "oracle MultipleRefresh: output capacity"
+    out.n = int(k)
+    return out

@@ -209,6 +209,35 @@ def test_consecutive_steps_deposit(eng):
     assert gen > 0, "no GENERATE clones: the merge and append paths were not exercised"
 
 
+def test_multiple_refresh_parity(eng):
+    """emc_multiple_refresh against the oracle on one population, slot for slot and
+    bit for bit: a flown population (holes included) with one particle of fifty
+    times the charge in every third cell (those groups are resampled), some
+    particles below pckill (dropped) and the rest kept."""
+    eng.particles_init()
+    pot, f = field_like(eng, seed=9)
+    eng.fields_set(f)
+    eng.fly_step(3)
+    p = eng.download()
+    m = eng.mesh
+    live = np.nonzero(p.i[:p.n] >= 0)[0]
+    c = (p.j[live].astype(np.int64) * m.cnx + p.i[live]) * m.cnz + p.k[live]
+    g = 2 * c + p.ptype[live]
+    first = np.unique(g, return_index=True)[1]
+    p.charge[live[first[c[first] % 3 == 0]]] *= 50.0
+    p.charge[live[::97]] = 1e-31
+    eng.upload(p)
+    print("[gpu] multiple refresh parity", flush=True)
+    before, after = eng.multiple_refresh(29)
+    pg = eng.download()
+    po = oracle_lib.multiple_refresh(eng, p, p.n, 29)
+    assert before == live.size
+    assert after == po.n == pg.n
+    for fld in emcgpu.PARTICLE_F64 + emcgpu.PARTICLE_I32 + ("key",):
+        np.testing.assert_array_equal(getattr(pg, fld)[:pg.n], getattr(po, fld)[:po.n], err_msg=fld)
+    assert not np.all(np.isin(pg.key[:pg.n], p.key[:p.n])), "no group was resampled"
+
+
 def test_poisson_multilevel_mesh():
     """60x120x36 cells (273 k nodes): the hierarchy has two grid-wide levels before
     the one-workgroup bottom, so every V-cycle kernel runs.  Random charge on the
