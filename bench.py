@@ -224,6 +224,12 @@ def main():
     print(f"[bench] {a.steps} steps in {elapsed:.3f}s; per step ms: deposit+rhs {phase[0]:.3f} "
           f"pcg {phase[1]:.3f} field {phase[2]:.3f} fly(all) {phase[3]:.3f} k_fly {phase[4]:.3f}; "
           f"pcg iters {np.mean(pcg_it):.1f}", file=sys.stderr, flush=True)
+    # MultipleRefresh of the same population, once, after the timed steps (the bench
+    # deck, like the reference's plain configuration, runs without it)
+    t_mr = time.perf_counter()
+    mr_before, mr_after = eng.multiple_refresh(10 ** 6)
+    mr_ms = (time.perf_counter() - t_mr) * 1e3
+    print(f"[bench] multiple refresh {mr_before} -> {mr_after} particles in {mr_ms:.2f} ms", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
         v, dt, nps = cpu_baseline(eng, a.cpu_steps)
@@ -251,6 +257,7 @@ def main():
                    "phase_ms": {"deposit_rhs": phase[0], "poisson": phase[1], "field": phase[2],
                                 "flight_total": phase[3], "k_fly": phase[4]},
                    "pcg_iters_mean": float(np.mean(pcg_it)),
+                   "multiple_refresh_ms": mr_ms,
                    "full_event_loop_fraction": full_loop / max(1, n_ps),
                    "events_per_particle_step": events / max(1, n_ps)},
         "roofline": {"bound": "hbm", "kernel": "k_fly_fast + k_fly_slow", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -1173,7 +1173,6 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
     if (after) *after = 0;
     return EMC_OK;
   }
-  HIPCHECK(c, emc_launch_zero(a.nlive, 1, g.stream));
   HIPCHECK(c, emc_launch_mr_plan(a, g.stream));
   unsigned long long h[3];
   HIPCHECK(c, hipMemcpyAsync(&h[0], a.nlive, 8, hipMemcpyDeviceToHost, g.stream));

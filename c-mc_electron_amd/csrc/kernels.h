@@ -152,7 +152,7 @@ struct MrArgs {
   unsigned long long *cnt, *off;
   double* tot;
   int* keep;
-  unsigned long long* nlive;   // live slots of P (zero on entry)
+  unsigned long long* nlive;   // live slots of P (set by k_mr_bounds)
   void* temp;                  // hipCUB storage
   size_t temp_bytes;
 };

@@ -1,9 +1,9 @@
 // MultipleRefresh (src/MultipleRefresh.cpp:9-143) on the GPU: per-cell,
 // per-carrier-type resampling of the population to the desired counts.
 //
-//   k_mr_keys     group (2*cell + type) and key of every slot; live count
+//   k_mr_keys     group (2*cell + type) and key of every slot
 //   radix sorts   by key, then (stable) by group: each group's particles in key order
-//   k_mr_bounds   first and last position of each group in the sorted order
+//   k_mr_bounds   first and last position of each group in the sorted order; live count
 //   k_mr_plan     per group: charge sums, the keep-or-resample test, output count
 //   scan          output offsets
 //   k_mr_write    per group: the kept particles, or new_num copies drawn with
@@ -30,17 +30,13 @@ __device__ inline long long mr_cell(const MrArgs& a, uint32_t pk) {
 }
 
 __global__ __launch_bounds__(256) void k_mr_keys(MrArgs a) {
-  long long live = 0;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (long long)gridDim.x * blockDim.x) {
     const uint32_t pk = a.P.pk[q];
     const bool on = pk != PK_DEAD;
     a.gk[q] = on ? (unsigned)(2 * mr_cell(a, pk) + pk_type(pk)) : (unsigned)a.ngroup;
     a.sk[q] = a.P.key[q];
     a.idx[q] = (unsigned)q;
-    live += on;
   }
-  for (int o = 32; o > 0; o >>= 1) live += __shfl_xor(live, o, 64);
-  if ((threadIdx.x & 63) == 0 && live) atomicAdd(a.nlive, (unsigned long long)live);
 }
 
 // group key of each position after the sort by particle key
@@ -56,11 +52,16 @@ __global__ __launch_bounds__(256) void k_mr_clear(MrArgs a) {
   }
 }
 
-// after the sorts: gk (group) and idx (slot) in (group, key) order
+// after the sorts: gk (group) and idx (slot) in (group, key) order; empty slots
+// sort last, so the live count is the position of the first of them
 __global__ __launch_bounds__(256) void k_mr_bounds(MrArgs a) {
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += (long long)gridDim.x * blockDim.x) {
     const unsigned g = a.gk[p];
-    if (g >= (unsigned)a.ngroup) continue;
+    if (g >= (unsigned)a.ngroup) {
+      if (p == 0 || a.gk[p - 1] < (unsigned)a.ngroup) *a.nlive = (unsigned long long)p;
+      continue;
+    }
+    if (p == a.n - 1) *a.nlive = (unsigned long long)a.n;
     if (p == 0 || a.gk[p - 1] != g) a.gstart[g] = (unsigned)p;
     if (p == a.n - 1 || a.gk[p + 1] != g) a.gend[g] = (unsigned)(p + 1);
   }
@@ -168,8 +169,8 @@ size_t emc_mr_temp_bytes(long long n, long long ngroup) {
   return std::max(a, std::max(b, c));
 }
 
-// Everything up to the output offsets (a.nlive must read zero on entry); the
-// output count is off[ngroup-1] + cnt[ngroup-1].
+// Everything up to the output offsets and the live count; the output count is
+// off[ngroup-1] + cnt[ngroup-1].
 hipError_t emc_launch_mr_plan(const MrArgs& a, hipStream_t s) {
   const int g = mr_grid(a.n), gg = mr_grid(a.ngroup);
   hipLaunchKernelGGL(k_mr_keys, dim3(g), dim3(256), 0, s, a);
