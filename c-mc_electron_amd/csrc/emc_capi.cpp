@@ -191,6 +191,8 @@ static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
   B.ntlists = b.ntlists.data(); B.ptlists = b.ptlists.data(); B.tlists = b.tlists.data();
   B.ibt = b.ibt.data();
   B.energy = b.energy.data(); B.dostetmax = b.dostetmax.data(); B.sumscatt = b.sumscatt.data();
+  B.dos = b.dos.data(); B.sumdos = b.sumdos.data();
+  for (int q = 0; q < 3; ++q) B.dosmax[q] = b.DOSMAX[q];
   B.scatte = b.scatte.data(); B.dose = b.dose.data(); B.scattiie = b.scattiie.data();
   B.scatth = b.scatth.data(); B.dosh = b.dosh.data(); B.scattiih = b.scattiih.data();
   B.nt = b.nt; B.scpre = b.scpre; B.scprh = b.scprh;
@@ -360,6 +362,7 @@ static int gpu_setup(emc_ctx* c) {
   B.ntlists = m.dup(b.ntlists); B.ptlists = m.dup(b.ptlists); B.tlists = m.dup(b.tlists);
   B.ibt = m.dup(b.ibt);
   B.energy = m.dup(b.energy); B.dostetmax = m.dup(b.dostetmax); B.sumscatt = m.dup(b.sumscatt);
+  B.dos = m.dup(b.dos); B.sumdos = m.dup(b.sumdos);
   B.scatte = m.dup(b.scatte); B.dose = m.dup(b.dose); B.scattiie = m.dup(b.scattiie);
   B.scatth = m.dup(b.scatth); B.dosh = m.dup(b.dosh); B.scattiih = m.dup(b.scattiih);
 
@@ -509,12 +512,12 @@ static void mr_targets(emc_ctx* c) {
 // ---------------------------------------------------------------- initial population
 // init_particle_data / compute_par_num / select_kstate (src/particle_init.cpp:17-219,
 // src/mcmodel.cpp:3006-3057), with the per-particle Philox stream of the build
-// (draw stream step = 2^64-1 reserved for initialisation).
-static void sample_population(emc_ctx* c) {
-  const Device& d = c->dev;
-  const Band& b = c->band;
-  struct Spec { int i, j, k, type; double q; };
-  std::vector<Spec> spec;
+// (draw stream step = 2^64-1 reserved for initialisation).  Particle s of the
+// population is number s of the reference's creation order: type, then cells
+// with i outer, k, j inner, then the cell's count; its key is init_key(seed, s).
+struct SpecGroup { int i, j, k, type, count; double q; };
+static std::vector<SpecGroup> spec_groups(const Device& d) {
+  std::vector<SpecGroup> gr;
   std::vector<int> ne(d.ncell, 0), nh(d.ncell, 0);
   for (int i = 0; i < d.cnx; ++i)
     for (int k = 0; k < d.cnz; ++k)
@@ -532,8 +535,18 @@ static void sample_population(emc_ctx* c) {
           if (d.material[cc] != SILICON) continue;
           const int pn = t == 0 ? ne[cc] : nh[cc];
           const double q = (t == 0 ? d.init_e_charge[cc] : d.init_h_charge[cc]) / pn;
-          for (int a = 0; a < pn; ++a) spec.push_back(Spec{i, j, k, t, q});
+          gr.push_back(SpecGroup{i, j, k, t, pn, q});
         }
+  return gr;
+}
+
+static void sample_population(emc_ctx* c) {
+  const Device& d = c->dev;
+  const Band& b = c->band;
+  struct Spec { int i, j, k, type; double q; };
+  std::vector<Spec> spec;
+  for (const SpecGroup& g : spec_groups(d))
+    for (int a = 0; a < g.count; ++a) spec.push_back(Spec{g.i, g.j, g.k, g.type, g.q});
   HostParticles& hp = c->hp;
   const size_t n = spec.size();
   hp.resize(n);
@@ -558,24 +571,7 @@ static void sample_population(emc_ctx* c) {
       lam = f.Random();
       hp.z[s] = lam * d.lz[sp.k] + (1 - lam) * d.lz[sp.k + 1];
       f.par_type = sp.type;
-      f.itet = 0;
-      for (;;) {
-        f.energy = -dm_log(1 - f.Random() * tmp_const);
-        const double ddos = b.caldossum(f.energy, sp.type);
-        double rdos = f.Random() * b.DOSMAX[sp.type];
-        if (rdos > ddos) continue;
-        int ib = b.bandof[sp.type];
-        for (; ib < b.bandof[sp.type] + b.nband[sp.type]; ++ib) {
-          rdos -= b.caldos(f.energy, ib);
-          if (rdos < 0) break;
-        }
-        if (ib >= b.bandof[sp.type] + b.nband[sp.type]) ib = b.bandof[sp.type] + b.nband[sp.type] - 1;
-        f.iband = ib;
-        f.isym = ((int)(f.Random() * 48)) % 48;
-        f.get_state();
-        if (f.Flag_SelfScatter) continue;
-        break;
-      }
+      f.sample_kstate(tmp_const);
       hp.kx[s] = f.kx; hp.ky[s] = f.ky; hp.kz[s] = f.kz; hp.energy[s] = f.energy;
       hp.charge[s] = sp.q;
       hp.i[s] = sp.i; hp.j[s] = sp.j; hp.k[s] = sp.k;
@@ -613,6 +609,52 @@ static int gpu_upload_host(emc_ctx* c) {
   HIPCHECK(c, up(g.P.pk, pk.data(), n * 4)); HIPCHECK(c, up(g.P.itet, hp.itet.data(), n * 4));
   HIPCHECK(c, up(g.P.isym, hp.isym.data(), n * 4)); HIPCHECK(c, up(g.P.key, hp.key.data(), n * 8));
   g.n = n;
+  return EMC_OK;
+}
+
+static int sync_fly_const(emc_ctx* c);
+
+// The same sampling on the GPU, one lane per particle (k_sample_population): the
+// particles are written straight into the device population, identical to the
+// host sampler's slot for slot.
+static int gpu_sample_population(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  const std::vector<SpecGroup> gr = spec_groups(c->dev);
+  std::vector<unsigned long long> start(gr.size() + 1, 0);
+  std::vector<uint32_t> gpk(gr.size());
+  std::vector<double> gq(gr.size());
+  for (size_t q = 0; q < gr.size(); ++q) {
+    start[q + 1] = start[q] + (unsigned long long)gr[q].count;
+    gpk[q] = pk_make(gr[q].type, gr[q].i, gr[q].j, gr[q].k);
+    gq[q] = gr[q].q;
+  }
+  const long long n = (long long)start.back();
+  const long long want = (long long)std::ceil(n * std::max(1.0, c->opts.capacity_factor)) + 1024;
+  st = gpu_alloc_particles(c, want);
+  if (st) return st;
+  if (n > g.cap) return fail(c, EMC_ERR_CAPACITY, "particle count exceeds the device capacity");
+  st = sync_fly_const(c);
+  if (st) return st;
+  DevBuf tmp;
+  SampleArgs a{};
+  a.C = g.cst; a.P = g.P; a.n = n; a.ngroup = (long long)gr.size();
+  a.start = tmp.dup(start); a.gpk = tmp.dup(gpk); a.gq = tmp.dup(gq);
+  a.seed = c->opts.seed;
+  a.tmp_const = 1 - std::exp(-c->band.emax);
+  if (tmp.last != hipSuccess) {
+    tmp.free_all();
+    return fail(c, EMC_ERR_HIP, std::string("sampling tables: ") + hipGetErrorString(tmp.last));
+  }
+  hipError_t e = emc_launch_sample(a, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  tmp.free_all();
+  if (e != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("initial sampling: ") + hipGetErrorString(e));
+  g.n = n;
+  g.n_pending = false;
+  c->hp.resize(0);
   return EMC_OK;
 }
 
@@ -777,9 +819,14 @@ int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
 
 int emc_particles_init(emc_ctx* c, int64_t* n_out) {
   if (!c) return EMC_ERR_ARG;
+  if (c->gpu) {
+    const int st = gpu_sample_population(c);
+    if (st) return st;
+    if (n_out) *n_out = c->gpu->n;
+    return EMC_OK;
+  }
   sample_population(c);
   if (n_out) *n_out = (int64_t)c->hp.size();
-  if (c->gpu) return gpu_upload_host(c);
   return EMC_OK;
 }
 
@@ -1190,6 +1237,112 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
 }
 
 extern "C" {
+
+// ---- checkpoint: the device state a later load continues from exactly
+static const char CKPT_MAGIC[8] = {'E', 'M', 'C', 'C', 'K', 'P', 'T', '1'};
+
+int emc_checkpoint_save(emc_ctx* c, const char* path) {
+  if (!c || !path) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the checkpoint holds device state");
+  Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  const Device& d = c->dev;
+  const long long n = g.n;
+  const size_t NN = d.nnode;
+  std::vector<uint32_t> pk(n);
+  HIPCHECK(c, hipMemcpy(pk.data(), g.P.pk, n * 4, hipMemcpyDeviceToHost));
+  std::vector<long long> live;
+  live.reserve(n);
+  for (long long s = 0; s < n; ++s)
+    if (pk[s] != PK_DEAD) live.push_back(s);
+  const long long nl = (long long)live.size();
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, EMC_ERR_IO, std::string("cannot write ") + path);
+  bool ok = true;
+  auto put = [&](const void* p, size_t bytes) { ok = ok && std::fwrite(p, 1, bytes, f) == bytes; };
+  const uint32_t version = 1;
+  const int32_t dims[4] = {d.cnx, d.cny, d.cnz, 0};
+  const uint64_t seed = c->opts.seed, next = c->next_step;
+  put(CKPT_MAGIC, 8); put(&version, 4); put(dims, 16); put(&seed, 8); put(&next, 8); put(&nl, 8);
+  // live particles, compacted in slot order
+  auto put_live = [&](const void* dptr, size_t width) {
+    std::vector<unsigned char> all((size_t)n * width), sel((size_t)nl * width);
+    if (hipMemcpy(all.data(), dptr, (size_t)n * width, hipMemcpyDeviceToHost) != hipSuccess) { ok = false; return; }
+    for (long long q = 0; q < nl; ++q) std::memcpy(&sel[(size_t)q * width], &all[(size_t)live[q] * width], width);
+    put(sel.data(), sel.size());
+  };
+  for (const void* a : {(const void*)g.P.x, (const void*)g.P.y, (const void*)g.P.z, (const void*)g.P.kx,
+                        (const void*)g.P.ky, (const void*)g.P.kz, (const void*)g.P.energy, (const void*)g.P.charge})
+    put_live(a, 8);
+  put_live(g.P.pk, 4); put_live(g.P.itet, 4); put_live(g.P.isym, 4); put_live(g.P.key, 8);
+  auto put_dev = [&](const void* dptr, size_t bytes) {
+    std::vector<unsigned char> h(bytes);
+    if (hipMemcpy(h.data(), dptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) { ok = false; return; }
+    put(h.data(), bytes);
+  };
+  put_dev(g.pot, NN * 8);
+  put_dev(g.node_stat, 10 * NN * 8);
+  put_dev(g.heat_e, NN * 8); put_dev(g.heat_h, NN * 8);
+  put_dev(g.rcur, (size_t)d.pny * 8); put_dev(g.lcur, (size_t)d.pny * 8);
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? EMC_OK : fail(c, EMC_ERR_IO, std::string("writing ") + path + " failed");
+}
+
+int emc_checkpoint_load(emc_ctx* c, const char* path) {
+  if (!c || !path) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the checkpoint holds device state");
+  Gpu& g = *c->gpu;
+  int st = settle_n(c);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  const Device& d = c->dev;
+  const size_t NN = d.nnode;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, EMC_ERR_IO, std::string("cannot read ") + path);
+  bool ok = true;
+  auto get = [&](void* p, size_t bytes) { ok = ok && std::fread(p, 1, bytes, f) == bytes; };
+  char magic[8];
+  uint32_t version = 0;
+  int32_t dims[4];
+  uint64_t seed = 0, next = 0;
+  long long nl = 0;
+  get(magic, 8); get(&version, 4); get(dims, 16); get(&seed, 8); get(&next, 8); get(&nl, 8);
+  if (!ok || std::memcmp(magic, CKPT_MAGIC, 8) != 0 || version != 1) {
+    std::fclose(f);
+    return fail(c, EMC_ERR_IO, std::string(path) + " is not a checkpoint of this format");
+  }
+  if (dims[0] != d.cnx || dims[1] != d.cny || dims[2] != d.cnz || nl < 0) {
+    std::fclose(f);
+    return fail(c, EMC_ERR_DECK, "checkpoint mesh does not match the deck");
+  }
+  const long long want = (long long)std::ceil(nl * std::max(1.0, c->opts.capacity_factor)) + 1024;
+  st = gpu_alloc_particles(c, want);
+  if (st) { std::fclose(f); return st; }
+  if (nl > g.cap) { std::fclose(f); return fail(c, EMC_ERR_CAPACITY, "checkpoint population exceeds the device capacity"); }
+  auto get_dev = [&](void* dptr, size_t bytes) {
+    std::vector<unsigned char> h(bytes);
+    get(h.data(), bytes);
+    if (ok && hipMemcpy(dptr, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
+  };
+  for (void* a : {(void*)g.P.x, (void*)g.P.y, (void*)g.P.z, (void*)g.P.kx, (void*)g.P.ky, (void*)g.P.kz,
+                  (void*)g.P.energy, (void*)g.P.charge})
+    get_dev(a, (size_t)nl * 8);
+  get_dev(g.P.pk, (size_t)nl * 4); get_dev(g.P.itet, (size_t)nl * 4); get_dev(g.P.isym, (size_t)nl * 4);
+  get_dev(g.P.key, (size_t)nl * 8);
+  get_dev(g.pot, NN * 8);
+  get_dev(g.node_stat, 10 * NN * 8);
+  get_dev(g.heat_e, NN * 8); get_dev(g.heat_h, NN * 8);
+  get_dev(g.rcur, (size_t)d.pny * 8); get_dev(g.lcur, (size_t)d.pny * 8);
+  std::fclose(f);
+  if (!ok) return fail(c, EMC_ERR_IO, std::string("reading ") + path + " failed");
+  g.n = nl;
+  g.n_pending = false;
+  c->next_step = next;
+  c->hp.resize(0);
+  return EMC_OK;
+}
 
 int emc_multiple_refresh(emc_ctx* c, uint64_t step, int64_t* n_before, int64_t* n_after) {
   if (!c) return EMC_ERR_ARG;

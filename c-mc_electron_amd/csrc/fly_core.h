@@ -186,6 +186,48 @@ struct Flight {
     return B.sumscatt[(size_t)itab * MNB + ib] +
            intp * (B.sumscatt[(size_t)(itab + 1) * MNB + ib] - B.sumscatt[(size_t)itab * MNB + ib]);
   }
+  // Band::CALDOS / CALDOSSUM (src/Band.cpp:2234-2295)
+  EMC_HD EMC_INL double caldos(double e, int ib) const {
+    const double* d = &B.dos[(size_t)ib * NTAB1];
+    if (e <= 0e0) return d[0];
+    if (e >= B.emax) return d[MTAB];
+    const int itab = (int)(e / B.dtable);
+    const double intp = (e - B.energy[itab]) / (B.energy[itab + 1] - B.energy[itab]);
+    return d[itab] + intp * (d[itab + 1] - d[itab]);
+  }
+  EMC_HD EMC_INL double caldossum(double e, int ip) const {
+    if (e <= 0e0) return B.sumdos[0 * 3 + ip];
+    if (e >= B.emax) return B.sumdos[(size_t)MTAB * 3 + ip];
+    const int itab = (int)(e / B.dtable);
+    const double intp = (e - B.energy[itab]) / (B.energy[itab + 1] - B.energy[itab]);
+    return B.sumdos[(size_t)itab * 3 + ip] +
+           intp * (B.sumdos[(size_t)(itab + 1) * 3 + ip] - B.sumdos[(size_t)itab * 3 + ip]);
+  }
+  // select_kstate (src/mcmodel.cpp:3006-3057) for a new carrier of type par_type:
+  // a Boltzmann energy accepted against the total DOS, a band by its share of the
+  // DOS, a random symmetry operation, then get_state; tmp_const = 1 - exp(-emax).
+  EMC_HD EMC_INL void sample_kstate(double tmp_const) {
+    const int b0 = par_type == 0 ? B.bandofE : B.bandofH;
+    const int nb = par_type == 0 ? B.nbandE : B.nbandH;
+    itet = 0;
+    for (;;) {
+      energy = -dm_log(1 - Random() * tmp_const);
+      const double ddos = caldossum(energy, par_type);
+      double rdos = Random() * B.dosmax[par_type];
+      if (rdos > ddos) continue;
+      int ib = b0;
+      for (; ib < b0 + nb; ++ib) {
+        rdos -= caldos(energy, ib);
+        if (rdos < 0) break;
+      }
+      if (ib >= b0 + nb) ib = b0 + nb - 1;
+      iband = ib;
+      isym = ((int)(Random() * 48)) % 48;
+      get_state();
+      if (Flag_SelfScatter) continue;
+      break;
+    }
+  }
   EMC_HD EMC_INL double caldostetmax(double e, int ib) const {
     const double* d = &B.dostetmax[(size_t)ib * NTAB1];
     if (e <= 0.0) return d[0];

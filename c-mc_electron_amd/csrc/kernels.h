@@ -137,6 +137,19 @@ struct MgArgs {
   int npart, pad;
 };
 
+// Initial population on the device: particle s belongs to spec group g with
+// start[g] <= s < start[g+1] (type|cell in gpk[g], charge gq[g]).
+struct SampleArgs {
+  const FlyConst* C;
+  PartDev P;
+  long long n, ngroup;
+  const unsigned long long* start;  // [ngroup + 1]
+  const uint32_t* gpk;
+  const double* gq;
+  unsigned long long seed;
+  double tmp_const;                 // 1 - exp(-emax)
+};
+
 // MultipleRefresh: the population P is resampled per (cell, carrier type) into Q.
 struct MrArgs {
   PartDev P, Q;
@@ -159,6 +172,7 @@ struct MrArgs {
 
 }  // namespace emc
 
+hipError_t emc_launch_sample(const emc::SampleArgs& a, hipStream_t s);
 size_t emc_mr_temp_bytes(long long n, long long ngroup);
 hipError_t emc_launch_mr_plan(const emc::MrArgs& a, hipStream_t s);
 hipError_t emc_launch_mr_write(const emc::MrArgs& a, hipStream_t s);

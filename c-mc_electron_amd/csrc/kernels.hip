@@ -1014,6 +1014,40 @@ __global__ __launch_bounds__(256) void k_mg_dot_rz(MgArgs m, const double* z, do
   block_reduce3(rz, 0.0, 0.0, part_rz);
 }
 
+// ------------------------------------------------------------------ initial population
+// init_particle_data (src/particle_init.cpp:112-219) for particle s: position
+// uniform in its cell, then select_kstate (Flight::sample_kstate); draws from
+// the particle's stream at the reserved step 2^64-1, as the host sampler.
+__global__ __launch_bounds__(256) void k_sample_population(SampleArgs a) {
+  const TallyDev T{};
+  for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < a.n; s += (long long)gridDim.x * blockDim.x) {
+    long long lo = 0, hi = a.ngroup - 1;  // last group with start <= s
+    while (lo < hi) {
+      const long long mid = (lo + hi + 1) >> 1;
+      if ((long long)a.start[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t pk = a.gpk[lo];
+    const int i = pk_i(pk), j = pk_j(pk), k = pk_k(pk);
+    const MeshDev& M = a.C->M;
+    Flight f(a.C->B, M, a.C->F, T);
+    const uint64_t key = init_key(a.seed, (uint64_t)s);
+    f.rng.init(key, ~0ull);
+    double lam = f.Random();
+    const double x = lam * M.lx[i] + (1 - lam) * M.lx[i + 1];
+    lam = f.Random();
+    const double y = lam * M.ly[j] + (1 - lam) * M.ly[j + 1];
+    lam = f.Random();
+    const double z = lam * M.lz[k] + (1 - lam) * M.lz[k + 1];
+    f.par_type = pk_type(pk);
+    f.sample_kstate(a.tmp_const);
+    a.P.x[s] = x; a.P.y[s] = y; a.P.z[s] = z;
+    a.P.kx[s] = f.kx; a.P.ky[s] = f.ky; a.P.kz[s] = f.kz;
+    a.P.energy[s] = f.energy; a.P.charge[s] = a.gq[lo];
+    a.P.pk[s] = pk; a.P.itet[s] = f.itet; a.P.isym[s] = f.isym; a.P.key[s] = key;
+  }
+}
+
 // ------------------------------------------------------------------ helpers
 __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = 0;
@@ -1089,6 +1123,11 @@ hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
 }
 hipError_t emc_launch_field(const FieldArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_field, dim3(grid_for((long long)a.ncell, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t emc_launch_sample(const SampleArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_population, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {

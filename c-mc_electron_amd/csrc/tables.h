@@ -82,6 +82,8 @@ struct BandDev {
   const int *ntlist, *ptlist, *tlist, *nslist, *pslist, *slist, *ntlists, *ptlists, *tlists;
   const int* ibt;
   const double *energy, *dostetmax, *sumscatt, *scatte, *dose, *scattiie, *scatth, *dosh, *scattiih;
+  const double *dos, *sumdos;  // [MNB][NTAB1], [NTAB1][3]: the initial-state sampler's DOS
+  double dosmax[3];
   int nt, scpre, scprh, nbandE, nbandH, bandofE, bandofH, pad0;
   double dtable, emax, dlist, dlists, a0pi, sieg, mell, melt, meld;
   double difpr[3];

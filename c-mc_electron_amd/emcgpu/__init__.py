@@ -254,6 +254,12 @@ class Engine:
         ent = np.frombuffer((C.c_char * (b.tindex_entries * edt.itemsize)).from_address(b.tindex_entry), edt).copy()
         return roots, nodes, ent
 
+    def checkpoint_save(self, path):
+        self._check(self.lib.emc_checkpoint_save(self.ctx, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        self._check(self.lib.emc_checkpoint_load(self.ctx, os.fsencode(path)))
+
     def multiple_refresh(self, step):
         """MultipleRefresh of the device population; returns (live before, after)."""
         b, a = C.c_int64(0), C.c_int64(0)

@@ -217,6 +217,19 @@ int emc_mesh_view_get(emc_ctx *ctx, emc_mesh_view *out);
  * flight of every mr_step-th step when the deck sets Flag_MultipleRefresh. */
 int emc_multiple_refresh(emc_ctx *ctx, uint64_t step, int64_t *n_before, int64_t *n_after);
 
+/* Checkpoint and resume (the reference's output_for_restart / Flag_restart,
+ * src/mcmodel.cpp:853-965, in a format of this library): save writes the device
+ * state a later load continues from exactly -- the live particles (compacted),
+ * the potential (the next solve's warm start), the step counter, the window
+ * statistics, heat and y-plane current accumulators.  Stepping a loaded context
+ * gives the same particles, tallies and potential, bit for bit, as stepping the
+ * saved one.  The file is binary, native byte order: "EMCCKPT1", u32 version 1,
+ * i32 cnx cny cnz pad, u64 seed, u64 next step, i64 n, then the particle arrays
+ * (x y z kx ky kz energy charge f64, packed cell u32, itet isym i32, key u64) and
+ * the node/plane arrays. */
+int emc_checkpoint_save(emc_ctx *ctx, const char *path);
+int emc_checkpoint_load(emc_ctx *ctx, const char *path);
+
 /* Getisym (src/ParticleUtils.cpp:395-625) on the host tables, the code the flight
  * kernels run: for each of n points k[q][0..2] (wave vector, normalised) of energy
  * energy[q] in band iband[q], isym[q] is the symmetry operation mapping k into the

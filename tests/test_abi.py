@@ -73,3 +73,15 @@ def test_upload_validates_indices():
         eng.upload(p)
     assert e.value.code == emcgpu.abi.EMC_ERR_ARG
     eng.close()
+
+
+def test_checkpoint_and_refresh_need_a_gpu(tmp_path):
+    eng = emcgpu.Engine(cfg1_deck(), device=-1)
+    try:
+        lib = eng.lib
+        assert lib.emc_checkpoint_save(eng.ctx, str(tmp_path / "x").encode()) == emcgpu.abi.EMC_ERR_NOGPU
+        assert lib.emc_checkpoint_load(eng.ctx, str(tmp_path / "x").encode()) == emcgpu.abi.EMC_ERR_NOGPU
+        b, a = C.c_int64(0), C.c_int64(0)
+        assert lib.emc_multiple_refresh(eng.ctx, 9, C.byref(b), C.byref(a)) == emcgpu.abi.EMC_ERR_NOGPU
+    finally:
+        eng.close()

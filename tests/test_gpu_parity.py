@@ -238,6 +238,49 @@ def test_multiple_refresh_parity(eng):
     assert not np.all(np.isin(pg.key[:pg.n], p.key[:p.n])), "no group was resampled"
 
 
+def test_device_initial_sampling_equals_host(eng):
+    """emc_particles_init on the GPU (k_sample_population) gives the host
+    sampler's population slot for slot and bit for bit (same code, same draws)."""
+    print("[gpu] device initial sampling", flush=True)
+    eng.particles_init()
+    pg = eng.download()
+    h = emcgpu.Engine(cfg1_deck(), device=-1)
+    try:
+        h.particles_init()
+        ph = h.download()
+    finally:
+        h.close()
+    assert pg.n == ph.n
+    for fld in emcgpu.PARTICLE_F64 + emcgpu.PARTICLE_I32 + ("key",):
+        np.testing.assert_array_equal(getattr(pg, fld)[:pg.n], getattr(ph, fld)[:ph.n], err_msg=fld)
+
+
+def test_checkpoint_resume(tmp_path):
+    """Save after three steps, then four more steps from the saved context and from
+    a fresh context that loaded the file: same particles, tallies and potential."""
+    path = str(tmp_path / "emc.ckpt")
+    a = emcgpu.Engine(cfg1_deck(), device=0)
+    b = emcgpu.Engine(cfg1_deck(), device=0)
+    try:
+        a.particles_init()
+        a.step(3)
+        a.checkpoint_save(path)
+        print("[gpu] checkpoint saved, stepping both contexts", flush=True)
+        ta = a.step(4)
+        b.checkpoint_load(path)
+        tb = b.step(4)
+        assert_same_population(a.download(), b.download(), "resume")
+        assert_same_tallies(ta, tb, "resume")
+        np.testing.assert_array_equal(a.potential(), b.potential())
+        for x, y in zip(a.heat(), b.heat()):
+            np.testing.assert_array_equal(x, y)
+        for x, y in zip(a.currents(), b.currents()):
+            np.testing.assert_array_equal(x, y)
+    finally:
+        a.close()
+        b.close()
+
+
 def test_poisson_multilevel_mesh():
     """60x120x36 cells (273 k nodes): the hierarchy has two grid-wide levels before
     the one-workgroup bottom, so every V-cycle kernel runs.  Random charge on the
