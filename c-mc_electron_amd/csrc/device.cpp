@@ -78,6 +78,8 @@ static std::string parse_input(Config& c, const std::string& path) {
     gb("Flag_compute_heat", c.flag_heat);
     gb("Flag_QuantumCorrection", c.flag_qc);
     gb("Flag_restart", c.flag_restart);
+    gi("restart_step", c.restart_step);
+    gs("restart_filename", c.restart_file);
   } catch (...) {
     return "input.txt: malformed numeric value";
   }

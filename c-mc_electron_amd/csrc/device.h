@@ -35,6 +35,8 @@ struct Config {
   int stat_step = 5000, stat_heat_step = 5000, heat_steps = 20000, mr_step = 10;
   bool flag_mr = false, flag_surface = true, flag_rough = true, flag_sphonon = true;
   bool flag_simp = false, flag_heat = false, flag_qc = false, flag_restart = false;
+  int restart_step = 0;
+  std::string restart_file = "nosuchname";  // resolved against dir when relative
   std::map<std::string, std::string> raw;
 };
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <thread>
@@ -18,6 +19,7 @@
 #include "fly_core.h"
 #include "kernels.h"
 #include "multigrid.h"
+#include "output.h"
 #include "tetindex.h"
 
 using namespace emc;
@@ -103,6 +105,11 @@ struct Gpu {
   unsigned long long *node_q = nullptr, *node_stat = nullptr, *cell_q = nullptr, *cell_n = nullptr;
   unsigned long long *heat_e = nullptr, *heat_h = nullptr, *rcur = nullptr, *lcur = nullptr;
   unsigned long long *contact = nullptr, *counters = nullptr;
+  // statistics window (statistic()'s sums since the last reset)
+  double* stat_pot = nullptr;
+  unsigned long long *stat_enum = nullptr, *stat_heat_e = nullptr, *stat_heat_h = nullptr;
+  bool heat_live = false;             // heat was tallied since the last reset
+  double* pot_avg = nullptr;          // heat phase: the fixed potential of the field
   // Poisson
   double *diag = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *dirichlet = nullptr, *corr = nullptr;
   double *pvol = nullptr, *pdop = nullptr, *rhs = nullptr, *pot = nullptr;
@@ -136,6 +143,13 @@ static void alloc_part(DevBuf& m, PartDev& P, long long cap) {
 
 }  // namespace
 
+// Per-cell contents of a restart file (read_par_info_for_restart).
+struct RestartCells {
+  std::vector<double> ec, hc;
+  std::vector<int> en, hn;
+  std::vector<uint8_t> have;
+};
+
 struct emc_ctx {
   std::string err;
   emc_opts opts{};
@@ -156,7 +170,20 @@ struct emc_ctx {
   double last_relres = 0;
   double timing[5] = {0, 0, 0, 0, 0};
   uint64_t next_step = 0;
+  // statistics window: host side of the sums, and what output_stat keeps
+  emc_fly_tallies win{};
+  int64_t win_steps = 0;
+  SavedStat saved;
+  RestartCells restart;               // Flag_restart: the deck's restart file
+  bool heat_phase = false;            // the reference's flag_heat
+  uint64_t heat_next = 0;             // next heat-phase step
 };
+
+// Heat-phase step s draws from RNG step HEAT_STEP0 + s, apart from the transient's.
+static constexpr uint64_t HEAT_STEP0 = 1ull << 40;
+extern "C" {
+static int run_steps(emc_ctx* c, int nsteps, emc_fly_tallies* acc, bool heat_phase);
+}
 
 static int fail(emc_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -390,6 +417,10 @@ static int gpu_setup(emc_ctx* c) {
   g.cell_n = m.alloc<unsigned long long>(NC);
   g.heat_e = m.alloc<unsigned long long>(NN);
   g.heat_h = m.alloc<unsigned long long>(NN);
+  g.stat_pot = m.alloc<double>(NN);
+  g.stat_enum = m.alloc<unsigned long long>(NC);
+  g.stat_heat_e = m.alloc<unsigned long long>(NN);
+  g.stat_heat_h = m.alloc<unsigned long long>(NN);
   g.rcur = m.alloc<unsigned long long>(d.pny);
   g.lcur = m.alloc<unsigned long long>(d.pny);
   g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
@@ -463,7 +494,9 @@ static int gpu_setup(emc_ctx* c) {
   for (size_t q = 0; q < NC; ++q) rho0[q] = 0.5 * d.da[q];
   HIPCHECK(c, hipMemcpy(g.rho, rho0.data(), NC * sizeof(double), hipMemcpyHostToDevice));
   for (auto* pz : {g.node_stat}) HIPCHECK(c, hipMemset(pz, 0, 10 * NN * 8));
-  for (auto* pz : {g.heat_e, g.heat_h}) HIPCHECK(c, hipMemset(pz, 0, NN * 8));
+  for (auto* pz : {g.heat_e, g.heat_h, g.stat_heat_e, g.stat_heat_h}) HIPCHECK(c, hipMemset(pz, 0, NN * 8));
+  HIPCHECK(c, hipMemset(g.stat_pot, 0, NN * 8));
+  HIPCHECK(c, hipMemset(g.stat_enum, 0, NC * 8));
   for (auto* pz : {g.rcur, g.lcur}) HIPCHECK(c, hipMemset(pz, 0, d.pny * 8));
   HIPCHECK(c, hipDeviceSynchronize());
   return EMC_OK;
@@ -516,7 +549,34 @@ static void mr_targets(emc_ctx* c) {
 // population is number s of the reference's creation order: type, then cells
 // with i outer, k, j inner, then the cell's count; its key is init_key(seed, s).
 struct SpecGroup { int i, j, k, type, count; double q; };
-static std::vector<SpecGroup> spec_groups(const Device& d) {
+
+// read_par_info_for_restart (src/mcmodel.cpp:853-889): rows "i j k |Qe| Ne |Qh| Nh"
+// up to a row with i < 0; rows of cells outside the mesh are skipped.
+static std::string read_restart(const Device& d, const std::string& path, RestartCells& r) {
+  std::ifstream f(path);
+  if (!f) return "cannot open restart file " + path;
+  r.ec.assign(d.ncell, 0.0); r.hc.assign(d.ncell, 0.0);
+  r.en.assign(d.ncell, 0); r.hn.assign(d.ncell, 0);
+  r.have.assign(d.ncell, 0);
+  for (;;) {
+    long long i, j, k;
+    if (!(f >> i >> j >> k)) return "restart file " + path + ": no -1 -1 -1 terminator";
+    if (i < 0) break;
+    double c1, c2;
+    long long n1, n2;
+    if (!(f >> c1 >> n1 >> c2 >> n2)) return "restart file " + path + ": malformed row";
+    if (n1 < 0 || n2 < 0 || n1 > INT32_MAX || n2 > INT32_MAX) return "restart file " + path + ": bad particle count";
+    if (i >= d.cnx || j < 0 || j >= d.cny || k < 0 || k >= d.cnz) continue;
+    const size_t cc = d.cell((int)i, (int)j, (int)k);
+    r.ec[cc] = c1; r.en[cc] = (int)n1; r.hc[cc] = c2; r.hn[cc] = (int)n2; r.have[cc] = 1;
+  }
+  return "";
+}
+
+// compute_par_num's counts, or with a restart table (src/particle_init.cpp:128-170)
+// the file's counts; cells attached to a contact keep their neutral charge, the
+// others share the file's charge (electrons negative).
+static std::vector<SpecGroup> spec_groups(const Device& d, const RestartCells* r) {
   std::vector<SpecGroup> gr;
   std::vector<int> ne(d.ncell, 0), nh(d.ncell, 0);
   for (int i = 0; i < d.cnx; ++i)
@@ -526,6 +586,7 @@ static std::vector<SpecGroup> spec_groups(const Device& d) {
         if (d.material[cc] != SILICON) continue;
         ne[cc] = (int)(1e-10 + d.init_e_charge[cc] / d.sum_charge[0] * d.cfg.electron_number) + 1;
         nh[cc] = (int)(1e-10 + d.init_h_charge[cc] / d.sum_charge[1] * d.cfg.hole_number) + 1;
+        if (r && r->have[cc]) { ne[cc] = r->en[cc]; nh[cc] = r->hn[cc]; }
       }
   for (int t = 0; t < 2; ++t)
     for (int i = 0; i < d.cnx; ++i)
@@ -534,18 +595,36 @@ static std::vector<SpecGroup> spec_groups(const Device& d) {
           size_t cc = d.cell(i, j, k);
           if (d.material[cc] != SILICON) continue;
           const int pn = t == 0 ? ne[cc] : nh[cc];
-          const double q = (t == 0 ? d.init_e_charge[cc] : d.init_h_charge[cc]) / pn;
+          if (pn == 0) continue;
+          double q;
+          if (!r || d.attached[cc] > 0) q = (t == 0 ? d.init_e_charge[cc] : d.init_h_charge[cc]) / pn;
+          else q = t == 0 ? -r->ec[cc] / pn : r->hc[cc] / pn;
           gr.push_back(SpecGroup{i, j, k, t, pn, q});
         }
   return gr;
 }
 
-static void sample_population(emc_ctx* c) {
+// The restart table of the deck (Flag_restart), read on first use.
+static std::string restart_table(emc_ctx* c, const RestartCells** out) {
+  *out = nullptr;
+  const Config& cf = c->dev.cfg;
+  if (!cf.flag_restart) return "";
+  if (c->restart.have.empty()) {
+    std::string path = cf.restart_file;
+    if (!path.empty() && path[0] != '/' && !cf.dir.empty()) path = cf.dir + "/" + path;
+    const std::string e = read_restart(c->dev, path, c->restart);
+    if (!e.empty()) return e;
+  }
+  *out = &c->restart;
+  return "";
+}
+
+static void sample_population(emc_ctx* c, const RestartCells* rs) {
   const Device& d = c->dev;
   const Band& b = c->band;
   struct Spec { int i, j, k, type; double q; };
   std::vector<Spec> spec;
-  for (const SpecGroup& g : spec_groups(d))
+  for (const SpecGroup& g : spec_groups(d, rs))
     for (int a = 0; a < g.count; ++a) spec.push_back(Spec{g.i, g.j, g.k, g.type, g.q});
   HostParticles& hp = c->hp;
   const size_t n = spec.size();
@@ -617,12 +696,12 @@ static int sync_fly_const(emc_ctx* c);
 // The same sampling on the GPU, one lane per particle (k_sample_population): the
 // particles are written straight into the device population, identical to the
 // host sampler's slot for slot.
-static int gpu_sample_population(emc_ctx* c) {
+static int gpu_sample_population(emc_ctx* c, const RestartCells* rs) {
   Gpu& g = *c->gpu;
   int st = settle_n(c);
   if (st) return st;
   HIPCHECK(c, hipStreamSynchronize(g.stream));
-  const std::vector<SpecGroup> gr = spec_groups(c->dev);
+  const std::vector<SpecGroup> gr = spec_groups(c->dev, rs);
   std::vector<unsigned long long> start(gr.size() + 1, 0);
   std::vector<uint32_t> gpk(gr.size());
   std::vector<double> gq(gr.size());
@@ -819,13 +898,18 @@ int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
 
 int emc_particles_init(emc_ctx* c, int64_t* n_out) {
   if (!c) return EMC_ERR_ARG;
+  const RestartCells* rs = nullptr;
+  const std::string e = restart_table(c, &rs);
+  if (!e.empty()) return fail(c, EMC_ERR_IO, e);
+  // a restarted run continues at restart_step (src/mcmodel.cpp:507-510)
+  c->next_step = rs ? (uint64_t)std::max(0, c->dev.cfg.restart_step) : 0;
   if (c->gpu) {
-    const int st = gpu_sample_population(c);
+    const int st = gpu_sample_population(c, rs);
     if (st) return st;
     if (n_out) *n_out = c->gpu->n;
     return EMC_OK;
   }
-  sample_population(c);
+  sample_population(c, rs);
   if (n_out) *n_out = (int64_t)c->hp.size();
   return EMC_OK;
 }
@@ -1240,6 +1324,7 @@ extern "C" {
 
 // ---- checkpoint: the device state a later load continues from exactly
 static const char CKPT_MAGIC[8] = {'E', 'M', 'C', 'C', 'K', 'P', 'T', '1'};
+static const uint32_t CKPT_VERSION = 2;
 
 int emc_checkpoint_save(emc_ctx* c, const char* path) {
   if (!c || !path) return EMC_ERR_ARG;
@@ -1262,7 +1347,7 @@ int emc_checkpoint_save(emc_ctx* c, const char* path) {
   if (!f) return fail(c, EMC_ERR_IO, std::string("cannot write ") + path);
   bool ok = true;
   auto put = [&](const void* p, size_t bytes) { ok = ok && std::fwrite(p, 1, bytes, f) == bytes; };
-  const uint32_t version = 1;
+  const uint32_t version = CKPT_VERSION;
   const int32_t dims[4] = {d.cnx, d.cny, d.cnz, 0};
   const uint64_t seed = c->opts.seed, next = c->next_step;
   put(CKPT_MAGIC, 8); put(&version, 4); put(dims, 16); put(&seed, 8); put(&next, 8); put(&nl, 8);
@@ -1286,6 +1371,19 @@ int emc_checkpoint_save(emc_ctx* c, const char* path) {
   put_dev(g.node_stat, 10 * NN * 8);
   put_dev(g.heat_e, NN * 8); put_dev(g.heat_h, NN * 8);
   put_dev(g.rcur, (size_t)d.pny * 8); put_dev(g.lcur, (size_t)d.pny * 8);
+  // the statistics window, what output_stat keeps, and the heat phase
+  put_dev(g.stat_pot, NN * 8);
+  put_dev(g.stat_enum, d.ncell * 8);
+  put_dev(g.stat_heat_e, NN * 8); put_dev(g.stat_heat_h, NN * 8);
+  put(&c->win, sizeof(c->win));
+  const int64_t flags[4] = {c->win_steps, g.heat_live ? 1 : 0, c->heat_phase ? 1 : 0, (int64_t)c->heat_next};
+  put(flags, sizeof(flags));
+  for (const std::vector<double>* v : {&c->saved.pot, &c->saved.e_charge, &c->saved.h_charge}) {
+    std::vector<double> h(*v);
+    h.resize(NN, 0.0);
+    put(h.data(), NN * 8);
+  }
+  if (c->heat_phase) put_dev(g.pot_avg, NN * 8);
   ok = (std::fclose(f) == 0) && ok;
   return ok ? EMC_OK : fail(c, EMC_ERR_IO, std::string("writing ") + path + " failed");
 }
@@ -1309,7 +1407,7 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
   uint64_t seed = 0, next = 0;
   long long nl = 0;
   get(magic, 8); get(&version, 4); get(dims, 16); get(&seed, 8); get(&next, 8); get(&nl, 8);
-  if (!ok || std::memcmp(magic, CKPT_MAGIC, 8) != 0 || version != 1) {
+  if (!ok || std::memcmp(magic, CKPT_MAGIC, 8) != 0 || version != CKPT_VERSION) {
     std::fclose(f);
     return fail(c, EMC_ERR_IO, std::string(path) + " is not a checkpoint of this format");
   }
@@ -1335,11 +1433,34 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
   get_dev(g.node_stat, 10 * NN * 8);
   get_dev(g.heat_e, NN * 8); get_dev(g.heat_h, NN * 8);
   get_dev(g.rcur, (size_t)d.pny * 8); get_dev(g.lcur, (size_t)d.pny * 8);
+  get_dev(g.stat_pot, NN * 8);
+  get_dev(g.stat_enum, d.ncell * 8);
+  get_dev(g.stat_heat_e, NN * 8); get_dev(g.stat_heat_h, NN * 8);
+  emc_fly_tallies win;
+  int64_t flags[4] = {0, 0, 0, 0};
+  get(&win, sizeof(win));
+  get(flags, sizeof(flags));
+  SavedStat saved;
+  for (std::vector<double>* v : {&saved.pot, &saved.e_charge, &saved.h_charge}) {
+    v->resize(NN);
+    get(v->data(), NN * 8);
+  }
+  if (ok && flags[2]) {
+    if (!g.pot_avg) g.pot_avg = g.mem.alloc<double>(NN);
+    if (!g.pot_avg) ok = false;
+    else get_dev(g.pot_avg, NN * 8);
+  }
   std::fclose(f);
   if (!ok) return fail(c, EMC_ERR_IO, std::string("reading ") + path + " failed");
   g.n = nl;
   g.n_pending = false;
   c->next_step = next;
+  c->win = win;
+  c->win_steps = flags[0];
+  g.heat_live = flags[1] != 0;
+  c->heat_phase = flags[2] != 0;
+  c->heat_next = (uint64_t)flags[3];
+  c->saved = std::move(saved);
   c->hp.resize(0);
   return EMC_OK;
 }
@@ -1366,6 +1487,7 @@ int emc_fly_step(emc_ctx* c, uint64_t step, int flag_heat, emc_fly_tallies* out)
   if (st) return st;
   int passes = run_fly(c, step, flag_heat);
   if (passes < 0) return passes;
+  if (flag_heat) c->gpu->heat_live = true;
   read_tallies(c, out, passes);
   HIPCHECK(c, hipGetLastError());
   return EMC_OK;
@@ -1398,8 +1520,127 @@ int emc_tallies_reset(emc_ctx* c) {
   HIPCHECK(c, emc_launch_zero(g.rcur, c->dev.pny, g.stream));
   HIPCHECK(c, emc_launch_zero(g.lcur, c->dev.pny, g.stream));
   HIPCHECK(c, emc_launch_zero(g.node_stat, 10 * NN, g.stream));
+  HIPCHECK(c, emc_launch_zero((unsigned long long*)g.stat_pot, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.stat_enum, c->dev.ncell, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.stat_heat_e, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.stat_heat_h, NN, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
+  g.heat_live = false;
+  std::memset(&c->win, 0, sizeof(c->win));
+  c->win_steps = 0;
   return EMC_OK;
+}
+
+int emc_stat_window_get(emc_ctx* c, emc_stat_window* w) {
+  if (!c || !w) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the window is kept on the device");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode, NC = c->dev.ncell;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  if (w->stat_pot) HIPCHECK(c, hipMemcpy(w->stat_pot, g.stat_pot, NN * 8, hipMemcpyDeviceToHost));
+  if (w->node_stat) HIPCHECK(c, hipMemcpy(w->node_stat, g.node_stat, 10 * NN * 8, hipMemcpyDeviceToHost));
+  if (w->cell_count) HIPCHECK(c, hipMemcpy(w->cell_count, g.stat_enum, NC * 8, hipMemcpyDeviceToHost));
+  if (w->heat_e) HIPCHECK(c, hipMemcpy(w->heat_e, g.stat_heat_e, NN * 8, hipMemcpyDeviceToHost));
+  if (w->heat_h) HIPCHECK(c, hipMemcpy(w->heat_h, g.stat_heat_h, NN * 8, hipMemcpyDeviceToHost));
+  if (w->rcur) HIPCHECK(c, hipMemcpy(w->rcur, g.rcur, (size_t)c->dev.pny * 8, hipMemcpyDeviceToHost));
+  if (w->lcur) HIPCHECK(c, hipMemcpy(w->lcur, g.lcur, (size_t)c->dev.pny * 8, hipMemcpyDeviceToHost));
+  w->steps = c->win_steps;
+  w->tallies = c->win;
+  return EMC_OK;
+}
+
+static std::string out_dir(const char* d) { return d && *d ? std::string(d) : std::string("data"); }
+
+int emc_write_stat(emc_ctx* c, const emc_stat_window* w, int heat_file, const char* data_dir, int step) {
+  if (!c || !w) return EMC_ERR_ARG;
+  if (c->dev.cfg.stat_step <= 0) return fail(c, EMC_ERR_DECK, "stat_step must be positive");
+  StatWindow v;
+  v.stat_pot = w->stat_pot; v.node_stat = w->node_stat; v.cell_count = w->cell_count;
+  v.heat_e = w->heat_e; v.heat_h = w->heat_h; v.rcur = w->rcur; v.lcur = w->lcur;
+  v.tallies = &w->tallies;
+  const std::string e = write_stat(c->dev, v, heat_file != 0, step, out_dir(data_dir), c->saved);
+  return e.empty() ? EMC_OK : fail(c, EMC_ERR_IO, e);
+}
+
+int emc_output_stat(emc_ctx* c, const char* data_dir, int step) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the window is kept on the device");
+  const size_t NN = c->dev.nnode, NC = c->dev.ncell, NY = c->dev.pny;
+  std::vector<double> pot(NN);
+  std::vector<int64_t> ns(10 * NN), cnt(NC), he(NN), hh(NN), rc(NY), lc(NY);
+  emc_stat_window w;
+  std::memset(&w, 0, sizeof(w));
+  w.stat_pot = pot.data(); w.node_stat = ns.data(); w.cell_count = cnt.data();
+  w.heat_e = he.data(); w.heat_h = hh.data(); w.rcur = rc.data(); w.lcur = lc.data();
+  int st = emc_stat_window_get(c, &w);
+  if (st) return st;
+  return emc_write_stat(c, &w, c->heat_phase ? 1 : 0, data_dir, step);
+}
+
+int emc_write_restart(emc_ctx* c, const emc_particles* h, int64_t n, const char* data_dir, int step) {
+  if (!c || !h || n < 0) return EMC_ERR_ARG;
+  for (int64_t s = 0; s < n; ++s)
+    if (h->i[s] >= 0 && (h->i[s] >= c->dev.cnx || h->j[s] < 0 || h->j[s] >= c->dev.cny || h->k[s] < 0 ||
+                         h->k[s] >= c->dev.cnz))
+      return fail(c, EMC_ERR_ARG, "particle outside the mesh");
+  const std::string e = write_restart(c->dev, *h, n, step, out_dir(data_dir));
+  return e.empty() ? EMC_OK : fail(c, EMC_ERR_IO, e);
+}
+
+int emc_output_for_restart(emc_ctx* c, const char* data_dir, int step) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the population is on the device");
+  int64_t n = 0;
+  int st = emc_particles_count(c, &n);
+  if (st) return st;
+  std::vector<double> d8[8];
+  std::vector<int32_t> d4[6];
+  std::vector<uint64_t> key((size_t)n + 1);
+  for (auto& v : d8) v.resize((size_t)n + 1);
+  for (auto& v : d4) v.resize((size_t)n + 1);
+  emc_particles h{d8[0].data(), d8[1].data(), d8[2].data(), d8[3].data(), d8[4].data(), d8[5].data(),
+                  d8[6].data(), d8[7].data(), d4[0].data(), d4[1].data(), d4[2].data(), d4[3].data(),
+                  d4[4].data(), d4[5].data(), key.data()};
+  int64_t got = 0;
+  st = emc_particles_download(c, &h, n, &got);
+  if (st) return st;
+  return emc_write_restart(c, &h, got, data_dir, step);
+}
+
+int emc_output_pvolume(emc_ctx* c, const char* data_dir) {
+  if (!c) return EMC_ERR_ARG;
+  const std::string e = write_pvolume(c->dev, out_dir(data_dir));
+  return e.empty() ? EMC_OK : fail(c, EMC_ERR_IO, e);
+}
+
+// heat_field (src/heat.cpp:100-103): the field of the last window's mean potential
+int emc_heat_begin(emc_ctx* c) {
+  if (!c) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the heat phase runs on the GPU only");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  if (!g.pot_avg) {
+    g.pot_avg = g.mem.alloc<double>(NN);
+    if (!g.pot_avg) return fail(c, EMC_ERR_HIP, "device allocation of the heat-phase potential");
+  }
+  std::vector<double> avg = c->saved.pot;
+  avg.resize(NN, 0.0);
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(g.pot_avg, avg.data(), NN * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, emc_launch_zero(g.heat_e, NN, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.heat_h, NN, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  c->heat_phase = true;
+  c->heat_next = 0;
+  return EMC_OK;
+}
+
+int emc_heat_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
+  if (!c || nsteps < 0) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the heat phase runs on the GPU only");
+  if (!c->heat_phase) return fail(c, EMC_ERR_STATE, "emc_heat_begin has not been called");
+  if (c->gpu->cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  return run_steps(c, nsteps, acc, true);
 }
 
 int emc_deposit(emc_ctx* c, int64_t* node_q, int64_t* node_stat, int64_t* cell_q, int64_t* cell_n) {
@@ -1477,61 +1718,107 @@ int emc_potential_get(emc_ctx* c, double* pot) {
   return EMC_OK;
 }
 
-// density() + update_particle for nsteps (src/mcmodel.cpp:424-471): deposit the
-// current population, solve Poisson (warm start), fields and surface rates,
-// then fly.  Window statistics (node_stat) accumulate across calls.
-int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
-  if (!c || nsteps < 0) return EMC_ERR_ARG;
-  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the step runs on the GPU only");
+// statistic()'s window sums of the step; in the heat phase the node heat is
+// cleared once added (heat_density, src/heat.cpp:48-66).
+static int launch_stat_window(emc_ctx* c, bool heat_phase) {
   Gpu& g = *c->gpu;
-  if (g.cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  StatArgs a{};
+  a.nnode = c->dev.nnode; a.ncell = c->dev.ncell;
+  a.pot = g.pot; a.stat_pot = g.stat_pot;
+  a.cell_n = g.cell_n; a.stat_enum = g.stat_enum;
+  if (g.heat_live || heat_phase) {
+    a.heat_e = g.heat_e; a.heat_h = g.heat_h;
+    a.stat_heat_e = g.stat_heat_e; a.stat_heat_h = g.stat_heat_h;
+  }
+  a.zero_heat = heat_phase ? 1 : 0;
+  HIPCHECK(c, emc_launch_stat_window(a, g.stream));
+  return EMC_OK;
+}
+
+// One step of density() + update_particle (src/mcmodel.cpp:424-471), or with
+// heat_phase one of heat_density (src/heat.cpp:48-98): deposit and window sums,
+// then the Poisson solve (transient only), fields and surface rates, the flight
+// and its clone passes, MultipleRefresh every mr_step steps.
+static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double tsum[5]) {
+  Gpu& g = *c->gpu;
   const size_t NN = c->dev.nnode;
-  double tsum[5] = {0, 0, 0, 0, 0};
-  for (int s = 0; s < nsteps; ++s) {
-    const uint64_t step = c->next_step++;
-    HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
-    int st = run_sort(c);
-    if (st) return st;
-    HIPCHECK(c, emc_launch_deposit(deposit_args(c, 1), g.stream));
+  const Config& cf = c->dev.cfg;
+  const uint64_t count = heat_phase ? c->heat_next++ : c->next_step++;
+  const uint64_t step = heat_phase ? HEAT_STEP0 + count : count;
+  HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
+  int st = run_sort(c);
+  if (st) return st;
+  HIPCHECK(c, emc_launch_deposit(deposit_args(c, 1), g.stream));
+  st = launch_stat_window(c, heat_phase);
+  if (st) return st;
+  if (!heat_phase) {
     RhsArgs ra{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
     HIPCHECK(c, emc_launch_rhs(ra, g.stream));
-    HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
+  }
+  HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
+  if (!heat_phase) {
     st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr, true);
     if (st) return st;
-    HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
-    HIPCHECK(c, emc_launch_field(field_args(c, g.pot, true), g.stream));
-    HIPCHECK(c, hipEventRecord(g.ev[3], g.stream));
-    st = zero_tallies(c);
-    if (st) return st;
-    FlyArgs fa{};
-    st = launch_fly(c, step, c->dev.cfg.flag_heat ? 1 : 0, fa);
-    if (st) return st;
-    HIPCHECK(c, hipEventRecord(g.ev[4], g.stream));
-    HIPCHECK(c, hipEventSynchronize(g.ev[4]));
-    // the one wait of the step: the solve's result, then the flight's counts
+  }
+  HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
+  HIPCHECK(c, emc_launch_field(field_args(c, heat_phase ? g.pot_avg : g.pot, true), g.stream));
+  HIPCHECK(c, hipEventRecord(g.ev[3], g.stream));
+  st = zero_tallies(c);
+  if (st) return st;
+  FlyArgs fa{};
+  // the transient flies without heat tallies: run() clears flag_heat (src/mcmodel.cpp:515)
+  st = launch_fly(c, step, heat_phase ? 1 : 0, fa);
+  if (st) return st;
+  HIPCHECK(c, hipEventRecord(g.ev[4], g.stream));
+  HIPCHECK(c, hipEventSynchronize(g.ev[4]));
+  // the one wait of the step: the solve's result, then the flight's counts
+  if (!heat_phase) {
     st = finish_pcg(c, c->opts.poisson_rtol, nullptr, nullptr);
     if (st) return st;
-    const int passes = finish_fly(c, fa);
-    if (passes < 0) return passes;
-    // MultipleRefresh after the flight of every mr_step-th step (src/mcmodel.cpp:535-547)
-    const Config& cf = c->dev.cfg;
-    if (cf.flag_mr && cf.mr_step > 0 && step % (uint64_t)cf.mr_step == (uint64_t)cf.mr_step - 1) {
-      long long nb = 0, na = 0;
-      st = run_mr(c, step, &nb, &na);
-      if (st) return st;
-      if (acc) acc->n_mr_gen += na - nb;
-    }
-    float ms;
-    for (int q = 0; q < 4; ++q) {
-      HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[q], g.ev[q + 1]));
-      tsum[q] += ms;
-    }
-    HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[5], g.ev[6]));  // the k_fly launch alone
-    tsum[4] += ms;
-    read_tallies(c, acc, passes);
+  }
+  const int passes = finish_fly(c, fa);
+  if (passes < 0) return passes;
+  // MultipleRefresh after the flight of every mr_step-th step (src/mcmodel.cpp:535-547)
+  long long mr_gen = 0;
+  if (cf.flag_mr && cf.mr_step > 0 && count % (uint64_t)cf.mr_step == (uint64_t)cf.mr_step - 1) {
+    long long nb = 0, na = 0;
+    st = run_mr(c, step, &nb, &na);
+    if (st) return st;
+    mr_gen = na - nb;
+  }
+  float ms;
+  for (int q = 0; q < 4; ++q) {
+    HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[q], g.ev[q + 1]));
+    tsum[q] += ms;
+  }
+  HIPCHECK(c, hipEventElapsedTime(&ms, g.ev[5], g.ev[6]));  // the k_fly launch alone
+  tsum[4] += ms;
+  read_tallies(c, acc, passes);
+  read_tallies(c, &c->win, passes);
+  if (acc) acc->n_mr_gen += mr_gen;
+  c->win.n_mr_gen += mr_gen;
+  ++c->win_steps;
+  return EMC_OK;
+}
+
+static int run_steps(emc_ctx* c, int nsteps, emc_fly_tallies* acc, bool heat_phase) {
+  double tsum[5] = {0, 0, 0, 0, 0};
+  for (int s = 0; s < nsteps; ++s) {
+    const int st = one_step(c, heat_phase, acc, tsum);
+    if (st) return st;
   }
   for (int q = 0; q < 5; ++q) c->timing[q] = nsteps ? tsum[q] / nsteps : 0;
   return EMC_OK;
+}
+
+// density() + update_particle for nsteps (src/mcmodel.cpp:424-471): deposit the
+// current population, solve Poisson (warm start), fields and surface rates,
+// then fly.  Window statistics accumulate across calls until emc_tallies_reset.
+int emc_step(emc_ctx* c, int nsteps, emc_fly_tallies* acc) {
+  if (!c || nsteps < 0) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the step runs on the GPU only");
+  if (c->gpu->cap == 0) return fail(c, EMC_ERR_STATE, "no particles on the device");
+  return run_steps(c, nsteps, acc, false);
 }
 
 int emc_step_timing(emc_ctx* c, double* ms5, int* pcg_iters) {
@@ -1545,39 +1832,53 @@ int emc_step_timing(emc_ctx* c, double* ms5, int* pcg_iters) {
 int emc_run_file(const char* input_txt, const emc_opts* o) {
   int st = 0;
   emc_ctx* c = emc_create(input_txt, o, &st);
-  if (st) {
-    std::fprintf(stderr, "emc_create: %s\n", emc_last_error(c));
+  auto stop = [&](const char* what) {
+    std::fprintf(stderr, "%s: %s\n", what, emc_last_error(c));
     emc_destroy(c);
     return st;
-  }
+  };
+  if (st) return stop("emc_create");
   int64_t n = 0;
-  st = emc_particles_init(c, &n);
-  if (st) { std::fprintf(stderr, "init: %s\n", emc_last_error(c)); emc_destroy(c); return st; }
-  const int total = c->dev.cfg.total_step, stat = c->dev.cfg.stat_step;
-  emc_fly_tallies acc;
-  std::memset(&acc, 0, sizeof(acc));
-  FILE* fcur = std::fopen("data/current", "w");
-  for (int s = 0; s < total; ++s) {
-    st = emc_step(c, 1, &acc);
-    if (st) { std::fprintf(stderr, "step %d: %s\n", s, emc_last_error(c)); break; }
-    if (s % stat == stat - 1) {
-      // current_scatter_info (src/mcmodel.cpp:756-815): Current = (Gen - Catch)/(dt*stat_step)*curr0*1e-2
-      if (fcur) {
-        std::fprintf(fcur, "step = %d\n", s);
-        for (size_t q = 0; q < c->dev.contacts.size(); ++q) {
-          const double qg = acc.contact[q][EMC_CT_QGEN] / EMC_FX_CONTACT, qc = acc.contact[q][EMC_CT_QCATCH] / EMC_FX_CONTACT;
-          const double cur = (qg - qc) / (c->dev.cfg.dt * stat) * c->dev.phys.curr0 * 1e-2;
-          std::fprintf(fcur, "%zu %g %lld %g %lld %g %g %g %g\n", q, qg, (long long)acc.contact[q][EMC_CT_NGEN], qc,
-                       (long long)acc.contact[q][EMC_CT_NCATCH], acc.contact[q][EMC_CT_EGEN] / EMC_FX_CONTACT, 0.0, 0.0, cur);
-        }
-        std::fflush(fcur);
-      }
-      std::memset(&acc, 0, sizeof(acc));
+  if ((st = emc_particles_init(c, &n))) return stop("initial population");
+  const Config& cf = c->dev.cfg;
+  if (cf.stat_step <= 0 || (cf.flag_heat && cf.stat_heat_step <= 0)) {
+    st = fail(c, EMC_ERR_DECK, "stat_step and stat_heat_step must be positive");
+    return stop("deck");
+  }
+  const char* dir = "data";
+  const int s0 = cf.flag_restart ? std::max(0, cf.restart_step) : 0;
+  std::printf("Entering run(): total_step = %d, restart step = %d\n", cf.total_step, cf.restart_step);
+  if ((st = emc_output_pvolume(c, dir))) return stop("data/pvolume");
+  // run() (src/mcmodel.cpp:498-566)
+  for (int s = s0; s < cf.total_step; ++s) {
+    std::printf("************ step = %d ********\n%lld particles\n", s, (long long)c->gpu->n);
+    emc_fly_tallies t;
+    std::memset(&t, 0, sizeof(t));
+    if ((st = emc_step(c, 1, &t))) return stop("step");
+    if (cf.flag_mr && cf.mr_step > 0 && s % cf.mr_step == cf.mr_step - 1)
+      std::printf("MultipleRefresh Gen : %lld\n", (long long)t.n_mr_gen);
+    if (s % cf.stat_step == cf.stat_step - 1) {
+      if ((st = emc_output_stat(c, dir, s)) || (st = emc_output_for_restart(c, dir, s)) || (st = emc_tallies_reset(c)))
+        return stop("window output");
     }
   }
-  if (fcur) std::fclose(fcur);
+  // compute_heat (src/heat.cpp:108-167)
+  if (cf.flag_heat) {
+    std::printf("begin computing heating for fixed potential\n");
+    if ((st = emc_heat_begin(c))) return stop("heat phase");
+    for (int s = 0; s < cf.heat_steps; ++s) {
+      std::printf("************ step = %d ********\n%lld particles\n", s, (long long)c->gpu->n);
+      if ((st = emc_heat_step(c, 1, nullptr))) return stop("heat step");
+      if (s % cf.stat_heat_step == cf.stat_heat_step - 1) {
+        if ((st = emc_output_stat(c, dir, s)) || (st = emc_output_for_restart(c, dir, s)) ||
+            (st = emc_tallies_reset(c)))
+          return stop("window output");
+      }
+    }
+  }
+  std::fflush(stdout);
   emc_destroy(c);
-  return st;
+  return EMC_OK;
 }
 
 }  // extern "C"

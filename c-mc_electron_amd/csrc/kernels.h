@@ -78,6 +78,19 @@ struct RhsArgs {
   double* density;  // may be null
 };
 
+// statistic()'s per-step window sums (src/mcmodel.cpp:1189-1196, 1234)
+struct StatArgs {
+  size_t nnode, ncell;
+  const double* pot;                        // the potential the step starts from
+  double* stat_pot;                         // += pot
+  const unsigned long long* cell_n;         // this step's particle count per cell
+  unsigned long long* stat_enum;            // += cell_n
+  unsigned long long *heat_e, *heat_h;      // may be null: no heat tallied
+  unsigned long long *stat_heat_e, *stat_heat_h;  // += heat
+  int zero_heat;                            // heat phase: heat = 0 after it is added
+  int pad;
+};
+
 struct FieldArgs {
   size_t ncell;
   int cnx, cny, cnz, pnx, pny, pnz;
@@ -184,6 +197,7 @@ hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, unsigned long long* ctr,
 hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);
+hipError_t emc_launch_stat_window(const emc::StatArgs& a, hipStream_t s);
 hipError_t emc_launch_field(const emc::FieldArgs& a, hipStream_t s);
 hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 int emc_pcg_blocks(size_t n);

@@ -495,6 +495,28 @@ __global__ __launch_bounds__(256) void k_deposit_gather(DepositArgs a) {
 // ------------------------------------------------------------------ RHS / fields
 // particle_to_density normalisation (src/mcmodel.cpp:407-416) + compute_rhs (:2858-2893),
 // eliminated form: rhs_u - sum over contact neighbours a_uv phi_v.
+// Window sums of one step, taken where statistic() takes them: before the
+// solve, so pot is the previous step's solution.  Node and cell ranges share
+// one grid-stride loop.
+__global__ __launch_bounds__(256) void k_stat_window(StatArgs a) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t m = a.nnode > a.ncell ? a.nnode : a.ncell;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += stride) {
+    if (q < a.nnode) {
+      a.stat_pot[q] += a.pot[q];
+      if (a.heat_e) {
+        a.stat_heat_e[q] += a.heat_e[q];
+        a.stat_heat_h[q] += a.heat_h[q];
+        if (a.zero_heat) {
+          a.heat_e[q] = 0;
+          a.heat_h[q] = 0;
+        }
+      }
+    }
+    if (q < a.ncell) a.stat_enum[q] += a.cell_n[q];
+  }
+}
+
 __global__ void k_rhs(RhsArgs a) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.nnode; n += stride) {
@@ -1119,6 +1141,11 @@ hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
 }
 hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)a.nnode, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t emc_launch_stat_window(const StatArgs& a, hipStream_t s) {
+  const long long m = (long long)(a.nnode > a.ncell ? a.nnode : a.ncell);
+  hipLaunchKernelGGL(k_stat_window, dim3(grid_for(m, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_field(const FieldArgs& a, hipStream_t s) {

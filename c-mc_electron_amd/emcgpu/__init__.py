@@ -338,6 +338,69 @@ class Engine:
     def tallies_reset(self):
         self._check(self.lib.emc_tallies_reset(self.ctx))
 
+    # ---- statistics window and the data/ writers
+    def new_window(self):
+        """Host arrays of one statistics window (zeros) and the struct that points at them."""
+        nn, nc, ny = self.mesh.nnode, self.mesh.ncell, self.mesh.pny
+        w = {"stat_pot": np.zeros(nn), "node_stat": np.zeros(10 * nn, np.int64),
+             "cell_count": np.zeros(nc, np.int64), "heat_e": np.zeros(nn, np.int64),
+             "heat_h": np.zeros(nn, np.int64), "rcur": np.zeros(ny, np.int64), "lcur": np.zeros(ny, np.int64)}
+        st = abi.StatWindow()
+        st.stat_pot = _ptr(w["stat_pot"], C.c_double)
+        for k in ("node_stat", "cell_count", "heat_e", "heat_h", "rcur", "lcur"):
+            setattr(st, k, _ptr(w[k], C.c_int64))
+        w["struct"] = st
+        return w
+
+    def stat_window(self):
+        """The device window: arrays, "steps" and "tallies" (abi.FlyTallies)."""
+        w = self.new_window()
+        self._check(self.lib.emc_stat_window_get(self.ctx, C.byref(w["struct"])))
+        w["steps"] = w["struct"].steps
+        w["tallies"] = w["struct"].tallies
+        return w
+
+    def write_stat(self, window, data_dir, step, heat_file=False):
+        """Window files from a host window (new_window / stat_window); arrays set to
+        None are left out (their files are not written)."""
+        st = abi.StatWindow()
+        st.steps = int(window.get("steps", 0))
+        keep = []
+        if window.get("stat_pot") is not None:
+            a = np.ascontiguousarray(window["stat_pot"], np.float64)
+            keep.append(a)
+            st.stat_pot = _ptr(a, C.c_double)
+        for k in ("node_stat", "cell_count", "heat_e", "heat_h", "rcur", "lcur"):
+            if window.get(k) is not None:
+                a = np.ascontiguousarray(window[k], np.int64)
+                keep.append(a)
+                setattr(st, k, _ptr(a, C.c_int64))
+        if window.get("tallies") is not None:
+            st.tallies = window["tallies"]
+        self._check(self.lib.emc_write_stat(self.ctx, C.byref(st), int(bool(heat_file)),
+                                            os.fsencode(data_dir), int(step)))
+
+    def output_stat(self, data_dir, step):
+        self._check(self.lib.emc_output_stat(self.ctx, os.fsencode(data_dir), int(step)))
+
+    def write_restart(self, parts, n, data_dir, step):
+        self._check(self.lib.emc_write_restart(self.ctx, C.byref(parts.struct()), int(n),
+                                               os.fsencode(data_dir), int(step)))
+
+    def output_for_restart(self, data_dir, step):
+        self._check(self.lib.emc_output_for_restart(self.ctx, os.fsencode(data_dir), int(step)))
+
+    def output_pvolume(self, data_dir):
+        self._check(self.lib.emc_output_pvolume(self.ctx, os.fsencode(data_dir)))
+
+    def heat_begin(self):
+        self._check(self.lib.emc_heat_begin(self.ctx))
+
+    def heat_step(self, nsteps=1):
+        t = abi.FlyTallies()
+        self._check(self.lib.emc_heat_step(self.ctx, nsteps, C.byref(t)))
+        return t
+
 
 def tallies_dict(t):
     return {
@@ -357,3 +420,52 @@ def write_deck(out_dir, **kw):
     spec.loader.exec_module(mod)
     mod.write_decks(out_dir, **kw)
     return os.path.join(out_dir, "input.txt")
+
+
+def read_current(path):
+    """Windows of a data/current file: a list of {"step", "current": {icont: A},
+    "phonon", "impurity"} in file order (the fields current.py and plot_current.py read)."""
+    out = []
+    icont = None
+    with open(path, "r", encoding="utf-8") as f:
+        for line in f:
+            if line.startswith("step"):
+                out.append({"step": int(line.split("=")[1]), "current": {}, "phonon": None, "impurity": None})
+            elif line.startswith("icont"):
+                icont = int(line.split("=")[1])
+            elif line.startswith("Current") and out:
+                out[-1]["current"][icont] = float(line.split("=")[1])
+            elif line.startswith("times of phonon scatter") and out:
+                out[-1]["phonon"] = int(line.split(":")[1])
+            elif line.startswith("times of impurity scatter") and out:
+                out[-1]["impurity"] = int(line.split(":")[1])
+    return out
+
+
+def current_estimate(path, contacts=(0, 1), skip=1, level=0.95):
+    """Drain-current estimate from data/current with a confidence interval.
+
+    The point estimate is the one of finfet/scripts/current.py: the mean over the
+    windows after the first `skip`, and over `contacts`, of |Current|.  Each window
+    (stat_step steps) is one batch mean; the interval is the Student-t interval
+    of the mean of the per-window values.  Returns a dict with "mean",
+    "half_width", "windows" and "lag1" (lag-1 autocorrelation of the window
+    series: near 0 when windows are long enough for the batches to be independent).
+    """
+    win = read_current(path)[skip:]
+    x = np.array([np.mean([abs(w["current"][c]) for c in contacts]) for w in win], np.float64)
+    n = len(x)
+    if n == 0:
+        raise ValueError("no window after the skipped ones")
+    mean = float(x.mean())
+    if n < 2:
+        return {"mean": mean, "half_width": float("inf"), "windows": n, "lag1": float("nan")}
+    sd = float(x.std(ddof=1))
+    try:
+        from scipy.stats import t as student_t
+        q = float(student_t.ppf(0.5 + level / 2, n - 1))
+    except ImportError:  # normal quantile for 95 %
+        q = 1.959963984540054
+    d = x - mean
+    lag1 = float((d[1:] * d[:-1]).sum() / (d * d).sum()) if n > 2 and (d * d).sum() > 0 else float("nan")
+    return {"mean": mean, "half_width": q * sd / np.sqrt(n), "windows": n, "lag1": lag1}

@@ -112,6 +112,14 @@ class FlyTallies(C.Structure):
     ]
 
 
+class StatWindow(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("stat_pot", _d), ("node_stat", _i64), ("cell_count", _i64),
+        ("heat_e", _i64), ("heat_h", _i64), ("rcur", _i64), ("lcur", _i64),
+        ("tallies", FlyTallies),
+    ]
+
+
 class Opts(C.Structure):
     _fields_ = [
         ("table_dir", C.c_char_p), ("device", C.c_int), ("seed", C.c_uint64),
@@ -158,6 +166,14 @@ def declare(lib):
         "emc_currents_get": (C.c_int, [ctx, _i64, _i64]),
         "emc_tallies_reset": (C.c_int, [ctx]),
         "emc_rhs_from_charge": (C.c_int, [ctx, _i64, _d]),
+        "emc_stat_window_get": (C.c_int, [ctx, P(StatWindow)]),
+        "emc_write_stat": (C.c_int, [ctx, P(StatWindow), C.c_int, C.c_char_p, C.c_int]),
+        "emc_output_stat": (C.c_int, [ctx, C.c_char_p, C.c_int]),
+        "emc_write_restart": (C.c_int, [ctx, P(Particles), C.c_int64, C.c_char_p, C.c_int]),
+        "emc_output_for_restart": (C.c_int, [ctx, C.c_char_p, C.c_int]),
+        "emc_output_pvolume": (C.c_int, [ctx, C.c_char_p]),
+        "emc_heat_begin": (C.c_int, [ctx]),
+        "emc_heat_step": (C.c_int, [ctx, C.c_int, P(FlyTallies)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

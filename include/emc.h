@@ -294,6 +294,56 @@ int emc_heat_get(emc_ctx *ctx, int64_t *heat_e, int64_t *heat_h);
 int emc_currents_get(emc_ctx *ctx, int64_t *rcur, int64_t *lcur);
 int emc_tallies_reset(emc_ctx *ctx);
 
+/* ---- statistics window and the data/ writers --------------------------------
+ * Each emc_step / emc_heat_step adds to the window what statistic() adds
+ * (src/mcmodel.cpp:1167-1408): the node statistics of emc_deposit, the
+ * potential the step starts from (before its solve), the particle count of
+ * every cell, the heat tallied so far (heat phase), and the step's contact and
+ * scatter tallies.  emc_tallies_reset starts a new window (set_stat_zero). */
+typedef struct emc_stat_window {
+  int64_t steps;             /* steps added since the last reset                 */
+  double *stat_pot;          /* [nnode] sum of the potential (stat_pot)           */
+  int64_t *node_stat;        /* [10][nnode] as emc_deposit (EMC_FX_STAT)          */
+  int64_t *cell_count;       /* [ncell] sum of the cell counts (stat_enum)        */
+  int64_t *heat_e, *heat_h;  /* [nnode] sum of the node heat (EMC_FX_HEAT)        */
+  int64_t *rcur, *lcur;      /* [pny] crossing charge (EMC_FX_CONTACT)            */
+  emc_fly_tallies tallies;   /* contact columns and scatter counts of the window  */
+} emc_stat_window;
+
+/* Copy the device window into the non-null arrays of *w (and w->steps, w->tallies). */
+int emc_stat_window_get(emc_ctx *ctx, emc_stat_window *w);
+
+/* output_stat + current_scatter_info (src/mcmodel.cpp:971-1161, 756-842) from a
+ * host window: data_dir/current (appended, truncated at step 0),
+ * section_current<step>, heat<step> if heat_file, electron_num<step>,
+ * pot<step>, qc_pot<step>, Electron<step>, Hole<step>, in the reference's
+ * formats; the averages are over the deck's stat_step.  Keeps the window's
+ * mean potential and carrier densities for the next heat file and for
+ * emc_heat_begin.  Host only: works without a GPU.  NULL arrays skip their files. */
+int emc_write_stat(emc_ctx *ctx, const emc_stat_window *w, int heat_file, const char *data_dir, int step);
+
+/* emc_stat_window_get, then emc_write_stat; the heat file is written in the
+ * heat phase (emc_heat_begin), as the reference's flag_heat. */
+int emc_output_stat(emc_ctx *ctx, const char *data_dir, int step);
+
+/* output_for_restart (src/mcmodel.cpp:895-965): data_dir/par_info_restart<step>,
+ * per cell "i j k |Qe| Ne |Qh| Nh" then "-1 -1 -1", from a host population
+ * (emc_write_restart, host only) or from the device one (emc_output_for_restart). */
+int emc_write_restart(emc_ctx *ctx, const emc_particles *host, int64_t n, const char *data_dir, int step);
+int emc_output_for_restart(emc_ctx *ctx, const char *data_dir, int step);
+
+/* print_p_data(p_volume, "pvolume", conc0) (src/mcmodel.cpp:513): data_dir/pvolume. */
+int emc_output_pvolume(emc_ctx *ctx, const char *data_dir);
+
+/* Heating phase (compute_heat, src/heat.cpp:108-167): the field comes from the
+ * mean potential of the last window written by emc_write_stat and stays fixed;
+ * each emc_heat_step deposits (window statistics, then the node heat is added
+ * to the window and cleared), computes the surface rates, flies with phonon
+ * heat tallies and runs MultipleRefresh every mr_step steps; no Poisson solve.
+ * Heat steps count from 0 and draw from RNG counters disjoint from emc_step's. */
+int emc_heat_begin(emc_ctx *ctx);
+int emc_heat_step(emc_ctx *ctx, int nsteps, emc_fly_tallies *acc);
+
 /* Poisson right-hand side of the eliminated system from a node charge sum
  * (particle_to_density normalisation + compute_rhs, src/mcmodel.cpp:407-416, 2858-2893). */
 int emc_rhs_from_charge(emc_ctx *ctx, const int64_t *node_q, double *rhs_elim);
@@ -307,7 +357,11 @@ int emc_potential_get(emc_ctx *ctx, double *pot);
  * and merge, and the k_fly kernel alone; and PCG iterations of the last solve. */
 int emc_step_timing(emc_ctx *ctx, double *ms5, int *pcg_iters);
 
-/* Run the deck like `main input.txt` and write data/current. */
+/* Run the deck like `main input.txt` (MeshQuantities::run and compute_heat,
+ * src/mcmodel.cpp:498-566, src/heat.cpp:108-167): data/pvolume, total_step
+ * steps with the window files every stat_step steps, then heat_steps heat
+ * steps when Flag_compute_heat, with the window files every stat_heat_step.
+ * Writes into ./data, which must exist. */
 int emc_run_file(const char *input_txt, const emc_opts *o);
 
 #ifdef __cplusplus
