@@ -34,9 +34,9 @@ HBM_PEAK_GBS = 8000.0
 FLY_BYTES_PER_PARTICLE = 64 + 12 + 8 + 56 + 12
 
 
-def make_workload(refine, electrons, holes, tmp):
+def make_workload(refine, electrons, holes, tmp, qc=False):
     inp = emcgpu.write_deck(tmp, refine=refine, electrons=electrons, holes=holes, vd=0.5, vg=0.7,
-                            total_step=7000, stat_step=1000)
+                            total_step=7000, stat_step=1000, qc=int(qc), quantum_start_step=0)
     return inp
 
 
@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes (roofline.traffic = null)")
+    ap.add_argument("--qc", action="store_true",
+                    help="quantum correction on (Flag_QuantumCorrection=1 from step 0; not the default workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -186,10 +188,10 @@ def main():
         # before this process touches the GPU: the counter passes run as children
         traffic = measure_traffic(["--steps", "1", "--warmup", "1", "--no-cpu", "--no-traffic",
                                    "--refine", str(a.refine), "--electrons", str(a.electrons),
-                                   "--holes", str(a.holes)])
+                                   "--holes", str(a.holes)] + (["--qc"] if a.qc else []))
         print(f"[bench] traffic {traffic}", file=sys.stderr, flush=True)
     tmp = tempfile.mkdtemp(prefix="emc_bench_")
-    inp = make_workload(a.refine, a.electrons, a.holes, tmp)
+    inp = make_workload(a.refine, a.electrons, a.holes, tmp, qc=a.qc)
     print(f"[bench] deck {inp}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     eng = emcgpu.Engine(inp, device=0, seed=1)
@@ -252,7 +254,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"n-FinFET {eng.mesh.cnx}x{eng.mesh.cny}x{eng.mesh.cnz} cells, "
                                f"{n_start} particles (ElectronNumber={a.electrons}, HoleNumber={a.holes}), "
-                               f"dt 0.1 fs, full step: deposit + PCG Poisson + fields + flight",
+                               f"dt 0.1 fs, full step: deposit + PCG Poisson + fields + flight"
+                               + (" + quantum-correction potential" if a.qc else ""),
+                   "quantum_correction": bool(a.qc),
                    "particles": int(n_start), "nodes": int(eng.mesh.nnode),
                    "phase_ms": {"deposit_rhs": phase[0], "poisson": phase[1], "field": phase[2],
                                 "flight_total": phase[3], "k_fly": phase[4]},

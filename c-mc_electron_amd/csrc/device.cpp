@@ -80,6 +80,7 @@ static std::string parse_input(Config& c, const std::string& path) {
     gb("Flag_restart", c.flag_restart);
     gi("restart_step", c.restart_step);
     gs("restart_filename", c.restart_file);
+    gi("quantum_start_step", c.quantum_start_step);
   } catch (...) {
     return "input.txt: malformed numeric value";
   }
@@ -297,11 +298,11 @@ static std::string parse_ldg(Device& d, const std::string& path, std::vector<Cmd
         f >> t[0] >> t[1] >> t[2] >> t[3];
         break;
       }
-      case 15: {
-        double t[7];
-        for (double& v : t) f >> v;
+      case 15:  // ep_parm qc_xratio qc_xtheta qc_yratio qc_ytheta qc_zratio qc_ztheta Eb [nm, eV]
+        f >> d.qc_ratio[0] >> d.qc_theta[0] >> d.qc_ratio[1] >> d.qc_theta[1] >> d.qc_ratio[2] >> d.qc_theta[2] >>
+            d.qc_eb;
+        d.has_ep = true;
         break;
-      }
       case 16:
         f >> d.nsurf;
         if (d.nsurf < 0 || d.nsurf > MNSurface) return "device file: bad surface count";
@@ -374,6 +375,11 @@ std::string device_parse(Device& d, const std::string& input_path) {
   for (auto& ct : d.contacts) { ct.vapp /= p.pot0; ct.phims /= p.pot0; }
   for (size_t c = 0; c < nc; ++c) { d.donor[c] /= p.conc0; d.acceptor[c] /= p.conc0; }
   for (int q = 0; q < d.nsurf; ++q) d.surf_pos[q] /= p.spr0;
+  for (double& t : d.qc_theta) t /= 1e9 * p.spr0;  // src/mcmodel.cpp:4550-4555
+  d.qc_eb /= p.pot0;
+  if (d.cfg.flag_qc && !d.has_ep) return "device file: Flag_QuantumCorrection needs ep_parm";
+  if (d.cfg.flag_qc && !(d.qc_theta[0] > 0 && d.qc_theta[1] > 0 && d.qc_theta[2] > 0))
+    return "device file: ep_parm lengths must be positive";
   return "";
 }
 

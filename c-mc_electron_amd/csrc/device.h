@@ -36,6 +36,7 @@ struct Config {
   bool flag_mr = false, flag_surface = true, flag_rough = true, flag_sphonon = true;
   bool flag_simp = false, flag_heat = false, flag_qc = false, flag_restart = false;
   int restart_step = 0;
+  int quantum_start_step = 20;
   std::string restart_file = "nosuchname";  // resolved against dir when relative
   std::map<std::string, std::string> raw;
 };
@@ -56,6 +57,10 @@ struct Device {
   std::vector<double> init_e_charge, init_h_charge;
   double sum_charge[2] = {0, 0};
   int default_e = 0, default_h = 0;
+
+  // quantum correction: ep_parm (qc_*ratio, qc_*theta normalised, Eb normalised)
+  bool has_ep = false;
+  double qc_ratio[3] = {0, 0, 0}, qc_theta[3] = {0, 0, 0}, qc_eb = 0;
 
   int nsurf = 0;
   int surf_type[MNSurface] = {0}, surf_dir[MNSurface] = {0};

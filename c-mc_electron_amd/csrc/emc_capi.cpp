@@ -20,6 +20,7 @@
 #include "kernels.h"
 #include "multigrid.h"
 #include "output.h"
+#include "qc.h"
 #include "tetindex.h"
 
 using namespace emc;
@@ -110,6 +111,10 @@ struct Gpu {
   unsigned long long *stat_enum = nullptr, *stat_heat_e = nullptr, *stat_heat_h = nullptr;
   bool heat_live = false;             // heat was tallied since the last reset
   double* pot_avg = nullptr;          // heat phase: the fixed potential of the field
+  // quantum correction (decks with Flag_QuantumCorrection)
+  QcArgs qa{};                        // device tables
+  double *qc_pot = nullptr, *stat_qc_pot = nullptr, *qc_avg = nullptr;
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr;  // hole fields of the Poisson potential
   // Poisson
   double *diag = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *dirichlet = nullptr, *corr = nullptr;
   double *pvol = nullptr, *pdop = nullptr, *rhs = nullptr, *pot = nullptr;
@@ -175,6 +180,7 @@ struct emc_ctx {
   int64_t win_steps = 0;
   SavedStat saved;
   RestartCells restart;               // Flag_restart: the deck's restart file
+  QcTables qc;                        // quantum-correction tables (Flag_QuantumCorrection)
   bool heat_phase = false;            // the reference's flag_heat
   uint64_t heat_next = 0;             // next heat-phase step
 };
@@ -498,6 +504,30 @@ static int gpu_setup(emc_ctx* c) {
   HIPCHECK(c, hipMemset(g.stat_pot, 0, NN * 8));
   HIPCHECK(c, hipMemset(g.stat_enum, 0, NC * 8));
   for (auto* pz : {g.rcur, g.lcur}) HIPCHECK(c, hipMemset(pz, 0, d.pny * 8));
+  if (d.cfg.flag_qc) {
+    const std::string e = build_qc_tables(d, c->qc);
+    if (!e.empty()) return fail(c, EMC_ERR_DECK, e);
+    const QcTables& q = c->qc;
+    QcArgs& a = g.qa;
+    a.nq = (long long)q.qnodes.size();
+    a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz;
+    a.qnodes = m.dup(q.qnodes);
+    a.flag = m.dup(q.flag);
+    for (int ax = 0; ax < 3; ++ax) {
+      a.wlen[ax] = q.wlen[ax];
+      a.lo[ax] = m.dup(q.lo[ax]);
+      a.hi[ax] = m.dup(q.hi[ax]);
+      a.w[ax] = m.dup(q.w[ax]);
+    }
+    a.eb = q.eb;
+    g.qc_pot = m.alloc<double>(NN);
+    g.stat_qc_pot = m.alloc<double>(NN);
+    g.hx = m.alloc<double>(NC); g.hy = m.alloc<double>(NC); g.hz = m.alloc<double>(NC);
+    if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
+    for (double* pz : {g.qc_pot, g.stat_qc_pot}) HIPCHECK(c, hipMemset(pz, 0, NN * 8));
+    for (double* pz : {g.hx, g.hy, g.hz}) HIPCHECK(c, hipMemset(pz, 0, NC * 8));
+    g.F.hx = g.hx; g.F.hy = g.hy; g.F.hz = g.hz;
+  }
   HIPCHECK(c, hipDeviceSynchronize());
   return EMC_OK;
 }
@@ -893,6 +923,10 @@ int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
   v->rhs_corr = d.rhs_corr.data(); v->stencil = d.stencil.data();
   v->mr_desired_e = c->mr_desired_e.empty() ? nullptr : c->mr_desired_e.data();
   v->mr_desired_h = c->mr_desired_h.empty() ? nullptr : c->mr_desired_h.data();
+  v->flag_qc = d.cfg.flag_qc; v->quantum_start_step = d.cfg.quantum_start_step;
+  for (int a = 0; a < 3; ++a) { v->qc_ratio[a] = d.qc_ratio[a]; v->qc_theta[a] = d.qc_theta[a]; }
+  v->qc_eb = d.qc_eb;
+  v->qcregion = d.qcregion.data();
   return EMC_OK;
 }
 
@@ -984,7 +1018,7 @@ int emc_fields_set(emc_ctx* c, const emc_cell_fields* f) {
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
   Gpu& g = *c->gpu;
   const size_t NC = c->dev.ncell;
-  if (f->hx != f->ex || f->hy != f->ey || f->hz != f->ez) {
+  if (f->hx != f->ex || f->hy != f->ey || f->hz != f->ez || g.hx) {
     // distinct hole fields: give them their own storage once
     if (g.F.hx == g.ex) {
       g.F.hx = g.mem.alloc<double>(NC); g.F.hy = g.mem.alloc<double>(NC); g.F.hz = g.mem.alloc<double>(NC);
@@ -1188,8 +1222,9 @@ static DepositArgs deposit_args(emc_ctx* c, int stat_accumulate) {
 
 static FieldArgs field_args(emc_ctx* c, const double* pot, bool rho_from_cells) {
   Gpu& g = *c->gpu;
-  // the computed field serves both species (no quantum correction)
-  g.F.hx = g.ex; g.F.hy = g.ey; g.F.hz = g.ez;
+  // without the quantum correction one field serves both species
+  if (g.hx) { g.F.hx = g.hx; g.F.hy = g.hy; g.F.hz = g.hz; }
+  else { g.F.hx = g.ex; g.F.hy = g.ey; g.F.hz = g.ez; }
   const Device& d = c->dev;
   const Phys& p = d.phys;
   FieldArgs a{};
@@ -1324,7 +1359,7 @@ extern "C" {
 
 // ---- checkpoint: the device state a later load continues from exactly
 static const char CKPT_MAGIC[8] = {'E', 'M', 'C', 'C', 'K', 'P', 'T', '1'};
-static const uint32_t CKPT_VERSION = 2;
+static const uint32_t CKPT_VERSION = 3;
 
 int emc_checkpoint_save(emc_ctx* c, const char* path) {
   if (!c || !path) return EMC_ERR_ARG;
@@ -1384,6 +1419,17 @@ int emc_checkpoint_save(emc_ctx* c, const char* path) {
     put(h.data(), NN * 8);
   }
   if (c->heat_phase) put_dev(g.pot_avg, NN * 8);
+  // quantum correction: the corrected potential, its window sum, its window mean
+  const int64_t has_qc = g.qc_pot ? 1 : 0;
+  put(&has_qc, 8);
+  if (has_qc) {
+    put_dev(g.qc_pot, NN * 8);
+    put_dev(g.stat_qc_pot, NN * 8);
+    std::vector<double> h(c->saved.qc_pot);
+    h.resize(NN, 0.0);
+    put(h.data(), NN * 8);
+    if (c->heat_phase) put_dev(g.qc_avg, NN * 8);
+  }
   ok = (std::fclose(f) == 0) && ok;
   return ok ? EMC_OK : fail(c, EMC_ERR_IO, std::string("writing ") + path + " failed");
 }
@@ -1449,6 +1495,23 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
     if (!g.pot_avg) g.pot_avg = g.mem.alloc<double>(NN);
     if (!g.pot_avg) ok = false;
     else get_dev(g.pot_avg, NN * 8);
+  }
+  int64_t has_qc = 0;
+  get(&has_qc, 8);
+  if (ok && has_qc != (g.qc_pot ? 1 : 0)) {
+    std::fclose(f);
+    return fail(c, EMC_ERR_DECK, "checkpoint and deck disagree on Flag_QuantumCorrection");
+  }
+  if (ok && has_qc) {
+    get_dev(g.qc_pot, NN * 8);
+    get_dev(g.stat_qc_pot, NN * 8);
+    saved.qc_pot.resize(NN);
+    get(saved.qc_pot.data(), NN * 8);
+    if (ok && flags[2]) {
+      if (!g.qc_avg) g.qc_avg = g.mem.alloc<double>(NN);
+      if (!g.qc_avg) ok = false;
+      else get_dev(g.qc_avg, NN * 8);
+    }
   }
   std::fclose(f);
   if (!ok) return fail(c, EMC_ERR_IO, std::string("reading ") + path + " failed");
@@ -1524,6 +1587,7 @@ int emc_tallies_reset(emc_ctx* c) {
   HIPCHECK(c, emc_launch_zero(g.stat_enum, c->dev.ncell, g.stream));
   HIPCHECK(c, emc_launch_zero(g.stat_heat_e, NN, g.stream));
   HIPCHECK(c, emc_launch_zero(g.stat_heat_h, NN, g.stream));
+  if (g.stat_qc_pot) HIPCHECK(c, emc_launch_zero((unsigned long long*)g.stat_qc_pot, NN, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   g.heat_live = false;
   std::memset(&c->win, 0, sizeof(c->win));
@@ -1538,6 +1602,8 @@ int emc_stat_window_get(emc_ctx* c, emc_stat_window* w) {
   const size_t NN = c->dev.nnode, NC = c->dev.ncell;
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   if (w->stat_pot) HIPCHECK(c, hipMemcpy(w->stat_pot, g.stat_pot, NN * 8, hipMemcpyDeviceToHost));
+  if (w->stat_qc_pot)
+    HIPCHECK(c, hipMemcpy(w->stat_qc_pot, g.stat_qc_pot ? g.stat_qc_pot : g.stat_pot, NN * 8, hipMemcpyDeviceToHost));
   if (w->node_stat) HIPCHECK(c, hipMemcpy(w->node_stat, g.node_stat, 10 * NN * 8, hipMemcpyDeviceToHost));
   if (w->cell_count) HIPCHECK(c, hipMemcpy(w->cell_count, g.stat_enum, NC * 8, hipMemcpyDeviceToHost));
   if (w->heat_e) HIPCHECK(c, hipMemcpy(w->heat_e, g.stat_heat_e, NN * 8, hipMemcpyDeviceToHost));
@@ -1555,7 +1621,7 @@ int emc_write_stat(emc_ctx* c, const emc_stat_window* w, int heat_file, const ch
   if (!c || !w) return EMC_ERR_ARG;
   if (c->dev.cfg.stat_step <= 0) return fail(c, EMC_ERR_DECK, "stat_step must be positive");
   StatWindow v;
-  v.stat_pot = w->stat_pot; v.node_stat = w->node_stat; v.cell_count = w->cell_count;
+  v.stat_pot = w->stat_pot; v.stat_qc_pot = w->stat_qc_pot; v.node_stat = w->node_stat; v.cell_count = w->cell_count;
   v.heat_e = w->heat_e; v.heat_h = w->heat_h; v.rcur = w->rcur; v.lcur = w->lcur;
   v.tallies = &w->tallies;
   const std::string e = write_stat(c->dev, v, heat_file != 0, step, out_dir(data_dir), c->saved);
@@ -1566,11 +1632,11 @@ int emc_output_stat(emc_ctx* c, const char* data_dir, int step) {
   if (!c) return EMC_ERR_ARG;
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached: the window is kept on the device");
   const size_t NN = c->dev.nnode, NC = c->dev.ncell, NY = c->dev.pny;
-  std::vector<double> pot(NN);
+  std::vector<double> pot(NN), qpot(NN);
   std::vector<int64_t> ns(10 * NN), cnt(NC), he(NN), hh(NN), rc(NY), lc(NY);
   emc_stat_window w;
   std::memset(&w, 0, sizeof(w));
-  w.stat_pot = pot.data(); w.node_stat = ns.data(); w.cell_count = cnt.data();
+  w.stat_pot = pot.data(); w.stat_qc_pot = qpot.data(); w.node_stat = ns.data(); w.cell_count = cnt.data();
   w.heat_e = he.data(); w.heat_h = hh.data(); w.rcur = rc.data(); w.lcur = lc.data();
   int st = emc_stat_window_get(c, &w);
   if (st) return st;
@@ -1627,6 +1693,15 @@ int emc_heat_begin(emc_ctx* c) {
   avg.resize(NN, 0.0);
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   HIPCHECK(c, hipMemcpy(g.pot_avg, avg.data(), NN * 8, hipMemcpyHostToDevice));
+  if (g.qc_pot) {
+    if (!g.qc_avg) {
+      g.qc_avg = g.mem.alloc<double>(NN);
+      if (!g.qc_avg) return fail(c, EMC_ERR_HIP, "device allocation of the heat-phase potential");
+    }
+    std::vector<double> qa = c->saved.qc_pot.empty() ? c->saved.pot : c->saved.qc_pot;
+    qa.resize(NN, 0.0);
+    HIPCHECK(c, hipMemcpy(g.qc_avg, qa.data(), NN * 8, hipMemcpyHostToDevice));
+  }
   HIPCHECK(c, emc_launch_zero(g.heat_e, NN, g.stream));
   HIPCHECK(c, emc_launch_zero(g.heat_h, NN, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
@@ -1700,13 +1775,58 @@ int emc_poisson_mg_level(emc_ctx* c, int level, int dims[3], double* a27, double
   return EMC_OK;
 }
 
+// Fields of a step: holes from pot_h (pot2field_for_hole), electrons and the
+// surface rates from pot_e (pot2field, GetSurfRoughnessPhononScRate;
+// src/mcmodel.cpp:2239-2263).  Without the quantum correction both are one field.
+static int launch_fields(emc_ctx* c, const double* pot_h, const double* pot_e, bool rho_from_cells) {
+  Gpu& g = *c->gpu;
+  if (g.hx) {
+    FieldArgs h = field_args(c, pot_h, false);
+    h.ex = g.hx; h.ey = g.hy; h.ez = g.hz;
+    h.rates = nullptr;
+    HIPCHECK(c, emc_launch_field(h, g.stream));
+  }
+  HIPCHECK(c, emc_launch_field(field_args(c, pot_e, rho_from_cells), g.stream));
+  return EMC_OK;
+}
+
+// quantumPotential (src/mcmodel.cpp:2378-2405): out = pot, then the effective
+// potential at the nodes of the quantum region when active.
+static int launch_qc(emc_ctx* c, const double* pot, double* out, bool active) {
+  Gpu& g = *c->gpu;
+  HIPCHECK(c, hipMemcpyAsync(out, pot, c->dev.nnode * 8, hipMemcpyDeviceToDevice, g.stream));
+  if (active) {
+    QcArgs a = g.qa;
+    a.pot = pot;
+    a.qc_pot = out;
+    HIPCHECK(c, emc_launch_qc_potential(a, g.stream));
+  }
+  return EMC_OK;
+}
+
+int emc_qc_potential(emc_ctx* c, const double* pot, double* qc_pot) {
+  if (!c || !pot || !qc_pot) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  if (!g.qc_pot) return fail(c, EMC_ERR_STATE, "the deck does not set Flag_QuantumCorrection");
+  const size_t NN = c->dev.nnode;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(g.q, pot, NN * 8, hipMemcpyHostToDevice));  // CG work vectors, free between steps
+  int st = launch_qc(c, g.q, g.z, true);
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  HIPCHECK(c, hipMemcpy(qc_pot, g.z, NN * 8, hipMemcpyDeviceToHost));
+  return EMC_OK;
+}
+
 int emc_compute_field(emc_ctx* c, const double* pot, const double* cell_rho) {
   if (!c || !pot || !cell_rho) return EMC_ERR_ARG;
   if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
   Gpu& g = *c->gpu;
   HIPCHECK(c, hipMemcpy(g.pot, pot, c->dev.nnode * 8, hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemcpy(g.rho, cell_rho, c->dev.ncell * 8, hipMemcpyHostToDevice));
-  HIPCHECK(c, emc_launch_field(field_args(c, g.pot, false), g.stream));
+  int st = launch_fields(c, g.pot, g.pot, false);
+  if (st) return st;
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   return EMC_OK;
 }
@@ -1730,6 +1850,7 @@ static int launch_stat_window(emc_ctx* c, bool heat_phase) {
     a.heat_e = g.heat_e; a.heat_h = g.heat_h;
     a.stat_heat_e = g.stat_heat_e; a.stat_heat_h = g.stat_heat_h;
   }
+  a.qc_pot = g.qc_pot; a.stat_qc_pot = g.stat_qc_pot;
   a.zero_heat = heat_phase ? 1 : 0;
   HIPCHECK(c, emc_launch_stat_window(a, g.stream));
   return EMC_OK;
@@ -1761,7 +1882,20 @@ static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double ts
     if (st) return st;
   }
   HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));
-  HIPCHECK(c, emc_launch_field(field_args(c, heat_phase ? g.pot_avg : g.pot, true), g.stream));
+  if (heat_phase) {
+    // heat_field (src/heat.cpp:100-103): holes at the window-mean Poisson
+    // potential, electrons at the window-mean corrected one
+    st = launch_fields(c, g.pot_avg, g.qc_avg ? g.qc_avg : g.pot_avg, true);
+  } else {
+    const double* pe = g.pot;
+    if (g.qc_pot) {
+      st = launch_qc(c, g.pot, g.qc_pot, count >= (uint64_t)std::max(0, cf.quantum_start_step));
+      if (st) return st;
+      pe = g.qc_pot;
+    }
+    st = launch_fields(c, g.pot, pe, true);
+  }
+  if (st) return st;
   HIPCHECK(c, hipEventRecord(g.ev[3], g.stream));
   st = zero_tallies(c);
   if (st) return st;

@@ -87,8 +87,26 @@ struct StatArgs {
   unsigned long long* stat_enum;            // += cell_n
   unsigned long long *heat_e, *heat_h;      // may be null: no heat tallied
   unsigned long long *stat_heat_e, *stat_heat_h;  // += heat
+  const double* qc_pot;                     // may be null: no quantum correction
+  double* stat_qc_pot;                      // += qc_pot
   int zero_heat;                            // heat phase: heat = 0 after it is added
   int pad;
+};
+
+// Quantum-correction effective potential at the nodes of the quantum region
+// (effective_potential, src/mcmodel.cpp:2337-2376), from tables built on the host.
+struct QcArgs {
+  long long nq;                             // nodes of the quantum region
+  int pnx, pny, pnz;
+  int wlen[3];                              // window length per axis (table row stride)
+  const int* qnodes;                        // [nq] node indices
+  const uint16_t* flag;                     // [nnode] init_p_mat flags
+  const int* lo[3];                         // per axis: window start of each target index
+  const int* hi[3];                         // per axis: window end (inclusive)
+  const double* w[3];                       // per axis: erf differences [target][src - lo][shape]
+  double eb;                                // normalised barrier
+  const double* pot;                        // Poisson potential
+  double* qc_pot;                           // effective potential (quantum nodes written)
 };
 
 struct FieldArgs {
@@ -198,6 +216,7 @@ hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);
 hipError_t emc_launch_stat_window(const emc::StatArgs& a, hipStream_t s);
+hipError_t emc_launch_qc_potential(const emc::QcArgs& a, hipStream_t s);
 hipError_t emc_launch_field(const emc::FieldArgs& a, hipStream_t s);
 hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 int emc_pcg_blocks(size_t n);

@@ -20,6 +20,7 @@
 
 #include "fly_core.h"
 #include "kernels.h"
+#include "qc.h"
 
 namespace emc {
 
@@ -514,6 +515,44 @@ __global__ __launch_bounds__(256) void k_stat_window(StatArgs a) {
       }
     }
     if (q < a.ncell) a.stat_enum[q] += a.cell_n[q];
+    if (a.qc_pot && q < a.nnode) a.stat_qc_pot[q] += a.qc_pot[q];
+  }
+}
+
+// One lane per node of the quantum region; the window's sources are read in the
+// reference's loop order (i, then j, then k) and every term is formed as
+// ((V + Eb) * wz) * wy * wx, so the sum is the oracle's to the last bit.
+// Neighbouring lanes are neighbouring nodes (k fastest) whose windows overlap
+// almost entirely: the loads are served by L2.
+__global__ __launch_bounds__(256) void k_qc_potential(QcArgs a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.nq; q += stride) {
+    const int p = a.qnodes[q];
+    const int pk = p % a.pnz;
+    const int t = p / a.pnz;
+    const int pi = t % a.pnx, pj = t / a.pnx;
+    const int i0 = a.lo[0][pi], i1 = a.hi[0][pi];
+    const int j0 = a.lo[1][pj], j1 = a.hi[1][pj];
+    const int k0 = a.lo[2][pk], k1 = a.hi[2][pk];
+    const double* WX = a.w[0] + (size_t)pi * a.wlen[0] * 4;
+    const double* WY = a.w[1] + (size_t)pj * a.wlen[1] * 4;
+    const double* WZ = a.w[2] + (size_t)pk * a.wlen[2] * 4;
+    double ret = 0.0;
+    for (int i = i0; i <= i1; ++i)
+      for (int j = j0; j <= j1; ++j) {
+        const size_t row = ((size_t)j * a.pnx + i) * a.pnz;
+        for (int k = k0; k <= k1; ++k) {
+          const uint16_t f = a.flag[row + k];
+          if (!(f & (QN_OXIDE | QN_SILICON))) continue;
+          const double eb = ((f & QN_OXIDE) && !(f & QN_SILICON)) ? -a.eb : 0.0;
+          const double v = a.pot[row + k] + eb;
+          const int sx = ((f & QN_UP) ? 1 : 0) | ((f & QN_DOWN) ? 2 : 0);
+          const int sy = ((f & QN_LEFT) ? 1 : 0) | ((f & QN_RIGHT) ? 2 : 0);
+          const int sz = ((f & QN_FRONT) ? 1 : 0) | ((f & QN_BACK) ? 2 : 0);
+          ret += v * WZ[(k - k0) * 4 + sz] * WY[(j - j0) * 4 + sy] * WX[(i - i0) * 4 + sx];
+        }
+      }
+    a.qc_pot[p] = ret * 0.125;
   }
 }
 
@@ -1146,6 +1185,11 @@ hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
 hipError_t emc_launch_stat_window(const StatArgs& a, hipStream_t s) {
   const long long m = (long long)(a.nnode > a.ncell ? a.nnode : a.ncell);
   hipLaunchKernelGGL(k_stat_window, dim3(grid_for(m, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t emc_launch_qc_potential(const QcArgs& a, hipStream_t s) {
+  if (a.nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_qc_potential, dim3(grid_for(a.nq, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_field(const FieldArgs& a, hipStream_t s) {

@@ -131,8 +131,20 @@ std::string write_stat(const Device& d, const StatWindow& w, bool heat_file, int
           c << i << ' ' << j << ' ' << k << ' ' << saved.pot[n] * ph.pot0 << '\n';
         }
     if (!(e = put_file(file_name(dir, "pot", step), c, true)).empty()) return e;
-    // stat_qc_pot sums the potential after the quantum correction, which is
-    // the Poisson potential itself while the correction is off
+  }
+  // stat_qc_pot sums the potential after the quantum correction, which is the
+  // Poisson potential itself while the correction is off
+  const double* qsum = w.stat_qc_pot ? w.stat_qc_pot : w.stat_pot;
+  if (qsum) {
+    saved.qc_pot.assign(NN, 0.0);
+    Text c;
+    for (int i = 0; i < d.pnx; ++i)
+      for (int k = 0; k < d.pnz; ++k)
+        for (int j = 0; j < d.pny; ++j) {
+          const size_t n = d.node(i, j, k);
+          saved.qc_pot[n] = qsum[n] / stat_step;
+          c << i << ' ' << j << ' ' << k << ' ' << saved.qc_pot[n] * ph.pot0 << '\n';
+        }
     if (!(e = put_file(file_name(dir, "qc_pot", step), c, true)).empty()) return e;
   }
   if (S) {

@@ -17,6 +17,7 @@ namespace emc {
 // One statistics window as host arrays, in the fixed point of the device sums.
 struct StatWindow {
   const double* stat_pot = nullptr;       // [nnode] sum over the window's steps of the potential
+  const double* stat_qc_pot = nullptr;    // [nnode] the same for the corrected potential (null: stat_pot)
   const int64_t* node_stat = nullptr;     // [10][nnode] EMC_FX_STAT, layout of emc_deposit
   const int64_t* cell_count = nullptr;    // [ncell] sum of the per-cell particle counts
   const int64_t* heat_e = nullptr;        // [nnode] EMC_FX_HEAT
@@ -30,6 +31,7 @@ struct StatWindow {
 // phase: stat_pot_saved and stat_chargeE_saved / stat_chargeH_saved.
 struct SavedStat {
   std::vector<double> pot;                // stat_pot / stat_step, normalised
+  std::vector<double> qc_pot;             // stat_qc_pot_saved: stat_qc_pot / stat_step
   std::vector<double> e_charge, h_charge; // carrier density of the Electron / Hole files
 };
 

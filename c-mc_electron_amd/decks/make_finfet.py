@@ -26,7 +26,7 @@ def grid_axis(lo, hi, refine):
 
 def write_decks(out, refine=1, electrons=20000, holes=2000, vs=0.0, vd=0.5, vg=0.7,
                 total_step=2000, dt=1e-16, stat_step=1000, heat=0, heat_steps=0,
-                surface=1):
+                surface=1, qc=0, quantum_start_step=20):
     os.makedirs(out, exist_ok=True)
     axes = [grid_axis(-1, 19, refine), grid_axis(-20, 20, refine), grid_axis(-3, 9, refine)]
     with open(os.path.join(out, "lgrid.txt"), "w") as f:
@@ -110,7 +110,8 @@ dt={dt:g}
 stat_step={stat_step}
 stat_heat_step={stat_step}
 heat_steps={heat_steps}
-Flag_QuantumCorrection=0
+Flag_QuantumCorrection={qc}
+quantum_start_step={quantum_start_step}
 Flag_MultipleRefresh=0
 Flag_compute_heat={heat}
 Flag_SurfaceScatter={surface}
@@ -131,9 +132,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--heat", type=int, default=0)
     ap.add_argument("--heat-steps", type=int, default=0)
+    ap.add_argument("--qc", type=int, default=0)
     a = ap.parse_args()
     write_decks(a.out, a.refine, a.electrons, a.holes, vd=a.vd, vg=a.vg, total_step=a.steps,
-                heat=a.heat, heat_steps=a.heat_steps)
+                heat=a.heat, heat_steps=a.heat_steps, qc=a.qc)
 
 
 if __name__ == "__main__":

@@ -171,7 +171,7 @@ class Engine:
         for f, n in (("material", nc), ("attached", nc), ("nearest_surf", nc), ("eeff_dir", nc),
                      ("surface_dir", max(1, m.nsurf)), ("mr_desired_e", nc), ("mr_desired_h", nc)):
             v[f] = _view(getattr(m, f), n, np.int32)
-        for f, n in (("motion", 6 * nc), ("insurf", nc), ("node_kind", nn)):
+        for f, n in (("motion", 6 * nc), ("insurf", nc), ("node_kind", nn), ("qcregion", nc)):
             v[f] = _view(getattr(m, f), n, np.uint8)
         return v
 
@@ -342,11 +342,12 @@ class Engine:
     def new_window(self):
         """Host arrays of one statistics window (zeros) and the struct that points at them."""
         nn, nc, ny = self.mesh.nnode, self.mesh.ncell, self.mesh.pny
-        w = {"stat_pot": np.zeros(nn), "node_stat": np.zeros(10 * nn, np.int64),
+        w = {"stat_pot": np.zeros(nn), "stat_qc_pot": np.zeros(nn), "node_stat": np.zeros(10 * nn, np.int64),
              "cell_count": np.zeros(nc, np.int64), "heat_e": np.zeros(nn, np.int64),
              "heat_h": np.zeros(nn, np.int64), "rcur": np.zeros(ny, np.int64), "lcur": np.zeros(ny, np.int64)}
         st = abi.StatWindow()
         st.stat_pot = _ptr(w["stat_pot"], C.c_double)
+        st.stat_qc_pot = _ptr(w["stat_qc_pot"], C.c_double)
         for k in ("node_stat", "cell_count", "heat_e", "heat_h", "rcur", "lcur"):
             setattr(st, k, _ptr(w[k], C.c_int64))
         w["struct"] = st
@@ -366,10 +367,11 @@ class Engine:
         st = abi.StatWindow()
         st.steps = int(window.get("steps", 0))
         keep = []
-        if window.get("stat_pot") is not None:
-            a = np.ascontiguousarray(window["stat_pot"], np.float64)
-            keep.append(a)
-            st.stat_pot = _ptr(a, C.c_double)
+        for k in ("stat_pot", "stat_qc_pot"):
+            if window.get(k) is not None:
+                a = np.ascontiguousarray(window[k], np.float64)
+                keep.append(a)
+                setattr(st, k, _ptr(a, C.c_double))
         for k in ("node_stat", "cell_count", "heat_e", "heat_h", "rcur", "lcur"):
             if window.get(k) is not None:
                 a = np.ascontiguousarray(window[k], np.int64)
@@ -392,6 +394,13 @@ class Engine:
 
     def output_pvolume(self, data_dir):
         self._check(self.lib.emc_output_pvolume(self.ctx, os.fsencode(data_dir)))
+
+    def qc_potential(self, pot):
+        """Effective potential of a node potential (quantum correction) on the GPU."""
+        pot = np.ascontiguousarray(pot, np.float64)
+        out = np.zeros(self.mesh.nnode)
+        self._check(self.lib.emc_qc_potential(self.ctx, _ptr(pot, C.c_double), _ptr(out, C.c_double)))
+        return out
 
     def heat_begin(self):
         self._check(self.lib.emc_heat_begin(self.ctx))

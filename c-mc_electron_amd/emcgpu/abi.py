@@ -88,6 +88,9 @@ class MeshView(C.Structure):
         ("diag", _d), ("cx", _d), ("cy", _d), ("cz", _d),
         ("rhs_corr", _d), ("stencil", _d),
         ("mr_desired_e", C.POINTER(C.c_int32)), ("mr_desired_h", C.POINTER(C.c_int32)),
+        ("flag_qc", C.c_int), ("quantum_start_step", C.c_int),
+        ("qc_ratio", C.c_double * 3), ("qc_theta", C.c_double * 3), ("qc_eb", C.c_double),
+        ("qcregion", _u8),
     ]
 
 
@@ -114,7 +117,7 @@ class FlyTallies(C.Structure):
 
 class StatWindow(C.Structure):
     _fields_ = [
-        ("steps", C.c_int64), ("stat_pot", _d), ("node_stat", _i64), ("cell_count", _i64),
+        ("steps", C.c_int64), ("stat_pot", _d), ("stat_qc_pot", _d), ("node_stat", _i64), ("cell_count", _i64),
         ("heat_e", _i64), ("heat_h", _i64), ("rcur", _i64), ("lcur", _i64),
         ("tallies", FlyTallies),
     ]
@@ -173,6 +176,7 @@ def declare(lib):
         "emc_output_for_restart": (C.c_int, [ctx, C.c_char_p, C.c_int]),
         "emc_output_pvolume": (C.c_int, [ctx, C.c_char_p]),
         "emc_heat_begin": (C.c_int, [ctx]),
+        "emc_qc_potential": (C.c_int, [ctx, _d, _d]),
         "emc_heat_step": (C.c_int, [ctx, C.c_int, P(FlyTallies)]),
     }
     for name, (res, args) in sig.items():

@@ -151,6 +151,12 @@ typedef struct emc_mesh_view {
                                        the deck's counts, raised in silicon to the
                                        initial count and the default_par_number
                                        floor (src/particle_init.cpp:44-70)       */
+  /* quantum correction (src/mcmodel.cpp:2337-2662): Flag_QuantumCorrection,
+     quantum_start_step, the ep_parm values (lengths and Eb normalised) and the
+     quantumRegion cells (surface_scatter_range cells included, as in the reference) */
+  int flag_qc, quantum_start_step;
+  double qc_ratio[3], qc_theta[3], qc_eb;
+  const uint8_t *qcregion;          /* [ncell]                                */
 } emc_mesh_view;
 
 /* Structure-of-arrays particle buffers (caller-owned on the host). */
@@ -282,6 +288,13 @@ int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol,
  * the last level.  Works without a GPU. */
 int emc_poisson_mg_level(emc_ctx *ctx, int level, int dims[3], double *a27, double *omega);
 
+/* quantumPotential (src/mcmodel.cpp:2378-2405): the effective potential of a node
+ * potential on the GPU; qc_pot[n] = pot[n] outside the quantum region.  Needs a
+ * deck with Flag_QuantumCorrection (EMC_ERR_STATE otherwise).  emc_step applies it
+ * itself from step quantum_start_step on: electrons fly in the field of the
+ * corrected potential, holes in that of the Poisson potential. */
+int emc_qc_potential(emc_ctx *ctx, const double *pot, double *qc_pot);
+
 /* From a node potential: cell fields (pot2field) and surface rates, written
  * into the context's cell fields; cell_rho must hold the clamped Rho. */
 int emc_compute_field(emc_ctx *ctx, const double *pot, const double *cell_rho);
@@ -303,6 +316,9 @@ int emc_tallies_reset(emc_ctx *ctx);
 typedef struct emc_stat_window {
   int64_t steps;             /* steps added since the last reset                 */
   double *stat_pot;          /* [nnode] sum of the potential (stat_pot)           */
+  double *stat_qc_pot;       /* [nnode] sum of the electrons' potential after the
+                                quantum correction (stat_qc_pot); equal to
+                                stat_pot when the correction is off            */
   int64_t *node_stat;        /* [10][nnode] as emc_deposit (EMC_FX_STAT)          */
   int64_t *cell_count;       /* [ncell] sum of the cell counts (stat_enum)        */
   int64_t *heat_e, *heat_h;  /* [nnode] sum of the node heat (EMC_FX_HEAT)        */

@@ -97,6 +97,10 @@ uint64_t orc_mr_child_key(uint64_t parent, uint64_t step, int64_t i);
 /* y = A x with the operator as assembled by init_matrix (contact rows identity) */
 void orc_apply_full(const emc_mesh_view *m, const double *x, double *y);
 
+/* ---- quantum correction: out = pot, then the effective potential at the nodes
+ * of the quantum region (src/mcmodel.cpp:2337-2662); returns 0 */
+int orc_qc_potential(const emc_mesh_view *m, const double *pot, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,7 @@ def load():
         "orc_surface_rates": (None, [P(abi.MeshView), d, d, d, d, d, d, d, d]),
         "orc_pcg": (C.c_int, [P(abi.MeshView), d, d, C.c_double, C.c_int, d]),
         "orc_apply_full": (None, [P(abi.MeshView), d, d]),
+        "orc_qc_potential": (C.c_int, [P(abi.MeshView), d, d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,77 @@
+"""GPU: the quantum-correction effective potential (src/mcmodel.cpp:2337-2405) and
+the step with it on (compute_field, src/mcmodel.cpp:2239-2263).
+
+emc_qc_potential equals the oracle's restatement bit for bit.  From step
+quantum_start_step on, emc_step gives electrons the field of the corrected
+potential and holes the field of the Poisson potential, and the flight under
+those fields is the oracle's; the window keeps both potential sums.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from helpers import cfg1_deck, emcgpu
+from test_gpu_parity import assert_same_population
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_qc(eng, pot):
+    out = np.zeros(eng.mesh.nnode)
+    lib = oracle_lib.load()
+    assert lib.orc_qc_potential(C.byref(eng.mesh), oracle_lib.dptr(np.ascontiguousarray(pot)),
+                                oracle_lib.dptr(out)) == 0
+    return out
+
+
+def oracle_field(eng, pot):
+    lib = oracle_lib.load()
+    f = [np.zeros(eng.mesh.ncell) for _ in range(3)]
+    lib.orc_field(C.byref(eng.mesh), oracle_lib.dptr(np.ascontiguousarray(pot)), *[oracle_lib.dptr(x) for x in f])
+    return f
+
+
+def test_qc_potential_and_step():
+    e = emcgpu.Engine(cfg1_deck(qc=1, quantum_start_step=2), device=0)
+    try:
+        m = e.mesh
+        rng = np.random.default_rng(11)
+        for what, pot in (("random", rng.normal(size=m.nnode)), ("smooth", np.sin(np.arange(m.nnode) * 1e-3))):
+            print(f"[gpu] qc potential, {what} input", flush=True)
+            np.testing.assert_array_equal(e.qc_potential(pot), oracle_qc(e, pot), err_msg=what)
+        e.particles_init()
+        e.tallies_reset()
+        print("[gpu] qc steps 0-1 (correction not yet active)", flush=True)
+        e.step(1)
+        p1 = e.potential()
+        e.step(1)
+        p2 = e.potential()
+        f = e.fields_get()
+        np.testing.assert_array_equal(f.ex, f.hx)
+        p0 = e.download()
+        print("[gpu] qc step 2 (active)", flush=True)
+        e.step(1)
+        pot = e.potential()
+        f = e.fields_get()
+        qpot = oracle_qc(e, pot)
+        for got, want, name in zip((f.ex, f.ey, f.ez), oracle_field(e, qpot), ("ex", "ey", "ez")):
+            np.testing.assert_array_equal(got, want, err_msg="electron " + name)
+        for got, want, name in zip((f.hx, f.hy, f.hz), oracle_field(e, pot), ("hx", "hy", "hz")):
+            np.testing.assert_array_equal(got, want, err_msg="hole " + name)
+        assert np.any(f.ex != f.hx)
+        po, _, _ = oracle_lib.fly_step(e, p0, p0.n, f, 2)
+        assert_same_population(e.download(), po, "qc step")
+        # the window: statistic() adds the potentials each step starts from (the
+        # initial zeros, then each solve); the corrected sum takes the corrected
+        # potential from the step after the first corrected solve on
+        print("[gpu] qc step 3 (window sums)", flush=True)
+        e.step(1)
+        w = e.stat_window()
+        assert w["steps"] == 4
+        z = np.zeros(m.nnode)
+        np.testing.assert_array_equal(w["stat_pot"], ((z + p1) + p2) + pot)
+        np.testing.assert_array_equal(w["stat_qc_pot"], ((z + p1) + p2) + qpot)
+    finally:
+        e.close()

@@ -79,6 +79,7 @@ def known_window(eng, seed=0):
     rng = np.random.default_rng(seed)
     w = eng.new_window()
     w["stat_pot"][:] = rng.uniform(-1, 1, nn) * STAT_STEP
+    w["stat_qc_pot"][:] = rng.uniform(-1, 1, nn) * STAT_STEP
     ns = w["node_stat"].reshape(10, nn)
     q = rng.integers(1, 1 << 40, (10, nn))
     q[[0, 1, 2, 5, 6, 7]] -= 1 << 39                      # signed velocity sums
@@ -163,7 +164,13 @@ def test_window_files_from_known_window(eng, tmp_path):
     assert read_lines(tmp_path / "electron_num7") == want
     want = [f"{i} {j} {k} {g(w['stat_pot'][node_index(m, i, j, k)] / STAT_STEP * m.pot0)}" for i, j, k in node_order(m)]
     assert read_lines(tmp_path / "pot7") == want
+    want = [f"{i} {j} {k} {g(w['stat_qc_pot'][node_index(m, i, j, k)] / STAT_STEP * m.pot0)}"
+            for i, j, k in node_order(m)]
     assert read_lines(tmp_path / "qc_pot7") == want
+    # without a corrected-potential sum the qc_pot file holds the Poisson one
+    (tmp_path / "noqc").mkdir()
+    eng.write_stat({"stat_pot": w["stat_pot"]}, str(tmp_path / "noqc"), 7)
+    assert read_lines(tmp_path / "noqc" / "qc_pot7") == read_lines(tmp_path / "pot7")
     e_rows, e_dens = expected_carrier(eng, w, 0)
     h_rows, h_dens = expected_carrier(eng, w, 1)
     assert read_lines(tmp_path / "Electron7") == e_rows
