@@ -789,4 +789,23 @@ std::string band_build(Band& b, const Phys& p, const std::string& dir) {
   return "";
 }
 
+std::string load_phonon_tables(Band& b, const std::string& dir, const std::string files[PTAB_N]) {
+  for (int t = 0; t < PTAB_N; ++t) {
+    const std::string path = dir + "/" + files[t];
+    std::ifstream f(path);
+    if (!f) return "cannot open dispersion table " + path;
+    const int R = PTAB_ROWS[t], C = PTAB_COLS[t];
+    b.ptab[t].assign((size_t)R * C, 0.0);
+    b.ptab_t[t].assign((size_t)R * C, 0.0);
+    for (int c = 0; c < C; ++c)
+      for (int r = 0; r < R; ++r) {
+        double v;
+        if (!(f >> v)) return "dispersion table " + path + " is short";
+        b.ptab[t][(size_t)r * C + c] = v;
+        b.ptab_t[t][(size_t)c * R + r] = v;
+      }
+  }
+  return "";
+}
+
 }  // namespace emc

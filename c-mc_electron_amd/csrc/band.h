@@ -33,6 +33,10 @@ struct Phys {
 void phys_init(Phys& p, double T);
 
 struct Band {
+  // dispersive phonon tables (load_phonon_tables), empty unless loaded:
+  // ptab[t][row * PTAB_COLS[t] + col] as the reference indexes them, ptab_t the
+  // transpose (one energy bin's column contiguous) for the flight
+  std::vector<double> ptab[PTAB_N], ptab_t[PTAB_N];
   // ---- sizes
   int nband[NumParType] = {0, 0, 0};
   int bandof[NumParType] = {0, 0, 0};
@@ -107,6 +111,11 @@ struct Band {
 // Build every table from <input_dir>/bs.si.asc and <input_dir>/zd.si.asc.
 // Returns an empty string on success, otherwise an error message.
 std::string band_build(Band& b, const Phys& p, const std::string& input_dir);
+
+// init_deep (src/ParticleUtils.cpp:750-808): the six dispersive phonon tables from
+// `files` (kloem kloab ktoem ktoab klaem klaab) in input_dir, each stored column
+// by column in the file.
+std::string load_phonon_tables(Band& b, const std::string& input_dir, const std::string files[PTAB_N]);
 
 // sign() of the reference (src/utils.cpp:64): +-1; the build defines sign(0)=+1.
 inline int sgn(double x) { return x < 0 ? -1 : 1; }

@@ -236,6 +236,7 @@ static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
   for (int q = 0; q < 3; ++q) B.difpr[q] = b.difpr[q];
   B.temptag = b.temptag; B.templag = b.templag; B.templog = b.templog; B.temptaf = b.temptaf;
   B.templaf = b.templaf; B.temptof = b.temptof; B.temphop = b.temphop;
+  for (int t = 0; t < PTAB_N; ++t) B.ptab[t] = b.ptab_t[t].empty() ? nullptr : b.ptab_t[t].data();
 }
 
 static void fill_mesh_dev_scalars(emc_ctx* c, MeshDev& M) {
@@ -396,6 +397,7 @@ static int gpu_setup(emc_ctx* c) {
   B.ibt = m.dup(b.ibt);
   B.energy = m.dup(b.energy); B.dostetmax = m.dup(b.dostetmax); B.sumscatt = m.dup(b.sumscatt);
   B.dos = m.dup(b.dos); B.sumdos = m.dup(b.sumdos);
+  for (int t = 0; t < PTAB_N; ++t) B.ptab[t] = b.ptab_t[t].empty() ? nullptr : m.dup(b.ptab_t[t]);
   B.scatte = m.dup(b.scatte); B.dose = m.dup(b.dose); B.scattiie = m.dup(b.scattiie);
   B.scatth = m.dup(b.scatth); B.dosh = m.dup(b.dosh); B.scattiih = m.dup(b.scattiih);
 
@@ -795,6 +797,18 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
   if (tdir.empty()) return done(fail(c, EMC_ERR_ARG, "table_dir is required"));
   err = band_build(c->band, c->dev.phys, tdir);
   if (!err.empty()) return done(fail(c, EMC_ERR_IO, err));
+  if (c->dev.cfg.flag_qc) {
+    // EPSCAT takes optical and f-phonon final energies from the dispersive
+    // tables in quantum-correction runs (src/ElectronPhononScatter.cpp:148-163)
+    std::string files[PTAB_N] = {"kloem.txt", "kloab.txt", "ktoem.txt", "ktoab.txt", "klaem.txt", "klaab.txt"};
+    const char* keys[PTAB_N] = {"kloem_file", "kloab_file", "ktoem_file", "ktoab_file", "klaem_file", "klaab_file"};
+    for (int t = 0; t < PTAB_N; ++t) {
+      auto it = c->dev.cfg.raw.find(keys[t]);
+      if (it != c->dev.cfg.raw.end()) files[t] = it->second;
+    }
+    err = load_phonon_tables(c->band, tdir, files);
+    if (!err.empty()) return done(fail(c, EMC_ERR_IO, err));
+  }
   err = device_finish(c->dev, c->band);
   if (!err.empty()) return done(fail(c, EMC_ERR_DECK, err));
   c->surface_dir.assign(std::max(1, c->dev.nsurf), 0);
@@ -863,6 +877,7 @@ int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
   v->energy = b.energy.data(); v->dos = b.dos.data(); v->sumdos = b.sumdos.data(); v->dostetmax = b.dostetmax.data();
   v->sumscatt = b.sumscatt.data(); v->scatte = b.scatte.data(); v->dose = b.dose.data(); v->scattiie = b.scattiie.data();
   v->scatth = b.scatth.data(); v->dosh = b.dosh.data(); v->scattiih = b.scattiih.data();
+  for (int t = 0; t < PTAB_N; ++t) v->ptab[t] = b.ptab[t].empty() ? nullptr : b.ptab[t].data();
   return EMC_OK;
 }
 

@@ -52,6 +52,12 @@ constexpr int MNSurface = 30;
 // ---- tolerances (include/defs.h:126)
 constexpr double MY_ZERO = 1e-13;
 
+// Dispersive phonon final-energy tables (init_deep, src/ParticleUtils.cpp:750-808):
+// kloem kloab ktoem ktoab klaem klaab, rows (final-energy classes) x columns (energy bins).
+constexpr int PTAB_N = 6;
+constexpr int PTAB_ROWS[PTAB_N] = {201, 201, 51, 51, 51, 51};
+constexpr int PTAB_COLS[PTAB_N] = {120, 110, 160, 150, 160, 150};
+
 // ---- packed particle word: type(1) | i(10) | j(11) | k(10); all ones = empty slot
 constexpr unsigned PK_DEAD = 0xFFFFFFFFu;
 constexpr int PK_IBITS = 10, PK_JBITS = 11, PK_KBITS = 10;

@@ -519,6 +519,63 @@ struct Flight {
   EMC_HD EMC_INL void restore(const Saved& s) {
     energy = s.e; kx = s.kx; ky = s.ky; kz = s.kz; iband = s.iband; itet = s.itet; isym = s.isym;
   }
+  // Final energy of electron phonon process p from the dispersive tables (runs
+  // with the quantum correction, src/ElectronPhononScatter.cpp:148-163): p = 5/6
+  // g-LO absorption/emission (FEKLOAB/FEKLOEM, :702-768, :632-700), 9/10 f-LA
+  // (FEKLAAB/FEKLAEM, :770-889), 11/12 f-TO (FEKTOAB/FEKTOEM, :891-1022).  The
+  // energy in units of the phonon's picks an energy bin (finer bins at low
+  // energy); one draw walks that bin's distribution over final-energy classes,
+  // and class i shifts the energy by the phonon's, widened or narrowed by
+  // (i/2) steps of the bin's width (up for odd i, down for even i).
+  EMC_HD EMC_INL double dispersive(int p, double eet) {
+    const bool lo = p <= 6, em = (p & 1) == 0;
+    const int tab = p == 6 ? 0 : p == 5 ? 1 : p == 12 ? 2 : p == 11 ? 3 : p == 10 ? 4 : 5;
+    const double hw = lo ? B.templog : p <= 10 ? B.templaf : B.temptof;
+    const double ie = eet / hw;
+    int it, ei;
+    double ed;
+    if (em) {
+      if (ie < 1.0) return p == 10 ? -1.0e-10 : 1.0e-10;
+      if (ie < 3.05) { it = (int)((ie - 1.0) * 10.0 + 0.5); it = it < 1 ? 1 : it; ei = it; ed = lo ? 1.0 : 0.25; }
+      else if (ie < 5.625) { it = (int)((ie - 3.0) * 4.0 + 0.5); it = it < 1 ? 1 : it; ei = 20 + it; ed = lo ? 2.0 : 0.5; }
+      else if (ie < 13.25) { it = (int)((ie - 5.5) * 2.0 + 0.5); it = it < 1 ? 1 : it; ei = 30 + it; ed = lo ? 4.0 : 0.5; }
+      else if (lo && ie < 28.5) { it = (int)(ie - 13.0 + 0.5); it = it < 1 ? 1 : it; ei = 45 + it; ed = 8.0; }
+      else if (lo) { it = (int)((ie - 28.0) + 0.5); it = it < 1 ? 1 : it > 60 ? 60 : it; ei = 60 + it; ed = 12.0; }
+      else { it = (int)((ie - 13.0) + 0.5); it = it < 1 ? 1 : it > 115 ? 115 : it; ei = 45 + it; ed = 1.0; }
+    } else {
+      if (ie < 1.05) { it = (int)(ie * 10.0 + 0.5); it = it < 1 ? 1 : it; ei = it; ed = lo ? 1.0 : 0.25; }
+      else if (ie < 3.625) { it = (int)((ie - 1.0) * 4.0 + 0.5); it = it < 1 ? 1 : it; ei = 10 + it; ed = lo ? 2.0 : 0.25; }
+      else if (ie < 11.25) { it = (int)((ie - 3.5) * 2.0 + 0.5); it = it < 1 ? 1 : it; ei = 20 + it; ed = lo ? 4.0 : 0.5; }
+      else if (lo && ie < 26.5) { it = (int)((ie - 11.0) + 0.5); it = it < 1 ? 1 : it; ei = 35 + it; ed = 8.0; }
+      else if (lo) { it = (int)((ie - 26.0) + 0.5); it = it < 1 ? 1 : it > 60 ? 60 : it; ei = 50 + it; ed = 12.0; }
+      else { it = (int)((ie - 11.0) + 0.5); it = it < 1 ? 1 : it > 60 ? 115 : it; ei = 35 + it; ed = 1.0; }  // > 60 jumps to 115, as the reference
+    }
+    const int R = tab < 2 ? 201 : 51;  // PTAB_ROWS
+    const double* col = B.ptab[tab] + (size_t)(ei - 1) * R;
+    const double ran = Random();
+    int i = 0;
+    double ksum = 0.0;
+    if (p == 6) {  // FEKLOEM wraps before it adds
+      while (ksum < ran) {
+        ++i;
+        if (i == R) { ksum = 1.1; i = 1; }
+        ksum += col[i - 1];
+      }
+    } else {
+      while (ksum < ran) {
+        ++i;
+        ksum += col[i - 1];
+        if (i == R) { ksum = 1.1; i = 1; }
+      }
+    }
+    const int j = i % 2, k = i / 2;
+    const double w = (lo ? 0.01 : 0.04) * ed * k;
+    double ef = em ? (j == 1 ? eet - hw * (1.0 - w) : eet - hw * (1.0 + w))
+                   : (j == 1 ? eet + hw * (1.0 + w) : eet + hw * (1.0 - w));
+    if ((p == 9 || p == 10) && ef <= 0.0) ef = 1.0e-5 * hw;
+    return ef;
+  }
+
   // The scattering routines that end in get_state are split in two around it:
   // prep_* does what comes before get_state and returns which finish() applies
   // after it, so get_state (with getK) and the process selection are compiled
@@ -541,9 +598,13 @@ struct Flight {
       const int p = iscat % B.scpre;
       if (p == 0) TALIND(1);
       else if (p <= 12) {
-        const double temps[6] = {B.temptag, B.templag, B.templog, B.temptaf, B.templaf, B.temptof};
-        const double hw = temps[(p - 1) >> 1];
-        energy = (p & 1) ? energy + hw : energy - hw;
+        if (B.ptab[0] && (p == 5 || p == 6 || p >= 9)) {
+          energy = dispersive(p, energy);
+        } else {
+          const double temps[6] = {B.temptag, B.templag, B.templog, B.temptaf, B.templaf, B.temptof};
+          const double hw = temps[(p - 1) >> 1];
+          energy = (p & 1) ? energy + hw : energy - hw;
+        }
         TALIND(p <= 6 ? 2 : 3);
       }
       return POST_EPSCAT;

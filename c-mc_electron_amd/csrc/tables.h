@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "devmath.h"
+#include "emc_const.h"
 
 namespace emc {
 
@@ -88,6 +89,7 @@ struct BandDev {
   double dtable, emax, dlist, dlists, a0pi, sieg, mell, melt, meld;
   double difpr[3];
   double temptag, templag, templog, temptaf, templaf, temptof, temphop;
+  const double* ptab[PTAB_N];  // dispersive phonon tables, column-contiguous; null: not used
 };
 
 struct MeshDev {

@@ -58,6 +58,7 @@ class BandView(C.Structure):
         ("tsphere", _d),
         ("tindex_nodes", C.c_int), ("tindex_entries", C.c_int),
         ("tindex_root", C.c_void_p), ("tindex_node", C.POINTER(C.c_int)), ("tindex_entry", C.c_void_p),
+        ("ptab", _d * 6),
     ]
 
 

@@ -121,6 +121,10 @@ typedef struct emc_band_view {
   const void *tindex_entry;            /* [entries] {float lo[3], hi[3]; int32 tet;
                                            uint32 bins: tlist lo 0-8, hi 9-17, tlists lo
                                            18-23, hi 24-29}                           */
+  const double *ptab[6];               /* dispersive phonon tables (init_deep), [row][col]:
+                                           kloem 201x120, kloab 201x110, ktoem 51x160,
+                                           ktoab 51x150, klaem 51x160, klaab 51x150; NULL
+                                           unless the deck sets Flag_QuantumCorrection    */
 } emc_band_view;
 
 /* Host view of the mesh.  cell c = (j*cnx + i)*cnz + k ; node n = (j*pnx + i)*pnz + k. */

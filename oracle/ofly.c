@@ -505,12 +505,143 @@ static void heat_to_point(FS *s, double e, int64_t *heat) {
   heat[NODE(i + 1, j + 1, k + 1)] += fx(e * ry * rx * rz, EMC_FX_HEAT);
 }
 
-/* EPSCAT, src/ElectronPhononScatter.cpp:98-195 (Jacoboni system, QC off) */
+/* The dispersive phonon final-energy draws used by EPSCAT in quantum-correction
+ * runs, src/ElectronPhononScatter.cpp:632-1022, one function each as the
+ * reference has them; tables indexed [row][col] as init_deep reads them
+ * (src/ParticleUtils.cpp:750-808). */
+#define PT(t, r, c, ncol) (B->ptab[t][(size_t)(r) * (ncol) + (c)])
+
+static double FEKLOEM(FS *s, double eet) { /* :632-700, table kloem 201 x 120 */
+  int i, j, k, it, ei;
+  double ie, edeta, ran, ksum, efinal;
+  ie = eet / B->templog;
+  if (ie < 1.0) return 1.0e-10;
+  if (ie < 3.05) { it = (int)((ie - 1.0) * 10.0 + 0.5); if (it < 1) it = 1; ei = it; edeta = 1.0; }
+  else if (ie < 5.625) { it = (int)((ie - 3.0) * 4.0 + 0.5); if (it < 1) it = 1; ei = 20 + it; edeta = 2.0; }
+  else if (ie < 13.25) { it = (int)((ie - 5.5) * 2.0 + 0.5); if (it < 1) it = 1; ei = 30 + it; edeta = 4.0; }
+  else if (ie < 28.5) { it = (int)(ie - 13.0 + 0.5); if (it < 1) it = 1; ei = 45 + it; edeta = 8.0; }
+  else { it = (int)((ie - 28.0) + 0.5); if (it < 1) it = 1; if (it > 60) it = 60; ei = 60 + it; edeta = 12.0; }
+  ran = Random(s);
+  i = 0;
+  ksum = 0.0;
+  while (ksum < ran) {
+    i++;
+    if (i == 201) { ksum = 1.1; i = 1; }
+    ksum += PT(0, i - 1, ei - 1, 120);
+  }
+  j = i % 2;
+  k = i / 2;
+  if (j == 1) efinal = eet - B->templog * (1.0 - 0.01 * edeta * k);
+  else efinal = eet - B->templog * (1.0 + 0.01 * edeta * k);
+  return efinal;
+}
+
+static double FEKLOAB(FS *s, double eet) { /* :702-768, table kloab 201 x 110 */
+  int i, j, k, it, ei;
+  double ie, edeta, ran, ksum, efinal;
+  ie = eet / B->templog;
+  if (ie < 1.05) { it = (int)(ie * 10.0 + 0.5); if (it < 1) it = 1; ei = it; edeta = 1.0; }
+  else if (ie < 3.625) { it = (int)((ie - 1.0) * 4.0 + 0.5); if (it < 1) it = 1; ei = 10 + it; edeta = 2.0; }
+  else if (ie < 11.25) { it = (int)((ie - 3.5) * 2.0 + 0.5); if (it < 1) it = 1; ei = 20 + it; edeta = 4.0; }
+  else if (ie < 26.5) { it = (int)((ie - 11.0) + 0.5); if (it < 1) it = 1; ei = 35 + it; edeta = 8.0; }
+  else { it = (int)((ie - 26.0) + 0.5); if (it < 1) it = 1; if (it > 60) it = 60; ei = 50 + it; edeta = 12.0; }
+  ran = Random(s);
+  i = 0;
+  ksum = 0.0;
+  while (ksum < ran) {
+    i++;
+    ksum += PT(1, i - 1, ei - 1, 110);
+    if (i == 201) { ksum = 1.1; i = 1; }
+  }
+  j = i % 2;
+  k = i / 2;
+  if (j == 1) efinal = eet + B->templog * (1.0 + 0.01 * edeta * k);
+  else efinal = eet + B->templog * (1.0 - 0.01 * edeta * k);
+  return efinal;
+}
+
+/* FEKLAAB (:770-826, klaab 51 x 150, phonon templaf) and FEKTOAB (:891-955, ktoab
+ * 51 x 150, phonon temptof) share their bins; "it > 60" sets it to 115 in both. */
+static double fek_ab51(FS *s, double eet, double hw, int tab, int clamp) {
+  int i, j, k, it, ei;
+  double ie, edeta, ran, ksum, efinal;
+  ie = eet / hw;
+  if (ie < 1.05) { it = (int)(ie * 10.0 + 0.5); if (it < 1) it = 1; ei = it; edeta = 0.25; }
+  else if (ie < 3.625) { it = (int)((ie - 1.0) * 4.0 + 0.5); if (it < 1) it = 1; ei = 10 + it; edeta = 0.25; }
+  else if (ie < 11.25) { it = (int)((ie - 3.5) * 2.0 + 0.5); if (it < 1) it = 1; ei = 20 + it; edeta = 0.5; }
+  else { it = (int)((ie - 11.0) + 0.5); if (it < 1) it = 1; if (it > 60) it = 115; ei = 35 + it; edeta = 1.0; }
+  ran = Random(s);
+  i = 0;
+  ksum = 0.0;
+  while (ksum < ran) {
+    i++;
+    ksum += PT(tab, i - 1, ei - 1, 150);
+    if (i == 51) { ksum = 1.1; i = 1; }
+  }
+  j = i % 2;
+  k = i / 2;
+  if (j == 1) efinal = eet + hw * (1.0 + 0.04 * edeta * k);
+  else efinal = eet + hw * (1.0 - 0.04 * edeta * k);
+  if (clamp && efinal <= 0.0) efinal = (1.0e-5) * hw;
+  return efinal;
+}
+
+/* FEKLAEM (:828-889, klaem 51 x 160, templaf; below one phonon -1e-10) and FEKTOEM
+ * (:957-1022, ktoem 51 x 160, temptof; below one phonon 1e-10). */
+static double fek_em51(FS *s, double eet, double hw, int tab, int clamp, double below) {
+  int i, j, k, it, ei;
+  double ie, edeta, ran, ksum, efinal;
+  ie = eet / hw;
+  if (ie < 1.0) return below;
+  if (ie < 3.05) { it = (int)((ie - 1) * 10.0 + 0.5); if (it < 1) it = 1; ei = it; edeta = 0.25; }
+  else if (ie < 5.625) { it = (int)((ie - 3.0) * 4.0 + 0.5); if (it < 1) it = 1; ei = 20 + it; edeta = 0.5; }
+  else if (ie < 13.25) { it = (int)((ie - 5.5) * 2.0 + 0.5); if (it < 1) it = 1; ei = 30 + it; edeta = 0.5; }
+  else { it = (int)((ie - 13.0) + 0.5); if (it < 1) it = 1; if (it > 115) it = 115; ei = 45 + it; edeta = 1.0; }
+  ran = Random(s);
+  i = 0;
+  ksum = 0.0;
+  while (ksum < ran) {
+    i++;
+    ksum += PT(tab, i - 1, ei - 1, 160);
+    if (i == 51) { ksum = 1.1; i = 1; }
+  }
+  j = i % 2;
+  k = i / 2;
+  if (j == 1) efinal = eet - hw * (1.0 - 0.04 * edeta * k);
+  else efinal = eet - hw * (1.0 + 0.04 * edeta * k);
+  if (clamp && efinal <= 0.0) efinal = (1.0e-5) * hw;
+  return efinal;
+}
+
+static double FEKLAAB(FS *s, double eet) { return fek_ab51(s, eet, B->templaf, 5, 1); }
+static double FEKLAEM(FS *s, double eet) { return fek_em51(s, eet, B->templaf, 4, 1, -1e-10); }
+static double FEKTOAB(FS *s, double eet) { return fek_ab51(s, eet, B->temptof, 3, 0); }
+static double FEKTOEM(FS *s, double eet) { return fek_em51(s, eet, B->temptof, 2, 0, 1e-10); }
+
+/* EPSCAT, src/ElectronPhononScatter.cpp:98-195 (Jacoboni system; with the quantum
+ * correction on, the optical and f-type LA/TO final energies come from the
+ * dispersive tables, :148-163) */
 static void EPSCAT(FS *s, int iscat) {
   const double eeold = s->energy, xkold = s->kx, ykold = s->ky, zkold = s->kz;
   const int ibold = s->iband, itetold = s->itet, isymold = s->isym;
   s->iband = iscat / B->scpre + B->bandof[s->par_type];
-  switch (iscat % B->scpre) {
+  if (M->flag_qc) switch (iscat % B->scpre) {
+    case 0: TALIND(s, 1); break;
+    case 1: s->energy = s->energy + B->temptag; TALIND(s, 2); break;
+    case 2: s->energy = s->energy - B->temptag; TALIND(s, 2); break;
+    case 3: s->energy = s->energy + B->templag; TALIND(s, 2); break;
+    case 4: s->energy = s->energy - B->templag; TALIND(s, 2); break;
+    case 5: s->energy = FEKLOAB(s, s->energy); TALIND(s, 2); break;
+    case 6: s->energy = FEKLOEM(s, s->energy); TALIND(s, 2); break;
+    case 7: s->energy = s->energy + B->temptaf; TALIND(s, 3); break;
+    case 8: s->energy = s->energy - B->temptaf; TALIND(s, 3); break;
+    case 9: s->energy = FEKLAAB(s, s->energy); TALIND(s, 3); break;
+    case 10: s->energy = FEKLAEM(s, s->energy); TALIND(s, 3); break;
+    case 11: s->energy = FEKTOAB(s, s->energy); TALIND(s, 3); break;
+    case 12: s->energy = FEKTOEM(s, s->energy); TALIND(s, 3); break;
+    default: break;
+  }
+  else switch (iscat % B->scpre) {
     case 0: TALIND(s, 1); break;
     case 1: s->energy = s->energy + B->temptag; TALIND(s, 2); break;
     case 2: s->energy = s->energy - B->temptag; TALIND(s, 2); break;
@@ -1189,4 +1320,24 @@ int orc_getisym(const emc_band_view *b, const double k[3], double energy, int ib
   *isym = s->isym;
   *itet = s->itet;
   return s->Flag_Catch ? -1 : 0;
+}
+
+/* Test hook: the final energy electron phonon process `process` (5, 6, 9-12)
+ * draws from the dispersive tables for energy eet, with the one random draw taken
+ * from the stream of (key, step). */
+double orc_fek(const emc_band_view *b, int process, double eet, uint64_t key, uint64_t step) {
+  FS st;
+  memset(&st, 0, sizeof st);
+  FS *s = &st;
+  s->b = b;
+  rng_init(&s->rng, key, step);
+  switch (process) {
+    case 5: return FEKLOAB(s, eet);
+    case 6: return FEKLOEM(s, eet);
+    case 9: return FEKLAAB(s, eet);
+    case 10: return FEKLAEM(s, eet);
+    case 11: return FEKTOAB(s, eet);
+    case 12: return FEKTOEM(s, eet);
+    default: return eet;
+  }
 }

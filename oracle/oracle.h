@@ -101,6 +101,11 @@ void orc_apply_full(const emc_mesh_view *m, const double *x, double *y);
  * of the quantum region (src/mcmodel.cpp:2337-2662); returns 0 */
 int orc_qc_potential(const emc_mesh_view *m, const double *pot, double *out);
 
+/* ---- dispersive phonon final energy (test hook; FEKLOAB .. FEKTOEM,
+ * src/ElectronPhononScatter.cpp:632-1022): process 5/6/9/10/11/12 at energy eet,
+ * its one draw from the stream of (key, step) */
+double orc_fek(const emc_band_view *b, int process, double eet, uint64_t key, uint64_t step);
+
 #ifdef __cplusplus
 }
 #endif

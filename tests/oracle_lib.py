@@ -53,6 +53,7 @@ def load():
         "orc_pcg": (C.c_int, [P(abi.MeshView), d, d, C.c_double, C.c_int, d]),
         "orc_apply_full": (None, [P(abi.MeshView), d, d]),
         "orc_qc_potential": (C.c_int, [P(abi.MeshView), d, d]),
+        "orc_fek": (C.c_double, [P(abi.BandView), C.c_int, C.c_double, C.c_uint64, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -75,3 +75,29 @@ def test_qc_potential_and_step():
         np.testing.assert_array_equal(w["stat_qc_pot"], ((z + p1) + p2) + qpot)
     finally:
         e.close()
+
+
+def test_dispersive_phonon_flight():
+    """With the quantum correction on, EPSCAT takes the g-LO and f-LA/TO final
+    energies from the dispersive tables (src/ElectronPhononScatter.cpp:148-163):
+    two flight steps of 1e-14 s under a device-like field (thousands of phonon
+    events) equal the oracle's, particles and tallies bit for bit."""
+    from helpers import field_like
+    from test_gpu_parity import assert_same_tallies
+    e = emcgpu.Engine(cfg1_deck(qc=1, dt=1e-14), device=0)
+    try:
+        e.particles_init()
+        pot, f = field_like(e, seed=5, scale=20.0)
+        e.fields_set(f)
+        p = e.download()
+        phonon = 0
+        for step in range(2):
+            print(f"[gpu] dispersive-table flight step {step}", flush=True)
+            tg = e.fly_step(step)
+            p, to, _ = oracle_lib.fly_step(e, p, p.n, f, step)
+            assert_same_population(e.download(), p, f"step {step}")
+            assert_same_tallies(tg, to, f"step {step}")
+            phonon += tg.n_phonon
+        assert phonon > 1000, phonon
+    finally:
+        e.close()
