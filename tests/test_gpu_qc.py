@@ -101,3 +101,50 @@ def test_dispersive_phonon_flight():
         assert phonon > 1000, phonon
     finally:
         e.close()
+
+
+def test_checkpoint_resume_with_correction_and_heat_phase(tmp_path):
+    """A checkpoint taken with the quantum correction active, and one taken in the
+    heating phase, resume exactly: particles, potentials, both potential sums of
+    the window, heat."""
+    deck = cfg1_deck(qc=1, quantum_start_step=1, stat_step=4)
+    a = emcgpu.Engine(deck, device=0)
+    b = emcgpu.Engine(deck, device=0)
+    try:
+        a.particles_init()
+        a.tallies_reset()
+        a.step(3)
+        path = str(tmp_path / "qc.ckpt")
+        a.checkpoint_save(path)
+        b.checkpoint_load(path)
+        print("[gpu] resume with the correction active", flush=True)
+        a.step(2)
+        b.step(2)
+        assert_same_population(a.download(), b.download(), "qc resume")
+        wa, wb = a.stat_window(), b.stat_window()
+        for k in ("stat_pot", "stat_qc_pot", "node_stat", "cell_count"):
+            np.testing.assert_array_equal(wa[k], wb[k], err_msg=k)
+        np.testing.assert_array_equal(a.fields_get().ex, b.fields_get().ex)
+        # heating phase: window means of the last window, then heat steps
+        a.write_stat(wa, str(tmp_path), 4)
+        a.tallies_reset()
+        a.heat_begin()
+        a.heat_step(1)
+        path = str(tmp_path / "heat.ckpt")
+        a.checkpoint_save(path)
+        b.checkpoint_load(path)
+        print("[gpu] resume in the heating phase", flush=True)
+        a.heat_step(2)
+        b.heat_step(2)
+        assert_same_population(a.download(), b.download(), "heat resume")
+        for x, y in zip(a.heat(), b.heat()):
+            np.testing.assert_array_equal(x, y)
+        wa, wb = a.stat_window(), b.stat_window()
+        for k in ("stat_pot", "stat_qc_pot", "node_stat", "heat_e", "heat_h"):
+            np.testing.assert_array_equal(wa[k], wb[k], err_msg=k)
+        fa, fb = a.fields_get(), b.fields_get()
+        np.testing.assert_array_equal(fa.ex, fb.ex)
+        np.testing.assert_array_equal(fa.hx, fb.hx)
+    finally:
+        a.close()
+        b.close()
