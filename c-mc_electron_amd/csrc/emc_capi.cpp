@@ -62,6 +62,14 @@ struct DevBuf {
   }
   template <class T>
   T* dup(const std::vector<T>& v) { return dup(v.data(), v.size()); }
+  // free one allocation made here (the device must no longer use it)
+  void release(const void* p) {
+    if (!p) return;
+    auto it = std::find(owned.begin(), owned.end(), p);
+    if (it == owned.end()) return;
+    (void)hipFree(*it);
+    owned.erase(it);
+  }
   void free_all() {
     for (void* p : owned) (void)hipFree(p);
     owned.clear();
@@ -100,6 +108,7 @@ struct Gpu {
   // MultipleRefresh work arrays, allocated on first use
   MrArgs mr{};
   bool mr_ready = false;
+  long long mr_cap = 0;               // capacity the slot-sized MultipleRefresh arrays hold
   uint32_t* slow = nullptr;           // [cap] flight slow list
   // fields / mesh work arrays
   double *ex = nullptr, *ey = nullptr, *ez = nullptr, *rho = nullptr, *rates = nullptr;
@@ -532,6 +541,14 @@ static int gpu_setup(emc_ctx* c) {
   }
   HIPCHECK(c, hipDeviceSynchronize());
   return EMC_OK;
+}
+
+static void release_part(DevBuf& m, PartDev& P) {
+  for (const void* p : {(const void*)P.x, (const void*)P.y, (const void*)P.z, (const void*)P.kx, (const void*)P.ky,
+                        (const void*)P.kz, (const void*)P.energy, (const void*)P.charge, (const void*)P.pk,
+                        (const void*)P.itet, (const void*)P.isym, (const void*)P.key})
+    m.release(p);
+  P = PartDev{};
 }
 
 static int gpu_alloc_particles(emc_ctx* c, long long cap) {
@@ -1330,12 +1347,28 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
   if (st) return st;
   const Device& d = c->dev;
   MrArgs& a = g.mr;
+  if (g.mr_ready && g.mr_cap < g.cap) {
+    // the population grew: slot-sized work arrays of the new capacity
+    DevBuf& m = g.mem;
+    for (const void* p : {(const void*)a.gk, (const void*)a.gk2, (const void*)a.idx, (const void*)a.idx2,
+                          (const void*)a.sk, (const void*)a.sk2, (const void*)a.temp})
+      m.release(p);
+    const long long cap = g.cap, ng = 2 * (long long)d.ncell;
+    a.gk = m.alloc<unsigned>(cap); a.gk2 = m.alloc<unsigned>(cap);
+    a.idx = m.alloc<unsigned>(cap); a.idx2 = m.alloc<unsigned>(cap);
+    a.sk = m.alloc<unsigned long long>(cap); a.sk2 = m.alloc<unsigned long long>(cap);
+    a.temp_bytes = emc_mr_temp_bytes(cap, ng);
+    a.temp = m.alloc<unsigned char>(a.temp_bytes > 0 ? a.temp_bytes : 1);
+    if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("MultipleRefresh buffers: ") + hipGetErrorString(m.last));
+    g.mr_cap = cap;
+  }
   if (!g.mr_ready) {
     DevBuf& m = g.mem;
     const long long cap = g.cap, ng = 2 * (long long)d.ncell;
     a.gk = m.alloc<unsigned>(cap); a.gk2 = m.alloc<unsigned>(cap);
     a.idx = m.alloc<unsigned>(cap); a.idx2 = m.alloc<unsigned>(cap);
     a.sk = m.alloc<unsigned long long>(cap); a.sk2 = m.alloc<unsigned long long>(cap);
+    g.mr_cap = cap;
     a.gstart = m.alloc<unsigned>(ng); a.gend = m.alloc<unsigned>(ng);
     a.cnt = m.alloc<unsigned long long>(ng); a.off = m.alloc<unsigned long long>(ng);
     a.tot = m.alloc<double>(ng); a.keep = m.alloc<int>(ng);
@@ -1361,7 +1394,37 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
   HIPCHECK(c, hipMemcpyAsync(&h[2], a.cnt + (a.ngroup - 1), 8, hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   const long long total = (long long)(h[1] + h[2]);
-  if (total > g.cap) return fail(c, EMC_ERR_CAPACITY, "MultipleRefresh: the refreshed population exceeds the device capacity");
+  if (total > g.cap) {
+    // The refreshed population needs more slots than it was given: it is written
+    // into new buffers of 1.25x its size, and the flight's work lists grow with them.
+    const long long cap = (long long)std::ceil(total * 1.25) + 1024;
+    if (cap > 0x7fffffffLL) return fail(c, EMC_ERR_CAPACITY, "MultipleRefresh: population above 2^31-1 slots");
+    DevBuf& m = g.mem;
+    PartDev out{}, spare{};
+    alloc_part(m, out, cap);
+    alloc_part(m, spare, cap);
+    const unsigned long long clone_cap = (unsigned long long)std::max<long long>(4096, cap / 16);
+    CloneRec* clones = m.alloc<CloneRec>(clone_cap);
+    long long* dead = m.alloc<long long>(clone_cap);
+    uint32_t* slow = m.alloc<uint32_t>(cap);
+    if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("particle allocation: ") + hipGetErrorString(m.last));
+    a.Q = out;
+    HIPCHECK(c, emc_launch_mr_write(a, g.stream));
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    release_part(m, g.P);
+    release_part(m, g.P2);
+    m.release(g.clones); m.release(g.dead); m.release(g.slow);
+    g.P = out;
+    g.P2 = spare;
+    g.cap = cap;
+    g.clones = clones; g.clone_cap = clone_cap;
+    g.dead = dead; g.dead_cap = clone_cap;
+    g.slow = slow;
+    g.n = total;
+    if (before) *before = (long long)h[0];
+    if (after) *after = total;
+    return EMC_OK;
+  }
   HIPCHECK(c, emc_launch_mr_write(a, g.stream));
   std::swap(g.P, g.P2);
   g.n = total;

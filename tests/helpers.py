@@ -7,6 +7,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(REPO, "c-mc_electron_amd")
+GOLDEN_DIR = os.path.join(REPO, "tests", "golden")
 if PKG not in sys.path:
     sys.path.insert(0, PKG)
 

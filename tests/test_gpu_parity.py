@@ -341,3 +341,44 @@ def test_flight_parity_large_population():
         assert tg.n_full_loop > 0 and tg.n_phonon + tg.n_impurity > 100
     finally:
         e.close()
+
+
+def test_multiple_refresh_grows_the_population():
+    """A refresh whose output needs more slots than the population was given:
+    the buffers grow (new particle buffers, flight lists and work arrays), the
+    output equals the oracle's slot for slot, and the engine steps and refreshes
+    on from the grown buffers."""
+    h = emcgpu.Engine(cfg1_deck(), device=-1)
+    try:
+        h.particles_init()
+        full = h.download()
+    finally:
+        h.close()
+    m_e = emcgpu.Engine(cfg1_deck(), device=0)
+    try:
+        m = m_e.mesh
+        live = np.nonzero(full.i[:full.n] >= 0)[0]
+        c = (full.j[live].astype(np.int64) * m.cnx + full.i[live]) * m.cnz + full.k[live]
+        g = 2 * c + full.ptype[live]
+        first = live[np.unique(g, return_index=True)[1]][:3000]
+        p = full.subset(first)
+        p.n = len(first)
+        m_e.upload(p)
+        print(f"[gpu] refresh growth: {p.n} particles uploaded", flush=True)
+        before, after = m_e.multiple_refresh(5)
+        po = oracle_lib.multiple_refresh(m_e, p, p.n, 5)
+        assert before == p.n and after == po.n
+        assert after > 1.25 * p.n + 1024, "the refresh did not need more slots"
+        pg = m_e.download()
+        for fld in emcgpu.PARTICLE_F64 + emcgpu.PARTICLE_I32 + ("key",):
+            np.testing.assert_array_equal(getattr(pg, fld)[:pg.n], getattr(po, fld)[:po.n], err_msg=fld)
+        m_e.step(2)
+        q = m_e.download()
+        before2, after2 = m_e.multiple_refresh(9)
+        qo = oracle_lib.multiple_refresh(m_e, q, q.n, 9)
+        qg = m_e.download()
+        assert after2 == qo.n
+        for fld in emcgpu.PARTICLE_F64 + emcgpu.PARTICLE_I32 + ("key",):
+            np.testing.assert_array_equal(getattr(qg, fld)[:qg.n], getattr(qo, fld)[:qo.n], err_msg=fld)
+    finally:
+        m_e.close()
