@@ -4,7 +4,11 @@ steps of 0.2 fs with MultipleRefresh every 50 steps.  The window-mean potential 
 steps 6 000..6 999 (pot6999, output_stat) equals the one in the result file the
 reference ships for its own run of this deck (finfet/pot6999.vtr, values printed
 to 6 significant digits; tests/golden/finfet_pot6999.npz) within 20 uV at every
-node.  Measured: 0.8 uV rms, 5.1 uV at most, the print precision of the file."""
+node.  Measured: 0.8 uV rms, 5.1 uV at most, the print precision of the file.
+The potential of this deck is set by its doping, oxide and contacts: the first
+solve from the initial (neutral) population already agrees within 11 uV.  The
+check therefore covers the deck, mesh, operator, deposit, solve and window
+average against the reference's own output, not the carrier transport."""
 import os
 
 import numpy as np
