@@ -279,6 +279,18 @@ __device__ inline int tile_of(const SortArgs& a, uint32_t pk) {
 // lane adds the run length to ctr[key] with one atomic and every lane of the run
 // gets old value + its rank in the run.  key < 0 takes no part.  All 64 lanes of
 // the wave must call it together.
+// The same counting without the positions: no lane waits for the atomics.
+__device__ inline void wave_run_count(unsigned* ctr, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const unsigned long long above = heads & ~upto;
+  const int nh = above ? __builtin_ctzll(above) : 64;
+  if (head && key >= 0) (void)atomicAdd(&ctr[key], (unsigned)(nh - lane));
+}
+
 __device__ inline unsigned wave_run_add(unsigned* ctr, int key) {
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(key, 1, 64);
@@ -303,7 +315,7 @@ __global__ __launch_bounds__(256) void k_tile_hist(SortArgs a) {
       const uint32_t pk = a.P.pk[q];
       if (pk != PK_DEAD) t = tile_of(a, pk);
     }
-    (void)wave_run_add(a.tcount, t);
+    wave_run_count(a.tcount, t);
   }
 }
 
@@ -353,13 +365,22 @@ __global__ __launch_bounds__(256) void k_tile_scatter(SortArgs a) {
       pk = a.P.pk[q];
       if (pk != PK_DEAD) t = tile_of(a, pk);
     }
+    // the particle is read before the slot is known, so the loads overlap the
+    // cursor atomic's round trip
+    double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0, en = 0, ch = 0;
+    int32_t itet = 0, isym = 0;
+    uint64_t key = 0;
+    if (t >= 0) {
+      x = a.P.x[q]; y = a.P.y[q]; z = a.P.z[q]; kx = a.P.kx[q]; ky = a.P.ky[q]; kz = a.P.kz[q];
+      en = a.P.energy[q]; ch = a.P.charge[q]; itet = a.P.itet[q]; isym = a.P.isym[q]; key = a.P.key[q];
+    }
     const unsigned r = wave_run_add(a.tcursor, t);
     if (t < 0) continue;
     const size_t d = (size_t)a.toff[t] + r;
-    a.Q.x[d] = a.P.x[q]; a.Q.y[d] = a.P.y[q]; a.Q.z[d] = a.P.z[q];
-    a.Q.kx[d] = a.P.kx[q]; a.Q.ky[d] = a.P.ky[q]; a.Q.kz[d] = a.P.kz[q];
-    a.Q.energy[d] = a.P.energy[q]; a.Q.charge[d] = a.P.charge[q];
-    a.Q.pk[d] = pk; a.Q.itet[d] = a.P.itet[q]; a.Q.isym[d] = a.P.isym[q]; a.Q.key[d] = a.P.key[q];
+    a.Q.x[d] = x; a.Q.y[d] = y; a.Q.z[d] = z;
+    a.Q.kx[d] = kx; a.Q.ky[d] = ky; a.Q.kz[d] = kz;
+    a.Q.energy[d] = en; a.Q.charge[d] = ch;
+    a.Q.pk[d] = pk; a.Q.itet[d] = itet; a.Q.isym[d] = isym; a.Q.key[d] = key;
   }
 }
 
