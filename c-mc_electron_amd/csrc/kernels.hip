@@ -1189,9 +1189,11 @@ hipError_t emc_launch_clones_dev(const FlyArgs& a, unsigned long long* ctr, hipS
 }
 // sort P into Q by tile; toff[ntile] is the live count afterwards
 hipError_t emc_launch_tile_sort(const SortArgs& a, hipStream_t s) {
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  // 2048 blocks striding over the population: the per-particle work is short, and
+  // one block per 256 particles (12 k blocks at 3 M) costs more in block dispatch
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, a);
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
