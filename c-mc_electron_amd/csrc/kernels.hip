@@ -1216,7 +1216,7 @@ hipError_t emc_launch_qc_potential(const QcArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t emc_launch_field(const FieldArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_field, dim3(grid_for((long long)a.ncell, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_field, dim3(grid_for((long long)a.ncell, 256, 1024)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_sample(const SampleArgs& a, hipStream_t s) {
