@@ -1303,17 +1303,24 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
   // queued iterations are then no-ops and the first check below stops the loop
   HIPCHECK(c, emc_launch_pcg_start(a, g.mg, g.stream));
   double* sh = g.scal_host;
+  // Each check reads the residual of the iteration before the one just queued, so
+  // the GPU never waits for the host; the iteration the previous solve converged in
+  // is checked as soon as it is queued instead, so that a solve converging there
+  // queues no iteration that would only find the done flag set.
+  const int guess = c->last_pcg_iters;
+  int checked = -1;
   for (int k = 0; k < maxit; ++k) {
     HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
     double* slot = sh + 8 * (k & 1);
     HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
     HIPCHECK(c, hipEventRecord(g.scal_ready[k & 1], g.stream));
-    if (k == 0) continue;
-    // residual after iteration k-1 (iteration k is already queued)
-    HIPCHECK(c, hipEventSynchronize(g.scal_ready[(k - 1) & 1]));
-    const double* prev = sh + 8 * ((k - 1) & 1);
-    if (!std::isfinite(prev[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
-    if (prev[0] <= prev[4]) break;
+    const int chk = (k + 1 == guess) ? k : k - 1;
+    if (chk < 0 || chk <= checked) continue;
+    HIPCHECK(c, hipEventSynchronize(g.scal_ready[chk & 1]));
+    checked = chk;
+    const double* res = sh + 8 * (chk & 1);
+    if (!std::isfinite(res[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
+    if (res[0] <= res[4]) break;
   }
   HIPCHECK(c, hipMemcpyAsync(sh + 16, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
   g.pcg_pending = true;
