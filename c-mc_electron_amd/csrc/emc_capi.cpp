@@ -104,7 +104,8 @@ struct Gpu {
                                       // [4] [5] clone range of the next pass [6] clone passes with work
   unsigned long long* ctr_host = nullptr;    // pinned copy of ctr after the flight
   unsigned long long* tally_host = nullptr;  // pinned copy of contact + counters after the flight
-  bool pcg_pending = false;           // the step's PCG result is still to be read (scal_host[16..23])
+  bool pcg_pending = false;           // the step's PCG result is still to be read
+  const double* pcg_result = nullptr; // where it will be: the pinned copy of the last queued iteration
   // MultipleRefresh work arrays, allocated on first use
   MrArgs mr{};
   bool mr_ready = false;
@@ -193,6 +194,9 @@ struct emc_ctx {
   bool heat_phase = false;            // the reference's flag_heat
   uint64_t heat_next = 0;             // next heat-phase step
 };
+
+// The flight's output block: contact columns, counters, clone/merge counters.
+static constexpr size_t FLYOUT_N = EMC_MAX_CONTACT * EMC_CT_NCOL + CNT_N + 8;
 
 // Heat-phase step s draws from RNG step HEAT_STEP0 + s, apart from the transient's.
 static constexpr uint64_t HEAT_STEP0 = 1ull << 40;
@@ -440,9 +444,11 @@ static int gpu_setup(emc_ctx* c) {
   g.stat_heat_h = m.alloc<unsigned long long>(NN);
   g.rcur = m.alloc<unsigned long long>(d.pny);
   g.lcur = m.alloc<unsigned long long>(d.pny);
-  g.contact = m.alloc<unsigned long long>(EMC_MAX_CONTACT * EMC_CT_NCOL);
-  g.counters = m.alloc<unsigned long long>(CNT_N);
-  g.ctr = m.alloc<unsigned long long>(8);
+  // contact columns, counters and the clone/merge counters in one block, so the
+  // host reads all of them after the flight with one copy
+  g.contact = m.alloc<unsigned long long>(FLYOUT_N);
+  g.counters = g.contact + EMC_MAX_CONTACT * EMC_CT_NCOL;
+  g.ctr = g.counters + CNT_N;
   g.cst = m.alloc<FlyConst>(1);
   g.ntx = (d.cnx + TILE_I - 1) / TILE_I;
   g.nty = (d.cny + TILE_J - 1) / TILE_J;
@@ -498,9 +504,8 @@ static int gpu_setup(emc_ctx* c) {
   HIPCHECK(c, hipMemset(g.tcount, 0, g.ntile * sizeof(unsigned)));
   HIPCHECK(c, hipHostMalloc((void**)&g.n_host, sizeof(unsigned), hipHostMallocDefault));
   HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 24 * sizeof(double), hipHostMallocDefault));
-  HIPCHECK(c, hipHostMalloc((void**)&g.ctr_host, 8 * sizeof(unsigned long long), hipHostMallocDefault));
-  HIPCHECK(c, hipHostMalloc((void**)&g.tally_host, (EMC_MAX_CONTACT * EMC_CT_NCOL + CNT_N) * sizeof(unsigned long long),
-                            hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.tally_host, FLYOUT_N * sizeof(unsigned long long), hipHostMallocDefault));
+  g.ctr_host = g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL + CNT_N;
   for (auto& e : g.scal_ready) HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHECK(c, hipEventCreateWithFlags(&g.n_ready, hipEventDisableTiming));
   HIPCHECK(c, hipMemset(g.ex, 0, NC * sizeof(double)));
@@ -849,7 +854,6 @@ void emc_destroy(emc_ctx* c) {
     if (c->gpu->n_ready) (void)hipEventDestroy(c->gpu->n_ready);
     if (c->gpu->n_host) (void)hipHostFree(c->gpu->n_host);
     if (c->gpu->scal_host) (void)hipHostFree(c->gpu->scal_host);
-    if (c->gpu->ctr_host) (void)hipHostFree(c->gpu->ctr_host);
     if (c->gpu->tally_host) (void)hipHostFree(c->gpu->tally_host);
     for (auto e : c->gpu->scal_ready)
       if (e) (void)hipEventDestroy(e);
@@ -1135,11 +1139,8 @@ static int launch_fly(emc_ctx* c, uint64_t step, int flag_heat, FlyArgs& a) {
   HIPCHECK(c, emc_launch_fly(a, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[6], g.stream));
   HIPCHECK(c, emc_launch_clones_dev(a, g.ctr, g.stream));
-  HIPCHECK(c, hipMemcpyAsync(g.ctr_host, g.ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
-  HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, g.stream));
-  HIPCHECK(c, hipMemcpyAsync(g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL, g.counters, CNT_N * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, FLYOUT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             g.stream));
   return EMC_OK;
 }
 
@@ -1163,11 +1164,8 @@ static int finish_fly(emc_ctx* c, const FlyArgs& a) {
       if (end > begin && ++passes > 64) return fail(c, EMC_ERR_STATE, "clone passes do not terminate");
     }
     HIPCHECK(c, emc_launch_merge(a, first, end, g.ctr + 1, g.ctr + 2, g.stream));
-    HIPCHECK(c, hipMemcpyAsync(h, g.ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
-    HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, g.stream));
-    HIPCHECK(c, hipMemcpyAsync(g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL, g.counters,
-                               CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, FLYOUT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               g.stream));
     HIPCHECK(c, hipStreamSynchronize(g.stream));
   }
   const unsigned long long surv = h[2];
@@ -1308,8 +1306,9 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
   // is checked as soon as it is queued instead, so that a solve converging there
   // queues no iteration that would only find the done flag set.
   const int guess = c->last_pcg_iters;
-  int checked = -1;
+  int checked = -1, last = -1;
   for (int k = 0; k < maxit; ++k) {
+    last = k;
     HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
     double* slot = sh + 8 * (k & 1);
     HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
@@ -1322,7 +1321,10 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
     if (!std::isfinite(res[0])) return fail(c, EMC_ERR_SOLVER, "PCG residual is not finite");
     if (res[0] <= res[4]) break;
   }
-  HIPCHECK(c, hipMemcpyAsync(sh + 16, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+  // the kernels queued after convergence leave scal as it was, so the copy of the
+  // last queued iteration holds the result (read once the stream has passed it)
+  if (last < 0) HIPCHECK(c, hipMemcpyAsync(sh + 16, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+  g.pcg_result = last < 0 ? sh + 16 : sh + 8 * (last & 1);
   g.pcg_pending = true;
   if (defer) return EMC_OK;
   HIPCHECK(c, hipStreamSynchronize(g.stream));
@@ -1333,7 +1335,7 @@ static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres) {
   Gpu& g = *c->gpu;
   if (!g.pcg_pending) return EMC_OK;
   g.pcg_pending = false;
-  const double* sh = g.scal_host + 16;
+  const double* sh = g.pcg_result;
   const double bn = std::sqrt(sh[2]) > 0 ? std::sqrt(sh[2]) : 1.0;
   const int it = (int)sh[3];
   const double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
