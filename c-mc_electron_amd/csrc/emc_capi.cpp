@@ -1133,7 +1133,7 @@ static int launch_fly(emc_ctx* c, uint64_t step, int flag_heat, FlyArgs& a) {
   if (st) return st;
   st = sync_fly_const(c);
   if (st) return st;
-  HIPCHECK(c, emc_launch_zero(g.ctr, 8, g.stream));
+  // every caller has cleared the block with the counters (zero_tallies)
   a = fly_args(c, step, flag_heat);
   HIPCHECK(c, hipEventRecord(g.ev[5], g.stream));
   HIPCHECK(c, emc_launch_fly(a, g.stream));
@@ -1202,10 +1202,10 @@ static void read_tallies(emc_ctx* c, emc_fly_tallies* out, int passes) {
   out->n_full_loop += g.last_full_loop;
 }
 
+// Contact columns, counters and the clone/merge counters of the flight: one block.
 static int zero_tallies(emc_ctx* c) {
   Gpu& g = *c->gpu;
-  HIPCHECK(c, emc_launch_zero(g.contact, EMC_MAX_CONTACT * EMC_CT_NCOL, g.stream));
-  HIPCHECK(c, emc_launch_zero(g.counters, CNT_N, g.stream));
+  HIPCHECK(c, emc_launch_zero(g.contact, FLYOUT_N, g.stream));
   return EMC_OK;
 }
 
