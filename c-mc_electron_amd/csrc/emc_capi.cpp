@@ -1927,7 +1927,7 @@ int emc_potential_get(emc_ctx* c, double* pot) {
 
 // statistic()'s window sums of the step; in the heat phase the node heat is
 // cleared once added (heat_density, src/heat.cpp:48-66).
-static int launch_stat_window(emc_ctx* c, bool heat_phase) {
+static StatArgs stat_args(emc_ctx* c, bool heat_phase) {
   Gpu& g = *c->gpu;
   StatArgs a{};
   a.nnode = c->dev.nnode; a.ncell = c->dev.ncell;
@@ -1939,8 +1939,7 @@ static int launch_stat_window(emc_ctx* c, bool heat_phase) {
   }
   a.qc_pot = g.qc_pot; a.stat_qc_pot = g.stat_qc_pot;
   a.zero_heat = heat_phase ? 1 : 0;
-  HIPCHECK(c, emc_launch_stat_window(a, g.stream));
-  return EMC_OK;
+  return a;
 }
 
 // One step of density() + update_particle (src/mcmodel.cpp:424-471), or with
@@ -1956,13 +1955,12 @@ static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double ts
   HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
   int st = run_sort(c);
   if (st) return st;
-  HIPCHECK(c, emc_launch_deposit(deposit_args(c, 1), g.stream));
-  st = launch_stat_window(c, heat_phase);
-  if (st) return st;
-  if (!heat_phase) {
-    RhsArgs ra{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
-    HIPCHECK(c, emc_launch_rhs(ra, g.stream));
-  }
+  // the deposit's node gather also forms the right-hand side (transient) and the
+  // window sums
+  DepositArgs da = deposit_args(c, 1);
+  da.st = stat_args(c, heat_phase);
+  if (!heat_phase) da.rhs = RhsArgs{NN, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
+  HIPCHECK(c, emc_launch_deposit(da, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
   if (!heat_phase) {
     st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr, true);

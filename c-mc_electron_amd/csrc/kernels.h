@@ -56,18 +56,6 @@ struct SortArgs {
   unsigned* tcursor;          // [ntile] zeroed by k_tile_scan
 };
 
-struct DepositArgs {
-  BandDev B;
-  MeshDev M;
-  PartDev P;                  // tile order
-  size_t nnode, ncell;
-  int ntx, nty, ntz, ntile;
-  const unsigned* toff;       // [ntile + 1]
-  unsigned long long* blk;    // [ntile][DEP_NQ][TILE_NODES] per-tile node sums
-  unsigned long long *node_q, *node_stat, *cell_q, *cell_n;  // any may be null
-  int stat_accumulate;        // 1: node_stat += this step's sums, 0: node_stat = sums
-  int pad0;
-};
 
 struct RhsArgs {
   size_t nnode;
@@ -78,7 +66,8 @@ struct RhsArgs {
   double* density;  // may be null
 };
 
-// statistic()'s per-step window sums (src/mcmodel.cpp:1189-1196, 1234)
+// statistic()'s per-step window sums (src/mcmodel.cpp:1189-1196, 1234), taken by
+// the deposit after its sums: before the solve, so pot is the previous step's solution
 struct StatArgs {
   size_t nnode, ncell;
   const double* pot;                        // the potential the step starts from
@@ -107,6 +96,22 @@ struct QcArgs {
   double eb;                                // normalised barrier
   const double* pot;                        // Poisson potential
   double* qc_pot;                           // effective potential (quantum nodes written)
+};
+
+struct DepositArgs {
+  BandDev B;
+  MeshDev M;
+  PartDev P;                  // tile order
+  size_t nnode, ncell;
+  int ntx, nty, ntz, ntile;
+  const unsigned* toff;       // [ntile + 1]
+  unsigned long long* blk;    // [ntile][DEP_NQ][TILE_NODES] per-tile node sums
+  unsigned long long *node_q, *node_stat, *cell_q, *cell_n;  // any may be null
+  int stat_accumulate;        // 1: node_stat += this step's sums, 0: node_stat = sums
+  int pad0;
+  // fused into the node gather and the tile kernel's cell pass when set:
+  RhsArgs rhs;                // the Poisson right-hand side (rhs.rhs null: not computed)
+  StatArgs st;                // the window sums (st.stat_pot null: not taken)
 };
 
 struct FieldArgs {
@@ -215,7 +220,6 @@ hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, unsigned long long* ctr,
 hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);
-hipError_t emc_launch_stat_window(const emc::StatArgs& a, hipStream_t s);
 hipError_t emc_launch_qc_potential(const emc::QcArgs& a, hipStream_t s);
 hipError_t emc_launch_field(const emc::FieldArgs& a, hipStream_t s);
 hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);

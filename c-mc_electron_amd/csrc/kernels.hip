@@ -384,13 +384,42 @@ __global__ __launch_bounds__(256) void k_tile_scatter(SortArgs a) {
   }
 }
 
+// compute_rhs at node n from its charge sum q (src/mcmodel.cpp:2858-2893)
+__device__ inline void rhs_node(const RhsArgs& a, size_t n, unsigned long long q) {
+  double d = (double)(long long)q / 281474976710656.0;
+  const double v = a.pvol[n];
+  if (fabs(v) > 0) d = d / v;
+  double r = 0.0;
+  if (fabs(d) > 0) r = d * v + a.pdop[n];
+  const uint8_t kind = a.kind[n];
+  if (kind == 1) r = a.dirichlet[n];
+  a.rhs[n] = kind == 0 ? r - a.corr[n] : r;
+  if (a.density) a.density[n] = d;
+}
+
+// statistic()'s window sums at node q
+__device__ inline void stat_node(const StatArgs& a, size_t q) {
+  a.stat_pot[q] += a.pot[q];
+  if (a.heat_e) {
+    a.stat_heat_e[q] += a.heat_e[q];
+    a.stat_heat_h[q] += a.heat_h[q];
+    if (a.zero_heat) {
+      a.heat_e[q] = 0;
+      a.heat_h[q] = 0;
+    }
+  }
+  if (a.qc_pot) a.stat_qc_pot[q] += a.qc_pot[q];
+}
+
 // ------------------------------------------------------------------ deposit
 // statistic (src/mcmodel.cpp:1167-1408), particle_to_density (:348-417) and
 // compute_cell_charge (:2664-2712) in one pass over the tile-sorted particles.
 // One workgroup per tile sums the fixed-point contributions in LDS; each cell
 // belongs to one tile and is stored directly, the node block goes to `blk` and
 // k_deposit_gather adds the (up to 8) blocks that share a node.  Integer sums:
-// the result does not depend on the order of the particles.
+// the result does not depend on the order of the particles.  In a step the
+// gather also forms the Poisson right-hand side and statistic()'s potential and
+// heat sums of each node, and the cell pass adds the cell counts to the window.
 __global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
   __shared__ unsigned long long sn[DEP_NQ * TILE_NODES];
   __shared__ unsigned long long sc[4 * TILE_CELLS];
@@ -465,6 +494,7 @@ __global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
     if (i >= a.M.cnx || j >= a.M.cny || k >= a.M.cnz) continue;
     const size_t c = ((size_t)j * a.M.cnx + i) * a.M.cnz + k;
     if (a.cell_n) a.cell_n[c] = sc[3 * TILE_CELLS + e];
+    if (a.st.stat_enum) a.st.stat_enum[c] += sc[3 * TILE_CELLS + e];
     if (a.cell_q) {
       a.cell_q[c] = sc[e];
       a.cell_q[NC + c] = sc[TILE_CELLS + e];
@@ -504,6 +534,8 @@ __global__ __launch_bounds__(256) void k_deposit_gather(DepositArgs a) {
           for (int q = 0; q < DEP_NQ; ++q) s[q] += blk[(size_t)q * TILE_NODES];
         }
     if (a.node_q) a.node_q[n] = s[0];
+    if (a.rhs.rhs) rhs_node(a.rhs, n, s[0]);
+    if (a.st.stat_pot) stat_node(a.st, n);
     if (a.node_stat) {
 #pragma unroll
       for (int q = 0; q < 10; ++q) {
@@ -517,29 +549,6 @@ __global__ __launch_bounds__(256) void k_deposit_gather(DepositArgs a) {
 // ------------------------------------------------------------------ RHS / fields
 // particle_to_density normalisation (src/mcmodel.cpp:407-416) + compute_rhs (:2858-2893),
 // eliminated form: rhs_u - sum over contact neighbours a_uv phi_v.
-// Window sums of one step, taken where statistic() takes them: before the
-// solve, so pot is the previous step's solution.  Node and cell ranges share
-// one grid-stride loop.
-__global__ __launch_bounds__(256) void k_stat_window(StatArgs a) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t m = a.nnode > a.ncell ? a.nnode : a.ncell;
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += stride) {
-    if (q < a.nnode) {
-      a.stat_pot[q] += a.pot[q];
-      if (a.heat_e) {
-        a.stat_heat_e[q] += a.heat_e[q];
-        a.stat_heat_h[q] += a.heat_h[q];
-        if (a.zero_heat) {
-          a.heat_e[q] = 0;
-          a.heat_h[q] = 0;
-        }
-      }
-    }
-    if (q < a.ncell) a.stat_enum[q] += a.cell_n[q];
-    if (a.qc_pot && q < a.nnode) a.stat_qc_pot[q] += a.qc_pot[q];
-  }
-}
-
 // One lane per node of the quantum region; the window's sources are read in the
 // reference's loop order (i, then j, then k) and every term is formed as
 // ((V + Eb) * wz) * wy * wx, so the sum is the oracle's to the last bit.
@@ -577,18 +586,11 @@ __global__ __launch_bounds__(256) void k_qc_potential(QcArgs a) {
   }
 }
 
+
 __global__ void k_rhs(RhsArgs a) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.nnode; n += stride) {
-    double d = (double)(long long)a.node_q[n] / 281474976710656.0;
-    const double v = a.pvol[n];
-    if (fabs(v) > 0) d = d / v;
-    double r = 0.0;
-    if (fabs(d) > 0) r = d * v + a.pdop[n];
-    const uint8_t kind = a.kind[n];
-    if (kind == 1) r = a.dirichlet[n];
-    a.rhs[n] = kind == 0 ? r - a.corr[n] : r;
-    if (a.density) a.density[n] = d;
+    rhs_node(a, n, a.node_q[n]);
   }
 }
 
@@ -1203,11 +1205,6 @@ hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
 }
 hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)a.nnode, 256)), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-hipError_t emc_launch_stat_window(const StatArgs& a, hipStream_t s) {
-  const long long m = (long long)(a.nnode > a.ncell ? a.nnode : a.ncell);
-  hipLaunchKernelGGL(k_stat_window, dim3(grid_for(m, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_qc_potential(const QcArgs& a, hipStream_t s) {
