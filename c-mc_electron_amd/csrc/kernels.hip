@@ -691,6 +691,23 @@ __device__ inline void total3(const double* part, int nb, double* out) {
   __syncthreads();
 }
 
+// First components of the totals of two partial arrays, each summed in the same
+// order as total3 sums it (the CG needs only r.z and p.q of its partials, so the
+// other two components are not read).
+__device__ inline void total_first2(const double* pa, const double* pb, int nb, double& ta, double& tb) {
+  __shared__ double sh[2][4];
+  double a0 = 0, b0 = 0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) { a0 += pa[3 * q]; b0 += pb[3 * q]; }
+  a0 = wave_sum(a0);
+  b0 = wave_sum(b0);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[0][w] = a0; sh[1][w] = b0; }
+  __syncthreads();
+  ta = tb = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { ta += sh[0][q]; tb += sh[1][q]; }
+  __syncthreads();
+}
+
 __device__ inline void node_ijk(const PcgArgs& a, size_t n, int& i, int& j, int& k) {
   k = (int)(n % a.pnz);
   const size_t t = n / a.pnz;
@@ -742,10 +759,9 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_o
   if (*a.done) return;
   double beta = 0.0;
   if (!first) {
-    double to[3], tn[3];
-    total3(rz_old, a.nblocks, to);
-    total3(rz_new, a.nblocks, tn);
-    beta = (to[0] != 0.0) ? tn[0] / to[0] : 0.0;
+    double to, tn;
+    total_first2(rz_old, rz_new, a.nblocks, to, tn);
+    beta = (to != 0.0) ? tn / to : 0.0;
   }
   const double* __restrict__ z = a.z;
   auto pv = [&](size_t m) { return first ? z[m] : z[m] + beta * p_in[m]; };
@@ -772,10 +788,9 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_o
 // alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
 __global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_old, const double* p) {
   if (*a.done) return;
-  double ta[3], tb[3];
-  total3(rz_old, a.nblocks, ta);
-  total3(a.part_pq, a.nblocks, tb);
-  const double alpha = (tb[0] != 0.0) ? ta[0] / tb[0] : 0.0;
+  double ta, tb;
+  total_first2(rz_old, a.part_pq, a.nblocks, ta, tb);
+  const double alpha = (tb != 0.0) ? ta / tb : 0.0;
   double rr = 0;
   GRID_LOOP(n, a.n) {
     a.x[n] += alpha * p[n];
