@@ -136,7 +136,7 @@ struct PcgArgs {
   const double *diag, *cx, *cy, *cz, *b;
   const uint8_t* kind;
   double *x, *r, *p, *p2, *q, *z;  // p and p2: search direction of even and odd iterations
-  double *part_a, *part_pq, *part_rr, *part_rz0, *part_rz1;  // [3*nblocks] block partial sums
+  double *part_a, *part_pq, *part_rr, *part_rz0, *part_rz1;  // block partial sums: part_a [3*nblocks], the others [nblocks]
   double* scal;  // [0] r.r [1] r.z [2] b.b [3] iterations done [4] r.r at the tolerance
   int* done;     // set once r.r reaches scal[4]: every later CG / V-cycle kernel returns at once
   double rtol;

@@ -675,6 +675,36 @@ __device__ inline void block_reduce3(double s0, double s1, double s2, double* pa
   }
 }
 
+// Block partial sum of one value into part[block] (the order of block_reduce3's
+// first component).
+__device__ inline void block_reduce1(double s0, double* part) {
+  __shared__ double sh[4];
+  s0 = wave_sum(s0);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = s0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a0 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) a0 += sh[q];
+    part[blockIdx.x] = a0;
+  }
+}
+
+// Total of one-value partials (block_reduce1) in total3's order.
+__device__ inline double total1(const double* part, int nb) {
+  __shared__ double sh[4];
+  double a0 = 0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) a0 += part[q];
+  a0 = wave_sum(a0);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = a0;
+  __syncthreads();
+  double t = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sh[q];
+  __syncthreads();
+  return t;
+}
+
 // Totals of the per-block partials in a fixed order; every block computes them.
 __device__ inline void total3(const double* part, int nb, double* out) {
   __shared__ double sh[3][4];
@@ -691,13 +721,12 @@ __device__ inline void total3(const double* part, int nb, double* out) {
   __syncthreads();
 }
 
-// First components of the totals of two partial arrays, each summed in the same
-// order as total3 sums it (the CG needs only r.z and p.q of its partials, so the
-// other two components are not read).
+// Totals of two one-value partial arrays (block_reduce1), each summed in total3's
+// order, in one pass.
 __device__ inline void total_first2(const double* pa, const double* pb, int nb, double& ta, double& tb) {
   __shared__ double sh[2][4];
   double a0 = 0, b0 = 0;
-  for (int q = threadIdx.x; q < nb; q += blockDim.x) { a0 += pa[3 * q]; b0 += pb[3 * q]; }
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) { a0 += pa[q]; b0 += pb[q]; }
   a0 = wave_sum(a0);
   b0 = wave_sum(b0);
   const int w = threadIdx.x >> 6;
@@ -740,11 +769,11 @@ __global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a) {
 // scal = (r.r, r.z, b.b, 0, rtol^2 b.b); done when the start meets the tolerance
 // (one block)
 __global__ __launch_bounds__(256) void k_pcg_begin(PcgArgs a) {
-  double ta[3], tz[3];
+  double ta[3];
   total3(a.part_a, a.nblocks, ta);
-  total3(a.part_rz0, a.nblocks, tz);
+  const double tz = total1(a.part_rz0, a.nblocks);
   if (threadIdx.x == 0) {
-    a.scal[0] = ta[0]; a.scal[1] = tz[0]; a.scal[2] = ta[1];
+    a.scal[0] = ta[0]; a.scal[1] = tz; a.scal[2] = ta[1];
     a.scal[4] = a.rtol * a.rtol * ta[1];
     if (ta[0] <= a.scal[4]) *a.done = 1;
   }
@@ -782,7 +811,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_o
     a.q[n] = q;
     pq += pn * q;
   }
-  block_reduce3(pq, 0.0, 0.0, a.part_pq);
+  block_reduce1(pq, a.part_pq);
 }
 
 // alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
@@ -798,7 +827,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_
     a.r[n] = r;
     rr += r * r;
   }
-  block_reduce3(rr, 0.0, 0.0, a.part_rr);
+  block_reduce1(rr, a.part_rr);
 }
 
 // ------------------------------------------------------------------ multigrid V-cycle
@@ -1030,12 +1059,11 @@ __device__ __host__ inline const double* mg_result(const MgArgs& m, int l) { ret
 // launch still runs; every later kernel returns at once).
 __device__ inline void book_residual(const MgArgs& m) {
   if (!m.part_rr || blockIdx.x != 0) return;
-  double tr[3];
-  total3(m.part_rr, m.npart, tr);
+  const double tr = total1(m.part_rr, m.npart);
   if (threadIdx.x == 0) {
-    m.scal[0] = tr[0];
+    m.scal[0] = tr;
     m.scal[3] += 1.0;
-    if (tr[0] <= m.scal[4]) *m.done = 1;
+    if (tr <= m.scal[4]) *m.done = 1;
   }
 }
 
@@ -1089,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, double* z, doub
     z[n] = v;
     rz += L.b[n] * v;
   }
-  block_reduce3(rz, 0.0, 0.0, part_rz);
+  block_reduce1(rz, part_rz);
 }
 
 // coarsest level: x = C^-1 b, one wave per row
@@ -1110,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_mg_dot_rz(MgArgs m, const double* z, do
   const MgLevelDev& L = m.L[0];
   double rz = 0;
   for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) rz += L.b[n] * z[n];
-  block_reduce3(rz, 0.0, 0.0, part_rz);
+  block_reduce1(rz, part_rz);
 }
 
 // ------------------------------------------------------------------ initial population
