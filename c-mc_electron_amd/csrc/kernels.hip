@@ -1035,14 +1035,28 @@ __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, cons
 }
 
 // out = C^-1 in (dense, row major), one wave per row; `wave` and `nwaves` number
-// the waves taking part
+// the waves taking part.  A lane sums its columns lane, lane + 64, ... in that
+// order; the loads of eight of them are issued before the first multiply, so a
+// row costs a few memory round trips instead of one per 64 columns.
 __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out, size_t wave, size_t nwaves) {
   const int nc = m.nc;
   const int lane = threadIdx.x & 63;
+  constexpr int U = 8;
   for (size_t r = wave; r < (size_t)nc; r += nwaves) {
     const double* row = m.cinv + r * nc;
     double v = 0;
-    for (int c = lane; c < nc; c += 64) v += row[c] * in[c];
+    int c = lane;
+    for (; c + 64 * (U - 1) < nc; c += 64 * U) {
+      double rv[U], xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        rv[u] = row[c + 64 * u];
+        xv[u] = in[c + 64 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) v += rv[u] * xv[u];
+    }
+    for (; c < nc; c += 64) v += row[c] * in[c];
     v = wave_sum(v);
     if (lane == 0) out[r] = v;
   }
