@@ -45,9 +45,12 @@ def test_particle_on_a_node_deposits_there(eng):
     p.y[:] = m["ly"][p.j]
     p.z[:] = m["lz"][p.k]
     node_q, _, _, _ = oracle_lib.deposit(eng, p, 3)
+    expect = np.zeros_like(node_q)
     for q in range(3):
         n = (p.j[q] * eng.mesh.pnx + p.i[q]) * eng.mesh.pnz + p.k[q]
-        assert node_q[n] == np.rint(p.charge[q] * FXQ) + (node_q[n] - np.rint(p.charge[q] * FXQ)) * 0 or True
+        expect[n] += np.int64(np.rint(p.charge[q] * FXQ))
+    # each particle's whole charge lands on its own node and nowhere else
+    assert np.array_equal(node_q, expect)
     nz = np.nonzero(node_q)[0]
     assert len(nz) <= 3
     assert node_q.sum() == np.rint(p.charge[:3] * FXQ).sum()
