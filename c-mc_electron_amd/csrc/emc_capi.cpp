@@ -19,6 +19,7 @@
 #include "fly_core.h"
 #include "kernels.h"
 #include "multigrid.h"
+#include "noalloc.h"
 #include "output.h"
 #include "qc.h"
 #include "tetindex.h"
@@ -840,6 +841,10 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
   if (c->opts.device >= 0) {
     int st = gpu_setup(c);
     if (st) return done(st);
+    // the scratch array of emc_poisson_solve_noalloc: its device address, once
+    // per device, with the device current
+    st = emc_noalloc_setup(c->gpu->device);
+    if (st) return done(fail(c, st, "emc_poisson_solve_noalloc: the scratch array's device address is not available"));
   }
   return done(EMC_OK);
 }
@@ -1293,13 +1298,23 @@ static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
 // With `defer` the solve is only queued past its last check: the result (iteration
 // count, residual) is read by finish_pcg() once the stream has passed it.
 static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres);
+static int run_pcg_with(emc_ctx* c, const PcgArgs& a, const MgArgs& mg, double rtol, int maxit, int* iters,
+                        double* relres, bool defer);
 static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres,
                    bool defer = false) {
   Gpu& g = *c->gpu;
-  PcgArgs a = pcg_args(c, b, x, rtol);
+  return run_pcg_with(c, pcg_args(c, b, x, rtol), g.mg, rtol, maxit, iters, relres, defer);
+}
+
+// The solve on explicit work vectors: `a` and `mg` name every buffer the
+// kernels write (the solve of the context uses its own, emc_poisson_solve_ws a
+// caller's workspace).
+static int run_pcg_with(emc_ctx* c, const PcgArgs& a, const MgArgs& mg, double rtol, int maxit, int* iters,
+                        double* relres, bool defer) {
+  Gpu& g = *c->gpu;
   // k_pcg_begin sets done when the start already meets the tolerance: the
   // queued iterations are then no-ops and the first check below stops the loop
-  HIPCHECK(c, emc_launch_pcg_start(a, g.mg, g.stream));
+  HIPCHECK(c, emc_launch_pcg_start(a, mg, g.stream));
   double* sh = g.scal_host;
   // Each check reads the residual of the iteration before the one just queued, so
   // the GPU never waits for the host; the iteration the previous solve converged in
@@ -1309,9 +1324,9 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
   int checked = -1, last = -1;
   for (int k = 0; k < maxit; ++k) {
     last = k;
-    HIPCHECK(c, emc_launch_pcg_iter(a, g.mg, k, g.stream));
+    HIPCHECK(c, emc_launch_pcg_iter(a, mg, k, g.stream));
     double* slot = sh + 8 * (k & 1);
-    HIPCHECK(c, hipMemcpyAsync(slot, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(slot, a.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
     HIPCHECK(c, hipEventRecord(g.scal_ready[k & 1], g.stream));
     const int chk = (k + 1 == guess) ? k : k - 1;
     if (chk < 0 || chk <= checked) continue;
@@ -1323,7 +1338,7 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
   }
   // the kernels queued after convergence leave scal as it was, so the copy of the
   // last queued iteration holds the result (read once the stream has passed it)
-  if (last < 0) HIPCHECK(c, hipMemcpyAsync(sh + 16, g.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+  if (last < 0) HIPCHECK(c, hipMemcpyAsync(sh + 16, a.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
   g.pcg_result = last < 0 ? sh + 16 : sh + 8 * (last & 1);
   g.pcg_pending = true;
   if (defer) return EMC_OK;
@@ -1849,6 +1864,63 @@ int emc_poisson_solve(emc_ctx* c, const double* rhs, double* sol, double rtol, i
   HIPCHECK(c, hipMemcpy(sol, g.pot, NN * 8, hipMemcpyDeviceToHost));
   return st;
 }
+
+}  // extern "C"
+
+// ---- the solve on a caller's workspace (emc_poisson_solve_noalloc)
+// Layout, in doubles: b and x (the staged right-hand side and solution), the CG
+// vectors r p p2 q z, five block-partial arrays of 3*nblocks, scal (8), the
+// done flag (one slot), then per multigrid level x t r (level 0) or x t r b.
+static size_t align32(size_t n) { return (n + 31) & ~(size_t)31; }  // 256-byte steps
+
+size_t emc_poisson_ws_doubles(emc_ctx* c) {
+  if (!c || !c->gpu) return 0;
+  const Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  size_t need = 7 * align32(NN) + 5 * align32(3 * (size_t)g.pcg_blocks) + align32(8) + align32(1);
+  for (int l = 0; l < g.mg.nl; ++l) need += (l == 0 ? 3 : 4) * align32(g.mg.L[l].n);
+  return need;
+}
+
+int emc_ctx_device(emc_ctx* c) { return (c && c->gpu) ? c->gpu->device : -1; }
+
+int emc_poisson_solve_ws(emc_ctx* c, const double* rhs, double* sol, double rtol, int maxit, int* iters,
+                         double* relres, double* ws, size_t ws_doubles) {
+  if (!c || !rhs || !sol || !ws) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t need = emc_poisson_ws_doubles(c);
+  if (need > ws_doubles)
+    return fail(c, EMC_ERR_CAPACITY, "Poisson workspace: the mesh needs " + std::to_string(need * 8) +
+                                         " bytes, the workspace holds " + std::to_string(ws_doubles * 8));
+  const size_t NN = c->dev.nnode;
+  size_t off = 0;
+  auto take = [&](size_t n) { double* p = ws + off; off += align32(n); return p; };
+  rtol = rtol > 0 ? rtol : c->opts.poisson_rtol;
+  maxit = maxit > 0 ? maxit : c->opts.poisson_maxit;
+  double* b = take(NN);
+  double* x = take(NN);
+  PcgArgs a = pcg_args(c, b, x, rtol);
+  a.r = take(NN); a.p = take(NN); a.p2 = take(NN); a.q = take(NN); a.z = take(NN);
+  for (double** pp : {&a.part_a, &a.part_pq, &a.part_rr, &a.part_rz0, &a.part_rz1}) *pp = take(3 * (size_t)g.pcg_blocks);
+  a.scal = take(8);
+  a.done = (int*)take(1);
+  MgArgs mg = g.mg;
+  mg.done = a.done;
+  for (int l = 0; l < mg.nl; ++l) {
+    MgLevelDev& D = mg.L[l];
+    D.x = take(D.n); D.t = take(D.n); D.r = take(D.n);
+    D.b = l > 0 ? take(D.n) : a.r;
+  }
+  HIPCHECK(c, hipMemcpyAsync(b, rhs, NN * 8, hipMemcpyHostToDevice, g.stream));
+  HIPCHECK(c, hipMemcpyAsync(x, sol, NN * 8, hipMemcpyHostToDevice, g.stream));
+  int st = run_pcg_with(c, a, mg, rtol, maxit, iters, relres, false);
+  HIPCHECK(c, hipMemcpyAsync(sol, x, NN * 8, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  return st;
+}
+
+extern "C" {
 
 int emc_poisson_mg_level(emc_ctx* c, int level, int dims[3], double* a27, double* omega) {
   if (!c || !dims) return EMC_ERR_ARG;

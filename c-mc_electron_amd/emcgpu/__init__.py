@@ -242,6 +242,17 @@ class Engine:
                                                C.byref(it), C.byref(rr)))
         return sol, it.value, rr.value
 
+    def poisson_solve_noalloc(self, rhs, x0=None, rtol=1e-10, maxit=20000):
+        """poisson_solve through emc_poisson_solve_noalloc (work vectors in the
+        library's static device array; nothing allocated per call)."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        sol = np.zeros(self.mesh.nnode) if x0 is None else np.array(x0, np.float64)
+        it = C.c_int(0)
+        rr = C.c_double(0)
+        self._check(self.lib.emc_poisson_solve_noalloc(self.ctx, _ptr(rhs, C.c_double), _ptr(sol, C.c_double), rtol,
+                                                       maxit, C.byref(it), C.byref(rr)))
+        return sol, it.value, rr.value
+
     def tet_index(self):
         """The point-location octree: (roots, nodes, entries) numpy copies, or None."""
         b = self.band

@@ -160,6 +160,7 @@ def declare(lib):
         "emc_fly_step": (C.c_int, [ctx, C.c_uint64, C.c_int, P(FlyTallies)]),
         "emc_deposit": (C.c_int, [ctx, _i64, _i64, _i64, _i64]),
         "emc_poisson_solve": (C.c_int, [ctx, _d, _d, C.c_double, C.c_int, _i, _d]),
+        "emc_poisson_solve_noalloc": (C.c_int, [ctx, _d, _d, C.c_double, C.c_int, _i, _d]),
         "emc_poisson_mg_level": (C.c_int, [ctx, C.c_int, _i, _d, _d]),
         "emc_compute_field": (C.c_int, [ctx, _d, _d]),
         "emc_step": (C.c_int, [ctx, C.c_int, P(FlyTallies)]),

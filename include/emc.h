@@ -284,6 +284,19 @@ int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell
 int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
                       int *iters, double *relres);
 
+/* emc_poisson_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * solve's work vectors (staged rhs and sol, the CG vectors and partial sums,
+ * the multigrid work vectors) live in one static 80 MiB device array per GPU,
+ * held until the process ends: meshes up to about 980 k nodes (10.6 doubles
+ * per node); a larger mesh returns EMC_ERR_CAPACITY.  Same arguments, results
+ * and iteration counts as emc_poisson_solve.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race, so the caller serialises
+ * them. */
+int emc_poisson_solve_noalloc(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
+                              int *iters, double *relres);
+
 /* Level `level` of the multigrid hierarchy used by emc_poisson_solve (no reference
  * counterpart: the reference factorises with KLU).  dims[3] = nodes along i, j, k;
  * a27 (may be NULL; levels >= 1) receives the [27][n] Galerkin coefficients, entry
