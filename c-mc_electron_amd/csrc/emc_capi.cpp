@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <thread>
@@ -25,6 +27,8 @@
 #include "tetindex.h"
 
 using namespace emc;
+
+struct DevBand;
 
 namespace {
 
@@ -81,6 +85,7 @@ struct Gpu {
   int device = -1;
   hipStream_t stream = nullptr;
   DevBuf mem;
+  std::shared_ptr<DevBand> band_dev;  // the band tables on this GPU (shared)
   BandDev B{};
   MeshDev M{};
   FieldsDev F{};
@@ -166,18 +171,26 @@ struct RestartCells {
   std::vector<uint8_t> have;
 };
 
-struct emc_ctx {
-  std::string err;
-  emc_opts opts{};
-  Device dev;
+// The full-band tables and the records built from them: they depend only on
+// the table files, the temperature and (quantum correction) the dispersive
+// phonon files, so contexts of one process that agree on those share one copy,
+// on the host and, per GPU, in device memory.
+struct BandBundle {
   Band band;
   std::vector<TetRec> tetrec;
   std::vector<SurfRec> surfrec;
   std::vector<CornerRec> cornerrec;
   std::vector<Sphere> tsph, tlist_s, tlists_s, slist_s;
-  std::vector<int> mr_desired_e, mr_desired_h;  // [ncell] MultipleRefresh target counts
   TetIndex tix;         // empty when build_tet_index refused the tables
   std::string tix_why;  // why it refused
+};
+
+struct emc_ctx {
+  std::string err;
+  emc_opts opts{};
+  Device dev;
+  std::shared_ptr<const BandBundle> bb;
+  std::vector<int> mr_desired_e, mr_desired_h;  // [ncell] MultipleRefresh target counts
   std::vector<int> surface_dir;
   HostParticles hp;
   std::unique_ptr<Gpu> gpu;
@@ -219,15 +232,15 @@ static int fail(emc_ctx* c, int code, const std::string& msg) {
 
 // ---------------------------------------------------------------- host band view
 static void fill_band_dev_host(emc_ctx* c, BandDev& B) {
-  const Band& b = c->band;
-  B.tr = c->tetrec.data();
-  B.sr = c->surfrec.data();
-  B.cr = c->cornerrec.data();
-  B.tsph = c->tsph.data();
-  B.troot = c->tix.nodes.empty() ? nullptr : c->tix.roots.data();
-  B.tnode = c->tix.nodes.empty() ? nullptr : c->tix.nodes.data();
-  B.tent = c->tix.nodes.empty() ? nullptr : c->tix.ent.data();
-  B.tlist_s = c->tlist_s.data(); B.tlists_s = c->tlists_s.data(); B.slist_s = c->slist_s.data();
+  const Band& b = c->bb->band;
+  B.tr = c->bb->tetrec.data();
+  B.sr = c->bb->surfrec.data();
+  B.cr = c->bb->cornerrec.data();
+  B.tsph = c->bb->tsph.data();
+  B.troot = c->bb->tix.nodes.empty() ? nullptr : c->bb->tix.roots.data();
+  B.tnode = c->bb->tix.nodes.empty() ? nullptr : c->bb->tix.nodes.data();
+  B.tent = c->bb->tix.nodes.empty() ? nullptr : c->bb->tix.ent.data();
+  B.tlist_s = c->bb->tlist_s.data(); B.tlists_s = c->bb->tlists_s.data(); B.slist_s = c->bb->slist_s.data();
   B.tet = b.tet.data();
   B.xkk = b.xkk.data(); B.ykk = b.ykk.data(); B.zkk = b.zkk.data(); B.eek = b.eek.data();
   B.vgt = b.vgt.data(); B.faclow = b.faclow.data(); B.facmed = b.facmedium.data(); B.fachigh = b.fachigh.data();
@@ -263,11 +276,11 @@ static void fill_mesh_dev_scalars(emc_ctx* c, MeshDev& M) {
   for (int q = 0; q < 3; ++q) M.charge_sign[q] = d.phys.charge_sign[q];
 }
 
-static void build_tetrec(emc_ctx* c) {
-  const Band& b = c->band;
-  c->tetrec.assign(b.nt, TetRec{});
+static void build_tetrec(BandBundle& bb) {
+  const Band& b = bb.band;
+  bb.tetrec.assign(b.nt, TetRec{});
   for (int it = 0; it < b.nt; ++it) {
-    TetRec& r = c->tetrec[it];
+    TetRec& r = bb.tetrec[it];
     for (int f = 0; f < 4; ++f)
       for (int q = 0; q < 4; ++q) r.n[f][q] = b.dn(q, f, it);
     for (int q = 0; q < 4; ++q) r.vt[q] = b.vt[4 * (size_t)it + q];
@@ -276,11 +289,11 @@ static void build_tetrec(emc_ctx* c) {
     for (int f = 0; f < 4; ++f) r.neib[f] = b.neib[(size_t)f * b.nt + it];
     r.ibt = b.ibt[it];
   }
-  c->surfrec.assign(b.nt, SurfRec{});
-  c->cornerrec.assign(b.nt, CornerRec{});
+  bb.surfrec.assign(b.nt, SurfRec{});
+  bb.cornerrec.assign(b.nt, CornerRec{});
   for (int it = 0; it < b.nt; ++it) {
-    SurfRec& s = c->surfrec[it];
-    CornerRec& q = c->cornerrec[it];
+    SurfRec& s = bb.surfrec[it];
+    CornerRec& q = bb.cornerrec[it];
     for (int k = 0; k < 4; ++k) {
       const int ck = b.tet_corner(k, it);
       s.e[k] = q.e[k] = b.eek[ck];
@@ -295,11 +308,11 @@ static void build_tetrec(emc_ctx* c) {
   // tetrahedron gets an infinite radius (always tested exactly).
   // The box of the grown corners, widened the same way, bounds the tetrahedron for
   // the point-location index.
-  c->tsph.assign(b.nt, Sphere{});
+  bb.tsph.assign(b.nt, Sphere{});
   std::vector<TetBox> tbox(b.nt);
   for (int it = 0; it < b.nt; ++it) {
     double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    const CornerRec& q = c->cornerrec[it];
+    const CornerRec& q = bb.cornerrec[it];
     const double cx = 0.25 * (q.x[0] + q.x[1] + q.x[2] + q.x[3]);
     const double cy = 0.25 * (q.y[0] + q.y[1] + q.y[2] + q.y[3]);
     const double cz = 0.25 * (q.z[0] + q.z[1] + q.z[2] + q.z[3]);
@@ -337,7 +350,7 @@ static void build_tetrec(emc_ctx* c) {
       }
     }
     const double r = ok ? std::sqrt(r2max) * (1.0 + 1e-9) + 1e-12 : HUGE_VAL;
-    c->tsph[it] = Sphere{cx, cy, cz, ok ? r * r : HUGE_VAL};
+    bb.tsph[it] = Sphere{cx, cy, cz, ok ? r * r : HUGE_VAL};
     for (int a = 0; a < 3; ++a) {
       const double w = 1e-9 * (std::fabs(blo[a]) + std::fabs(bhi[a]) + (bhi[a] - blo[a])) + 1e-12;
       tbox[it].lo[a] = ok ? blo[a] - w : -HUGE_VAL;
@@ -346,16 +359,44 @@ static void build_tetrec(emc_ctx* c) {
   }
   auto along = [&](const std::vector<int>& list, std::vector<Sphere>& out) {
     out.resize(list.size());
-    for (size_t q = 0; q < list.size(); ++q) out[q] = c->tsph[list[q]];
+    for (size_t q = 0; q < list.size(); ++q) out[q] = bb.tsph[list[q]];
   };
-  along(b.tlist, c->tlist_s);
-  along(b.tlists, c->tlists_s);
-  along(b.slist, c->slist_s);
-  c->tix_why = build_tet_index(b, tbox, c->tix);
-  if (!c->tix_why.empty()) c->tix = TetIndex{};
+  along(b.tlist, bb.tlist_s);
+  along(b.tlists, bb.tlists_s);
+  along(b.slist, bb.slist_s);
+  bb.tix_why = build_tet_index(b, tbox, bb.tix);
+  if (!bb.tix_why.empty()) bb.tix = TetIndex{};
 }
 
 // Multigrid hierarchy of the Poisson preconditioner, built once per context.
+// The band bundle for these tables and this temperature: built once per process
+// and shared by every context that asks for the same one while any holds it.
+static std::string band_bundle(const Device& d, const std::string& tdir, const std::string files[PTAB_N],
+                               std::shared_ptr<const BandBundle>& out) {
+  static std::mutex mu;
+  static std::map<std::string, std::weak_ptr<const BandBundle>> cache;
+  char tk[64];
+  std::snprintf(tk, sizeof(tk), "%.17g", d.phys.T0);
+  std::string key = tdir + "|" + tk;
+  if (d.cfg.flag_qc)
+    for (int t = 0; t < PTAB_N; ++t) key += "|" + files[t];
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end())
+    if ((out = it->second.lock())) return "";
+  auto bb = std::make_shared<BandBundle>();
+  std::string err = band_build(bb->band, d.phys, tdir);
+  if (!err.empty()) return err;
+  if (d.cfg.flag_qc) {
+    err = load_phonon_tables(bb->band, tdir, files);
+    if (!err.empty()) return err;
+  }
+  build_tetrec(*bb);
+  cache[key] = bb;
+  out = bb;
+  return "";
+}
+
 static std::string ensure_mg(emc_ctx* c) {
   if (!c->mg.lv.empty()) return "";
   const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -369,36 +410,48 @@ static std::string ensure_mg(emc_ctx* c) {
 }
 
 // ---------------------------------------------------------------- GPU setup
-static int gpu_setup(emc_ctx* c) {
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->opts.device)
-    return fail(c, EMC_ERR_NOGPU, "no HIP device available for the requested ordinal");
-  c->gpu.reset(new Gpu());
-  Gpu& g = *c->gpu;
-  g.device = c->opts.device;
-  HIPCHECK(c, hipSetDevice(g.device));
-  HIPCHECK(c, hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-  {
-    int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g.device) == hipSuccess && ncu > 0)
-      g.wave_slots = 4LL * ncu;
+// The device copy of a band bundle on one GPU, shared by the contexts on that
+// GPU that use the bundle; freed with the last of them.
+struct DevBand {
+  int device = -1;
+  BandDev B{};
+  DevBuf mem;
+  std::shared_ptr<const BandBundle> bb;  // keeps the host tables the key points at
+  ~DevBand() {
+    int prev = -1;
+    if (hipGetDevice(&prev) == hipSuccess && prev != device) (void)hipSetDevice(device);
+    mem.free_all();
+    if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
   }
-  for (auto& e : g.ev) HIPCHECK(c, hipEventCreate(&e));
-  const Band& b = c->band;
-  const Device& d = c->dev;
-  DevBuf& m = g.mem;
-  BandDev& B = g.B;
+};
+
+static std::string device_band(emc_ctx* c, std::shared_ptr<DevBand>& out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const BandBundle*>, std::weak_ptr<DevBand>> cache;
+  const int dev = c->opts.device;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_pair(dev, c->bb.get());
+  auto it = cache.find(key);
+  if (it != cache.end())
+    if ((out = it->second.lock())) return "";
+  auto db = std::make_shared<DevBand>();
+  db->device = dev;
+  db->bb = c->bb;
+  const BandBundle& bb = *c->bb;
+  const Band& b = bb.band;
+  DevBuf& m = db->mem;
+  BandDev& B = db->B;
   fill_band_dev_host(c, B);  // scalars; pointers replaced below
-  B.tr = m.dup(c->tetrec);
-  B.sr = m.dup(c->surfrec);
-  B.cr = m.dup(c->cornerrec);
-  B.tsph = m.dup(c->tsph);
-  if (!c->tix.nodes.empty()) {
-    B.troot = m.dup(c->tix.roots);
-    B.tnode = m.dup(c->tix.nodes);
-    B.tent = m.dup(c->tix.ent);
+  B.tr = m.dup(bb.tetrec);
+  B.sr = m.dup(bb.surfrec);
+  B.cr = m.dup(bb.cornerrec);
+  B.tsph = m.dup(bb.tsph);
+  if (!bb.tix.nodes.empty()) {
+    B.troot = m.dup(bb.tix.roots);
+    B.tnode = m.dup(bb.tix.nodes);
+    B.tent = m.dup(bb.tix.ent);
   }
-  B.tlist_s = m.dup(c->tlist_s); B.tlists_s = m.dup(c->tlists_s); B.slist_s = m.dup(c->slist_s);
+  B.tlist_s = m.dup(bb.tlist_s); B.tlists_s = m.dup(bb.tlists_s); B.slist_s = m.dup(bb.slist_s);
   B.tet = m.dup(b.tet);
   B.xkk = m.dup(b.xkk); B.ykk = m.dup(b.ykk); B.zkk = m.dup(b.zkk); B.eek = m.dup(b.eek);
   B.vgt = m.dup(b.vgt); B.faclow = m.dup(b.faclow); B.facmed = m.dup(b.facmedium); B.fachigh = m.dup(b.fachigh);
@@ -414,6 +467,34 @@ static int gpu_setup(emc_ctx* c) {
   for (int t = 0; t < PTAB_N; ++t) B.ptab[t] = b.ptab_t[t].empty() ? nullptr : m.dup(b.ptab_t[t]);
   B.scatte = m.dup(b.scatte); B.dose = m.dup(b.dose); B.scattiie = m.dup(b.scattiie);
   B.scatth = m.dup(b.scatth); B.dosh = m.dup(b.dosh); B.scattiih = m.dup(b.scattiih);
+  if (m.last != hipSuccess) return std::string("band tables: ") + hipGetErrorString(m.last);
+  cache[key] = db;
+  out = db;
+  return "";
+}
+
+static int gpu_setup(emc_ctx* c) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->opts.device)
+    return fail(c, EMC_ERR_NOGPU, "no HIP device available for the requested ordinal");
+  c->gpu.reset(new Gpu());
+  Gpu& g = *c->gpu;
+  g.device = c->opts.device;
+  HIPCHECK(c, hipSetDevice(g.device));
+  HIPCHECK(c, hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g.device) == hipSuccess && ncu > 0)
+      g.wave_slots = 4LL * ncu;
+  }
+  for (auto& e : g.ev) HIPCHECK(c, hipEventCreate(&e));
+  const Device& d = c->dev;
+  DevBuf& m = g.mem;
+  {
+    const std::string e = device_band(c, g.band_dev);
+    if (!e.empty()) return fail(c, EMC_ERR_HIP, e);
+    g.B = g.band_dev->B;
+  }
 
   MeshDev& M = g.M;
   fill_mesh_dev_scalars(c, M);
@@ -676,7 +757,7 @@ static std::string restart_table(emc_ctx* c, const RestartCells** out) {
 
 static void sample_population(emc_ctx* c, const RestartCells* rs) {
   const Device& d = c->dev;
-  const Band& b = c->band;
+  const Band& b = c->bb->band;
   struct Spec { int i, j, k, type; double q; };
   std::vector<Spec> spec;
   for (const SpecGroup& g : spec_groups(d, rs))
@@ -777,7 +858,7 @@ static int gpu_sample_population(emc_ctx* c, const RestartCells* rs) {
   a.C = g.cst; a.P = g.P; a.n = n; a.ngroup = (long long)gr.size();
   a.start = tmp.dup(start); a.gpk = tmp.dup(gpk); a.gq = tmp.dup(gq);
   a.seed = c->opts.seed;
-  a.tmp_const = 1 - std::exp(-c->band.emax);
+  a.tmp_const = 1 - std::exp(-c->bb->band.emax);
   if (tmp.last != hipSuccess) {
     tmp.free_all();
     return fail(c, EMC_ERR_HIP, std::string("sampling tables: ") + hipGetErrorString(tmp.last));
@@ -818,9 +899,7 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
   if (!err.empty()) return done(fail(c, err.find("cannot open") != std::string::npos ? EMC_ERR_IO : EMC_ERR_DECK, err));
   std::string tdir = c->opts.table_dir ? c->opts.table_dir : "";
   if (tdir.empty()) return done(fail(c, EMC_ERR_ARG, "table_dir is required"));
-  err = band_build(c->band, c->dev.phys, tdir);
-  if (!err.empty()) return done(fail(c, EMC_ERR_IO, err));
-  if (c->dev.cfg.flag_qc) {
+  {
     // EPSCAT takes optical and f-phonon final energies from the dispersive
     // tables in quantum-correction runs (src/ElectronPhononScatter.cpp:148-163)
     std::string files[PTAB_N] = {"kloem.txt", "kloab.txt", "ktoem.txt", "ktoab.txt", "klaem.txt", "klaab.txt"};
@@ -829,14 +908,13 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
       auto it = c->dev.cfg.raw.find(keys[t]);
       if (it != c->dev.cfg.raw.end()) files[t] = it->second;
     }
-    err = load_phonon_tables(c->band, tdir, files);
+    err = band_bundle(c->dev, tdir, files, c->bb);
     if (!err.empty()) return done(fail(c, EMC_ERR_IO, err));
   }
-  err = device_finish(c->dev, c->band);
+  err = device_finish(c->dev, c->bb->band);
   if (!err.empty()) return done(fail(c, EMC_ERR_DECK, err));
   c->surface_dir.assign(std::max(1, c->dev.nsurf), 0);
   for (int s = 0; s < c->dev.nsurf; ++s) c->surface_dir[s] = c->dev.surf_dir[s];
-  build_tetrec(c);
   mr_targets(c);
   if (c->opts.device >= 0) {
     int st = gpu_setup(c);
@@ -872,17 +950,17 @@ const char* emc_last_error(emc_ctx* c) { return c ? c->err.c_str() : "null conte
 
 int emc_band_view_get(emc_ctx* c, emc_band_view* v) {
   if (!c || !v) return EMC_ERR_ARG;
-  const Band& b = c->band;
+  const Band& b = c->bb->band;
   std::memset(v, 0, sizeof(*v));
   v->nk = b.nk; v->nt = b.nt; v->nbt = b.nbt;
   for (int q = 0; q < 3; ++q) { v->nband[q] = b.nband[q]; v->bandof[q] = b.bandof[q]; v->difpr[q] = b.difpr[q]; v->DOSMAX[q] = b.DOSMAX[q]; }
-  v->tsphere = c->tsph.empty() ? nullptr : &c->tsph[0].cx;
-  v->tindex_nodes = (int)c->tix.nodes.size();
-  v->tindex_entries = (int)c->tix.ent.size();
-  if (!c->tix.nodes.empty()) {
-    v->tindex_root = c->tix.roots.data();
-    v->tindex_node = &c->tix.nodes[0].first;
-    v->tindex_entry = c->tix.ent.data();
+  v->tsphere = c->bb->tsph.empty() ? nullptr : &c->bb->tsph[0].cx;
+  v->tindex_nodes = (int)c->bb->tix.nodes.size();
+  v->tindex_entries = (int)c->bb->tix.ent.size();
+  if (!c->bb->tix.nodes.empty()) {
+    v->tindex_root = c->bb->tix.roots.data();
+    v->tindex_node = &c->bb->tix.nodes[0].first;
+    v->tindex_entry = c->bb->tix.ent.data();
   }
   v->scpre = b.scpre; v->scprh = b.scprh;
   v->dtable = b.dtable; v->emax = b.emax; v->dlist = b.dlist; v->dlists = b.dlists; v->a0pi = b.a0pi; v->sieg = b.sieg;
@@ -911,7 +989,7 @@ int emc_band_locate(emc_ctx* c, int64_t n, const double* k, const double* energy
                     int* isym) {
   if (!c) return EMC_ERR_ARG;
   if (n < 0 || (n > 0 && (!k || !energy || !iband || !itet || !isym))) return fail(c, EMC_ERR_ARG, "band_locate: bad arguments");
-  const Band& b = c->band;
+  const Band& b = c->bb->band;
   for (int64_t q = 0; q < n; ++q)
     if (iband[q] < 0 || iband[q] >= MNB || itet[q] < 0 || itet[q] >= b.nt)
       return fail(c, EMC_ERR_ARG, "band_locate: band or tetrahedron index out of range");
@@ -1010,7 +1088,7 @@ int emc_particles_upload(emc_ctx* c, const emc_particles* h, int64_t n) {
     hp.ptype[s] = h->ptype[s]; hp.key[s] = h->key[s];
     const Device& d = c->dev;
     if (hp.i[s] >= 0 && (hp.i[s] >= d.cnx || hp.j[s] < 0 || hp.j[s] >= d.cny || hp.k[s] < 0 || hp.k[s] >= d.cnz ||
-                         hp.itet[s] < 0 || hp.itet[s] >= c->band.nt || hp.isym[s] < 0 || hp.isym[s] >= 48 ||
+                         hp.itet[s] < 0 || hp.itet[s] >= c->bb->band.nt || hp.isym[s] < 0 || hp.isym[s] >= 48 ||
                          hp.ptype[s] < 0 || hp.ptype[s] > 1))
       return fail(c, EMC_ERR_ARG, "particle " + std::to_string(s) + " has an out-of-range index");
   }
@@ -1143,7 +1221,7 @@ static int launch_fly(emc_ctx* c, uint64_t step, int flag_heat, FlyArgs& a) {
   HIPCHECK(c, hipEventRecord(g.ev[5], g.stream));
   HIPCHECK(c, emc_launch_fly(a, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[6], g.stream));
-  HIPCHECK(c, emc_launch_clones_dev(a, g.ctr, g.stream));
+  HIPCHECK(c, emc_launch_clones_dev(a, g.stream));
   HIPCHECK(c, hipMemcpyAsync(g.tally_host, g.contact, FLYOUT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              g.stream));
   return EMC_OK;
@@ -2120,6 +2198,355 @@ int emc_step_timing(emc_ctx* c, double* ms5, int* pcg_iters) {
   return EMC_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// A batch of K devices that share the mesh (the same deck geometry; contact
+// voltages, particle counts and seeds may differ), stepped together: each kernel
+// of the step is one launch for all of them (gridDim.y = K, the argument blocks
+// in device tables), so a bias sweep of small devices fills the GPU with one
+// stream of launches instead of K.  Each device's blocks run the code and grids
+// of its own launch, so every device's result equals its own emc_step bit for
+// bit.  The host waits on the GPU as one device does: the solve's residual checks
+// and once at the end of the step, for all K at once.
+struct emc_batch {
+  std::vector<emc_ctx*> c;
+  int K = 0, device = -1;
+  hipStream_t s = nullptr;
+  std::string err;
+  DevBuf mem;
+  // device argument tables: the per-step part (sort, deposit) and the flight
+  // are uploaded each step, the rest once
+  unsigned char* d_tab = nullptr;
+  unsigned char* h_tab = nullptr;     // pinned host image of d_tab
+  size_t off_sort = 0, off_dep = 0, off_pcg = 0, off_mg = 0, off_field = 0, off_fly = 0, off_cp = 0, tab_bytes = 0;
+  BatchTables bt{};
+  const CopyArgs *cp_n = nullptr, *cp_scal = nullptr, *cp_tally = nullptr, *cp_zero = nullptr;
+  unsigned long long* d_n = nullptr;   // [K] live counts after the sort
+  double* d_scal = nullptr;            // [2][K][8] PCG scalars of the last two iterations
+  unsigned long long* d_tally = nullptr;  // [K][FLYOUT_N]
+  unsigned long long* h_n = nullptr;
+  double* h_scal = nullptr;
+  unsigned long long* h_tally = nullptr;
+  hipEvent_t ev[7] = {};
+  hipEvent_t ev_scal[2] = {nullptr, nullptr};
+  hipEvent_t ev_n = nullptr;
+  MgArgs shape{};
+  int nblocks = 0, ntile = 0;
+  size_t nnode = 0, ncell = 0;
+  double timing[5] = {0, 0, 0, 0, 0};
+  int last_iters = 0;
+};
+
+static int bfail(emc_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+#define BHIP(b, expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return bfail(b, EMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+static size_t tab_align(size_t n) { return (n + 255) & ~(size_t)255; }
+
+static int batch_setup(emc_batch* b) {
+  const int K = b->K;
+  emc_ctx* c0 = b->c[0];
+  for (emc_ctx* c : b->c) {
+    if (!c || !c->gpu) return bfail(b, EMC_ERR_NOGPU, "every context of a batch needs a GPU");
+    if (c->gpu->cap == 0) return bfail(b, EMC_ERR_STATE, "a context of the batch has no particles on the device");
+    if (c->gpu->device != c0->gpu->device) return bfail(b, EMC_ERR_ARG, "the contexts of a batch share one GPU");
+    const Device &d = c->dev, &d0 = c0->dev;
+    if (d.cnx != d0.cnx || d.cny != d0.cny || d.cnz != d0.cnz)
+      return bfail(b, EMC_ERR_ARG, "the contexts of a batch share the mesh dimensions");
+    if (c->gpu->mg.nl != c0->gpu->mg.nl || c->gpu->mg.nc != c0->gpu->mg.nc)
+      return bfail(b, EMC_ERR_ARG, "the contexts of a batch need the same multigrid hierarchy shape");
+    for (int l = 0; l < c0->gpu->mg.nl; ++l)
+      if (c->gpu->mg.L[l].n != c0->gpu->mg.L[l].n)
+        return bfail(b, EMC_ERR_ARG, "the contexts of a batch need the same multigrid hierarchy shape");
+    if (d.cfg.flag_qc) return bfail(b, EMC_ERR_ARG, "batches run without the quantum correction");
+    if (c->opts.poisson_rtol != c0->opts.poisson_rtol || c->opts.poisson_maxit != c0->opts.poisson_maxit)
+      return bfail(b, EMC_ERR_ARG, "the contexts of a batch share the solver tolerance");
+    if (c->heat_phase) return bfail(b, EMC_ERR_STATE, "a context of the batch is in its heating phase");
+    for (emc_ctx* o : b->c)
+      if (o != c && o->gpu.get() == c->gpu.get()) return bfail(b, EMC_ERR_ARG, "a context appears twice");
+  }
+  for (int q = 0; q < K; ++q)
+    for (int r = 0; r < q; ++r)
+      if (b->c[q] == b->c[r]) return bfail(b, EMC_ERR_ARG, "a context appears twice in the batch");
+  b->device = c0->gpu->device;
+  BHIP(b, hipSetDevice(b->device));
+  for (emc_ctx* c : b->c) BHIP(b, hipStreamSynchronize(c->gpu->stream));
+  BHIP(b, hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+  for (auto& e : b->ev) BHIP(b, hipEventCreate(&e));
+  for (auto& e : b->ev_scal) BHIP(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  BHIP(b, hipEventCreateWithFlags(&b->ev_n, hipEventDisableTiming));
+  const Gpu& g0 = *c0->gpu;
+  b->shape = g0.mg;
+  b->nblocks = g0.pcg_blocks;
+  b->ntile = g0.ntile;
+  b->nnode = c0->dev.nnode;
+  b->ncell = c0->dev.ncell;
+  size_t off = 0;
+  auto place = [&](size_t bytes) { const size_t o = off; off += tab_align(bytes); return o; };
+  b->off_sort = place(K * sizeof(SortArgs));
+  b->off_dep = place(K * sizeof(DepositArgs));
+  b->off_fly = place(K * sizeof(FlyArgs));
+  b->off_pcg = place(K * sizeof(PcgArgs));
+  b->off_mg = place(3 * K * sizeof(MgArgs));
+  b->off_field = place(K * sizeof(FieldArgs));
+  b->off_cp = place(5 * K * sizeof(CopyArgs));
+  b->tab_bytes = off;
+  b->d_tab = b->mem.alloc<unsigned char>(off);
+  b->d_n = b->mem.alloc<unsigned long long>(K);
+  b->d_scal = b->mem.alloc<double>(2 * 8 * (size_t)K);
+  b->d_tally = b->mem.alloc<unsigned long long>((size_t)K * FLYOUT_N);
+  if (b->mem.last != hipSuccess) return bfail(b, EMC_ERR_HIP, std::string("batch tables: ") + hipGetErrorString(b->mem.last));
+  BHIP(b, hipHostMalloc((void**)&b->h_tab, off, hipHostMallocDefault));
+  BHIP(b, hipHostMalloc((void**)&b->h_n, K * sizeof(unsigned long long), hipHostMallocDefault));
+  BHIP(b, hipHostMalloc((void**)&b->h_scal, 2 * 8 * (size_t)K * sizeof(double), hipHostMallocDefault));
+  BHIP(b, hipHostMalloc((void**)&b->h_tally, (size_t)K * FLYOUT_N * sizeof(unsigned long long), hipHostMallocDefault));
+  std::memset(b->h_tab, 0, off);
+  b->bt.K = K;
+  b->bt.sort = (const SortArgs*)(b->d_tab + b->off_sort);
+  b->bt.dep = (const DepositArgs*)(b->d_tab + b->off_dep);
+  b->bt.fly = (const FlyArgs*)(b->d_tab + b->off_fly);
+  b->bt.pcg = (const PcgArgs*)(b->d_tab + b->off_pcg);
+  const MgArgs* mg = (const MgArgs*)(b->d_tab + b->off_mg);
+  b->bt.mg_start = mg;
+  b->bt.mg_even = mg + K;
+  b->bt.mg_odd = mg + 2 * K;
+  b->bt.field = (const FieldArgs*)(b->d_tab + b->off_field);
+  const CopyArgs* cp = (const CopyArgs*)(b->d_tab + b->off_cp);
+  b->cp_n = cp;
+  b->cp_scal = cp + K;  // [2][K]
+  b->cp_tally = cp + 3 * K;
+  b->cp_zero = cp + 4 * K;
+  // the tables that stay: solver, cycle, fields, readbacks
+  PcgArgs* hp = (PcgArgs*)(b->h_tab + b->off_pcg);
+  MgArgs* hm = (MgArgs*)(b->h_tab + b->off_mg);
+  FieldArgs* hf = (FieldArgs*)(b->h_tab + b->off_field);
+  CopyArgs* hc = (CopyArgs*)(b->h_tab + b->off_cp);
+  for (int q = 0; q < K; ++q) {
+    emc_ctx* c = b->c[q];
+    Gpu& g = *c->gpu;
+    hp[q] = pcg_args(c, g.rhs, g.pot, c->opts.poisson_rtol);
+    hm[q] = emc_mg_start_args(hp[q], g.mg);
+    hm[K + q] = emc_mg_iter_args(hp[q], g.mg, 0);
+    hm[2 * K + q] = emc_mg_iter_args(hp[q], g.mg, 1);
+    hf[q] = field_args(c, g.pot, true);
+    int st = sync_fly_const(c);
+    if (st) return bfail(b, st, c->err);
+    hc[q] = CopyArgs{g.toff + g.ntile, b->d_n + q, 1, 1, 0};  // the live count, a 32-bit word
+    hc[K + q] = CopyArgs{g.scal, (unsigned long long*)(b->d_scal + 8 * (size_t)q), 8, 0, 0};
+    hc[2 * K + q] = CopyArgs{g.scal, (unsigned long long*)(b->d_scal + 8 * ((size_t)K + q)), 8, 0, 0};
+    hc[3 * K + q] = CopyArgs{g.contact, b->d_tally + (size_t)q * FLYOUT_N, FLYOUT_N, 0, 0};
+    hc[4 * K + q] = CopyArgs{nullptr, g.contact, FLYOUT_N, 0, 0};
+  }
+  BHIP(b, hipMemcpyAsync(b->d_tab + b->off_pcg, b->h_tab + b->off_pcg, b->tab_bytes - b->off_pcg,
+                         hipMemcpyHostToDevice, b->s));
+  BHIP(b, hipStreamSynchronize(b->s));
+  return EMC_OK;
+}
+
+// One step of every device of the batch: one_step's phases, each one launch for all.
+static int batch_one_step(emc_batch* b, emc_fly_tallies* acc, double tsum[5], int* iters_max) {
+  const int K = b->K;
+  hipStream_t s = b->s;
+  std::vector<uint64_t> step(K);
+  SortArgs* hs = (SortArgs*)(b->h_tab + b->off_sort);
+  DepositArgs* hd = (DepositArgs*)(b->h_tab + b->off_dep);
+  FlyArgs* hfl = (FlyArgs*)(b->h_tab + b->off_fly);
+  long long nmax = 0;
+  int guess = 0;
+  for (int q = 0; q < K; ++q) {
+    emc_ctx* c = b->c[q];
+    Gpu& g = *c->gpu;
+    step[q] = c->next_step++;
+    int st = settle_n(c);
+    if (st) return bfail(b, st, c->err);
+    SortArgs& a = hs[q];
+    a = SortArgs{};
+    a.P = g.P; a.Q = g.P2; a.n = g.n;
+    a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
+    a.tcount = g.tcount; a.toff = g.toff; a.tcursor = g.tcursor;
+    nmax = std::max(nmax, g.n);
+    std::swap(g.P, g.P2);
+    hd[q] = deposit_args(c, 1);
+    hd[q].st = stat_args(c, false);
+    hd[q].rhs = RhsArgs{b->nnode, g.node_q, g.pvol, g.pdop, g.dirichlet, g.corr, g.kind, g.rhs, nullptr};
+    guess = std::max(guess, c->last_pcg_iters);
+  }
+  BHIP(b, hipEventRecord(b->ev[0], s));
+  BHIP(b, hipMemcpyAsync(b->d_tab + b->off_sort, b->h_tab + b->off_sort, b->off_fly - b->off_sort,
+                         hipMemcpyHostToDevice, s));
+  BHIP(b, emc_launch_tile_sort_b(b->bt, nmax, s));
+  // the live counts (32-bit toff[ntile] of each device) into d_n
+  BHIP(b, emc_launch_copy_b(b->cp_n, K, 1, s));
+  BHIP(b, hipMemcpyAsync(b->h_n, b->d_n, K * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  BHIP(b, hipEventRecord(b->ev_n, s));
+  BHIP(b, emc_launch_deposit_b(b->bt, b->ntile, b->nnode, s));
+  BHIP(b, hipEventRecord(b->ev[1], s));
+  // the solve: as run_pcg, one iteration of every device per launch; the host
+  // checks the iteration before the one just queued, for all devices at once
+  BHIP(b, emc_launch_pcg_start_b(b->bt, b->shape, b->nblocks, s));
+  const int maxit = b->c[0]->opts.poisson_maxit;
+  int checked = -1, last = -1;
+  for (int k = 0; k < maxit; ++k) {
+    last = k;
+    BHIP(b, emc_launch_pcg_iter_b(b->bt, b->shape, b->nblocks, k, s));
+    BHIP(b, emc_launch_copy_b(b->cp_scal + (size_t)(k & 1) * K, K, 8, s));
+    BHIP(b, hipMemcpyAsync(b->h_scal + (size_t)(k & 1) * 8 * K, b->d_scal + (size_t)(k & 1) * 8 * K,
+                           8 * K * sizeof(double), hipMemcpyDeviceToHost, s));
+    BHIP(b, hipEventRecord(b->ev_scal[k & 1], s));
+    const int chk = (k + 1 == guess) ? k : k - 1;
+    if (chk < 0 || chk <= checked) continue;
+    BHIP(b, hipEventSynchronize(b->ev_scal[chk & 1]));
+    checked = chk;
+    bool all = true;
+    for (int q = 0; q < K; ++q) {
+      const double* res = b->h_scal + (size_t)(chk & 1) * 8 * K + 8 * q;
+      if (!std::isfinite(res[0])) return bfail(b, EMC_ERR_SOLVER, "PCG residual is not finite");
+      if (!(res[0] <= res[4])) all = false;
+    }
+    if (all) break;
+  }
+  BHIP(b, hipEventRecord(b->ev[2], s));
+  BHIP(b, emc_launch_field_b(b->bt, b->ncell, s));
+  BHIP(b, hipEventRecord(b->ev[3], s));
+  BHIP(b, emc_launch_copy_b(b->cp_zero, K, FLYOUT_N, s));
+  // the flight needs the live counts of the sort (long since read: the residual
+  // checks waited on later work)
+  BHIP(b, hipEventSynchronize(b->ev_n));
+  nmax = 0;
+  for (int q = 0; q < K; ++q) {
+    emc_ctx* c = b->c[q];
+    Gpu& g = *c->gpu;
+    g.n = (long long)(unsigned)b->h_n[q];
+    g.n_pending = false;
+    hfl[q] = fly_args(c, step[q], 0);
+    nmax = std::max(nmax, g.n);
+  }
+  BHIP(b, hipMemcpyAsync(b->d_tab + b->off_fly, b->h_tab + b->off_fly, K * sizeof(FlyArgs), hipMemcpyHostToDevice, s));
+  BHIP(b, hipEventRecord(b->ev[5], s));
+  BHIP(b, emc_launch_fly_b(b->bt, nmax, b->c[0]->gpu->wave_slots, s));
+  BHIP(b, hipEventRecord(b->ev[6], s));
+  BHIP(b, emc_launch_copy_b(b->cp_tally, K, FLYOUT_N, s));
+  BHIP(b, hipMemcpyAsync(b->h_tally, b->d_tally, (size_t)K * FLYOUT_N * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, s));
+  BHIP(b, hipEventRecord(b->ev[4], s));
+  BHIP(b, hipEventSynchronize(b->ev[4]));
+  // per device: the solve's result, the flight's counts (finish_fly), the window
+  int imax = 0;
+  for (int q = 0; q < K; ++q) {
+    emc_ctx* c = b->c[q];
+    Gpu& g = *c->gpu;
+    if (last >= 0) {
+      const double* sh = b->h_scal + (size_t)(last & 1) * 8 * K + 8 * q;
+      const double bn = std::sqrt(sh[2]) > 0 ? std::sqrt(sh[2]) : 1.0;
+      const double rel = std::sqrt(std::max(0.0, sh[0])) / bn;
+      c->last_pcg_iters = (int)sh[3];
+      c->last_relres = rel;
+      imax = std::max(imax, (int)sh[3]);
+      if (!std::isfinite(rel)) return bfail(b, EMC_ERR_SOLVER, "PCG residual is not finite");
+      if (rel > c->opts.poisson_rtol) return bfail(b, EMC_ERR_SOLVER, "PCG did not reach the tolerance");
+    }
+    std::memcpy(g.tally_host, b->h_tally + (size_t)q * FLYOUT_N, FLYOUT_N * sizeof(unsigned long long));
+    const int passes = finish_fly(c, hfl[q]);
+    if (passes < 0) return bfail(b, passes, c->err);
+    const Config& cf = c->dev.cfg;
+    long long mr_gen = 0;
+    if (cf.flag_mr && cf.mr_step > 0 && step[q] % (uint64_t)cf.mr_step == (uint64_t)cf.mr_step - 1) {
+      long long nb = 0, na = 0;
+      const int st = run_mr(c, step[q], &nb, &na);
+      if (st) return bfail(b, st, c->err);
+      mr_gen = na - nb;
+    }
+    read_tallies(c, acc ? &acc[q] : nullptr, passes);
+    read_tallies(c, &c->win, passes);
+    if (acc) acc[q].n_mr_gen += mr_gen;
+    c->win.n_mr_gen += mr_gen;
+    ++c->win_steps;
+    // work finish_fly or the refresh queued on the device's own stream
+    BHIP(b, hipStreamSynchronize(g.stream));
+  }
+  *iters_max = imax;
+  float ms;
+  for (int q = 0; q < 3; ++q) {
+    BHIP(b, hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]));
+    tsum[q] += ms;
+  }
+  BHIP(b, hipEventElapsedTime(&ms, b->ev[3], b->ev[4]));
+  tsum[3] += ms;
+  BHIP(b, hipEventElapsedTime(&ms, b->ev[5], b->ev[6]));
+  tsum[4] += ms;
+  return EMC_OK;
+}
+
+extern "C" {
+
+emc_batch* emc_batch_create(emc_ctx* const* ctx, int n, int* status) {
+  int st = EMC_OK;
+  emc_batch* b = nullptr;
+  if (!ctx || n < 1) st = EMC_ERR_ARG;
+  else {
+    b = new emc_batch();
+    b->K = n;
+    b->c.assign(ctx, ctx + n);
+    st = batch_setup(b);
+  }
+  if (status) *status = st;
+  return b;
+}
+
+const char* emc_batch_last_error(emc_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int emc_batch_step(emc_batch* b, int nsteps, emc_fly_tallies* acc) {
+  if (!b || nsteps < 0) return EMC_ERR_ARG;
+  if (!b->s) return bfail(b, EMC_ERR_STATE, "the batch was not set up: " + b->err);
+  for (emc_ctx* c : b->c)
+    if (c->heat_phase) return bfail(b, EMC_ERR_STATE, "a context of the batch is in its heating phase");
+  double tsum[5] = {0, 0, 0, 0, 0};
+  int imax = 0;
+  for (int q = 0; q < nsteps; ++q) {
+    const int st = batch_one_step(b, acc, tsum, &imax);
+    if (st) return st;
+  }
+  for (int q = 0; q < 5; ++q) b->timing[q] = nsteps ? tsum[q] / nsteps : 0;
+  b->last_iters = imax;
+  return EMC_OK;
+}
+
+int emc_batch_timing(emc_batch* b, double* ms5, int* pcg_iters_max) {
+  if (!b) return EMC_ERR_ARG;
+  if (ms5)
+    for (int q = 0; q < 5; ++q) ms5[q] = b->timing[q];
+  if (pcg_iters_max) *pcg_iters_max = b->last_iters;
+  return EMC_OK;
+}
+
+void emc_batch_destroy(emc_batch* b) {
+  if (!b) return;
+  if (b->device >= 0) {
+    (void)hipSetDevice(b->device);
+    if (b->s) (void)hipStreamSynchronize(b->s);
+  }
+  for (auto e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : b->ev_scal)
+    if (e) (void)hipEventDestroy(e);
+  if (b->ev_n) (void)hipEventDestroy(b->ev_n);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  if (b->h_n) (void)hipHostFree(b->h_n);
+  if (b->h_scal) (void)hipHostFree(b->h_scal);
+  if (b->h_tally) (void)hipHostFree(b->h_tally);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  b->mem.free_all();
+  delete b;
+}
+
 int emc_run_file(const char* input_txt, const emc_opts* o) {
   int st = 0;
   emc_ctx* c = emc_create(input_txt, o, &st);
@@ -2129,6 +2556,10 @@ int emc_run_file(const char* input_txt, const emc_opts* o) {
     return st;
   };
   if (st) return stop("emc_create");
+  if (!c->gpu) {
+    st = fail(c, EMC_ERR_NOGPU, "a run needs a GPU (opts.device >= 0)");
+    return stop("emc_create");
+  }
   int64_t n = 0;
   if ((st = emc_particles_init(c, &n))) return stop("initial population");
   const Config& cf = c->dev.cfg;

@@ -171,6 +171,9 @@ struct MgArgs {
   const double* part_rr;
   double* scal;
   int npart, pad;
+  // the cycle's result z = M r and the block partials of r.z (CG grid)
+  double* z;
+  double* part_rz;
 };
 
 // Initial population on the device: particle s belongs to spec group g with
@@ -206,7 +209,38 @@ struct MrArgs {
   size_t temp_bytes;
 };
 
+// One word-copy job per device of a batch (k_copy_u64_b).
+struct CopyArgs {
+  const void* src;                // null: write zeros
+  unsigned long long* dst;
+  size_t n;
+  int src_u32, pad;               // 1: src holds 32-bit words (widened)
+};
+
+// Device tables of the argument blocks of a batch of K devices that share the
+// mesh (one entry per device, filled by the host before each launch).
+struct BatchTables {
+  int K, pad;
+  const SortArgs* sort;
+  const DepositArgs* dep;
+  const PcgArgs* pcg;
+  const MgArgs *mg_start, *mg_even, *mg_odd;
+  const FieldArgs* field;
+  const FlyArgs* fly;
+};
+
 }  // namespace emc
+
+emc::MgArgs emc_mg_start_args(const emc::PcgArgs& a, const emc::MgArgs& m);
+emc::MgArgs emc_mg_iter_args(const emc::PcgArgs& a, const emc::MgArgs& m, int it);
+hipError_t emc_launch_pcg_start_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, hipStream_t s);
+hipError_t emc_launch_pcg_iter_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, int it,
+                                 hipStream_t s);
+hipError_t emc_launch_tile_sort_b(const emc::BatchTables& t, long long nmax, hipStream_t s);
+hipError_t emc_launch_deposit_b(const emc::BatchTables& t, int ntile, size_t nnode, hipStream_t s);
+hipError_t emc_launch_field_b(const emc::BatchTables& t, size_t ncell, hipStream_t s);
+hipError_t emc_launch_fly_b(const emc::BatchTables& t, long long nmax, long long wave_slots, hipStream_t s);
+hipError_t emc_launch_copy_b(const emc::CopyArgs* t, int K, size_t nwords, hipStream_t s);
 
 hipError_t emc_launch_sample(const emc::SampleArgs& a, hipStream_t s);
 size_t emc_mr_temp_bytes(long long n, long long ngroup);
@@ -216,7 +250,7 @@ hipError_t emc_launch_fly(const emc::FlyArgs& a, hipStream_t s);
 hipError_t emc_launch_fly_clones(const emc::FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s);
 hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long begin, unsigned long long end,
                             const unsigned long long* ndead, unsigned long long* ticket, hipStream_t s);
-hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, unsigned long long* ctr, hipStream_t s);
+hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, hipStream_t s);
 hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);

@@ -24,6 +24,16 @@
 
 namespace emc {
 
+// ------------------------------------------------------------------ argument blocks
+// A kernel takes its argument block by value (one device) or a device table of
+// them, one per device of a batch that shares the mesh: one launch then covers
+// every device, the device being blockIdx.y, and each block runs exactly the
+// code of the single-device launch on its device's block.
+template <class A>
+__device__ __forceinline__ const A& kargs(const A& a) { return a; }
+template <class A>
+__device__ __forceinline__ const A& kargs(const A* t) { return t[blockIdx.y]; }
+
 // ------------------------------------------------------------------ reductions
 template <class T>
 __device__ inline T wave_sum(T v) {
@@ -74,7 +84,9 @@ __device__ inline void load_particle(Flight& f, const FlyArgs& a, long long q, u
 // wave does not wait for its longest history, was measured slower: most
 // particles end after one or two events, so nearly every iteration paid for a
 // refill.)
-__global__ __launch_bounds__(256, 3) void k_fly_fast(FlyArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256, 3) void k_fly_fast(AS as_) {
+  const FlyArgs& a = kargs(as_);
   const long long stride = (long long)gridDim.x * blockDim.x;
   long long ev = 0;
   const int lane = threadIdx.x & 63;
@@ -125,7 +137,9 @@ __global__ __launch_bounds__(256, 3) void k_fly_fast(FlyArgs a) {
 // one after the other, so the list is spread thinly: each wave takes `lpw` entries
 // (lanes 0..lpw-1), with lpw chosen so that the list covers about a.wave_slots
 // waves, the number the GPU runs at once.
-__global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_fly_slow(AS as_) {
+  const FlyArgs& a = kargs(as_);
   long long ev = 0, ph = 0, im = 0, ca = 0;
   const long long ns = (long long)*a.nslow;
   const long long slots = a.wave_slots > 0 ? a.wave_slots : 1;
@@ -161,7 +175,7 @@ __global__ __launch_bounds__(256) void k_fly_slow(FlyArgs a) {
 
 // Clones [begin, end) of the clone buffer fly the remaining time of the step;
 // `lpw` clones per wave, as in k_fly_slow.
-__device__ inline void fly_clone_range(const FlyArgs& a, unsigned long long begin, unsigned long long end, int lpw) {
+__device__ __forceinline__ void fly_clone_range(const FlyArgs& a, unsigned long long begin, unsigned long long end, int lpw) {
   long long ev = 0, ph = 0, im = 0, ca = 0;
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -206,7 +220,11 @@ __global__ __launch_bounds__(256) void k_fly_clones(FlyArgs a, unsigned long lon
 // Clone passes without the host: ctr[4] .. ctr[5] is the range the next pass
 // flies.  k_clone_range moves it on to the clones created since and counts the
 // non-empty ranges in ctr[6] (one thread).
-__global__ void k_clone_range(unsigned long long* ctr, unsigned long long cap) {
+template <class AS>
+__global__ void k_clone_range(AS as_) {
+  const FlyArgs& a = kargs(as_);
+  unsigned long long* ctr = a.nclones;
+  const unsigned long long cap = a.clone_cap;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctr[4] = ctr[5];
     ctr[5] = ctr[0] < cap ? ctr[0] : cap;
@@ -215,7 +233,10 @@ __global__ void k_clone_range(unsigned long long* ctr, unsigned long long cap) {
 }
 
 // k_fly_clones over ctr[4] .. ctr[5], spread as k_fly_slow spreads its list
-__global__ __launch_bounds__(256) void k_fly_clones_dev(FlyArgs a, const unsigned long long* ctr) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_fly_clones_dev(AS as_) {
+  const FlyArgs& a = kargs(as_);
+  const unsigned long long* ctr = a.nclones;
   const long long begin = (long long)ctr[4], n = (long long)ctr[5] - begin;
   if (n <= 0) return;
   const long long slots = a.wave_slots > 0 ? a.wave_slots : 1;
@@ -249,7 +270,10 @@ __global__ __launch_bounds__(256) void k_merge(FlyArgs a, unsigned long long beg
 }
 
 // k_merge over the clones flown by the device-driven passes: [0, ctr[4])
-__global__ __launch_bounds__(256) void k_merge_dev(FlyArgs a, unsigned long long* ctr) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_merge_dev(AS as_) {
+  const FlyArgs& a = kargs(as_);
+  unsigned long long* ctr = a.nclones;
   const unsigned long long end = ctr[4], ndead = ctr[1];
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < (long long)end; q += stride) {
@@ -306,7 +330,9 @@ __device__ inline unsigned wave_run_add(unsigned* ctr, int key) {
   return base + (unsigned)(lane - hl);
 }
 
-__global__ __launch_bounds__(256) void k_tile_hist(SortArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_tile_hist(AS as_) {
+  const SortArgs& a = kargs(as_);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
     const long long q = base + threadIdx.x;
@@ -321,7 +347,9 @@ __global__ __launch_bounds__(256) void k_tile_hist(SortArgs a) {
 
 // One block of 1024 threads: exclusive scan of the tile counts into toff, then the
 // counts and cursors are cleared for the next sort.
-__global__ __launch_bounds__(1024) void k_tile_scan(SortArgs a) {
+template <class AS>
+__global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
+  const SortArgs& a = kargs(as_);
   __shared__ unsigned wsum[16];
   const int nt = a.ntile;
   const int tid = threadIdx.x;
@@ -355,7 +383,9 @@ __global__ __launch_bounds__(1024) void k_tile_scan(SortArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_tile_scatter(SortArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_tile_scatter(AS as_) {
+  const SortArgs& a = kargs(as_);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
     const long long q = base + threadIdx.x;
@@ -420,7 +450,9 @@ __device__ inline void stat_node(const StatArgs& a, size_t q) {
 // the result does not depend on the order of the particles.  In a step the
 // gather also forms the Poisson right-hand side and statistic()'s potential and
 // heat sums of each node, and the cell pass adds the cell counts to the window.
-__global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_deposit_tile(AS as_) {
+  const DepositArgs& a = kargs(as_);
   __shared__ unsigned long long sn[DEP_NQ * TILE_NODES];
   __shared__ unsigned long long sc[4 * TILE_CELLS];
   const int t = blockIdx.x;
@@ -508,7 +540,9 @@ __global__ __launch_bounds__(256) void k_deposit_tile(DepositArgs a) {
 }
 
 // One thread per node: sum the node's entries of the non-empty tiles around it.
-__global__ __launch_bounds__(256) void k_deposit_gather(DepositArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_deposit_gather(AS as_) {
+  const DepositArgs& a = kargs(as_);
   const size_t NN = a.nnode;
   const int pnx = a.M.pnx, pnz = a.M.pnz;
   for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < NN; n += (size_t)gridDim.x * blockDim.x) {
@@ -596,7 +630,9 @@ __global__ void k_rhs(RhsArgs a) {
 
 // pot2field (src/mcmodel.cpp:2776-2813), Rho clamp (:2704-2711) and
 // GetSurfRoughnessPhononScRate (src/GetSurfRoughnessPhononScRate.cpp:4-71), per cell.
-__global__ void k_field(FieldArgs a) {
+template <class AS>
+__global__ void k_field(AS as_) {
+  const FieldArgs& a = kargs(as_);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const int cnx = a.cnx, cnz = a.cnz, pnx = a.pnx, pnz = a.pnz;
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.ncell; c += stride) {
@@ -747,13 +783,17 @@ __device__ inline void node_ijk(const PcgArgs& a, size_t n, int& i, int& j, int&
 #define GRID_LOOP(n, N) for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < (N); n += (size_t)gridDim.x * blockDim.x)
 
 // fixed nodes take their value from b; the solve starts not done
-__global__ __launch_bounds__(256) void k_pcg_fix(PcgArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_pcg_fix(AS as_) {
+  const PcgArgs& a = kargs(as_);
   GRID_LOOP(n, a.n) if (a.kind[n] != 0) a.x[n] = a.b[n];
   if (blockIdx.x == 0 && threadIdx.x == 0) { *a.done = 0; a.scal[3] = 0; }
 }
 
 // r = b - A x ; partials (r.r, b.b) -> part_a
-__global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_pcg_resid(AS as_) {
+  const PcgArgs& a = kargs(as_);
   double rr = 0, bb = 0;
   GRID_LOOP(n, a.n) {
     int i, j, k;
@@ -768,7 +808,9 @@ __global__ __launch_bounds__(256) void k_pcg_resid(PcgArgs a) {
 
 // scal = (r.r, r.z, b.b, 0, rtol^2 b.b); done when the start meets the tolerance
 // (one block)
-__global__ __launch_bounds__(256) void k_pcg_begin(PcgArgs a) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_pcg_begin(AS as_) {
+  const PcgArgs& a = kargs(as_);
   double ta[3];
   total3(a.part_a, a.nblocks, ta);
   const double tz = total1(a.part_rz0, a.nblocks);
@@ -783,8 +825,16 @@ __global__ __launch_bounds__(256) void k_pcg_begin(PcgArgs a) {
 // then q = A p and the partial p.q.  p at the six neighbours is recomputed from
 // z and p_in with the same operations, so p_out is written once and p_in never
 // changes during the pass.
-__global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_old, const double* rz_new,
-                                                   const double* p_in, double* p_out, int first) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
+  const PcgArgs& a = kargs(as_);
+  // iteration `it`: rz_new holds r.z of the last cycle (part_rz0 for even it),
+  // rz_old that of the cycle before; p alternates between p and p2
+  const double* rz_old = (it & 1) ? a.part_rz0 : a.part_rz1;
+  const double* rz_new = (it & 1) ? a.part_rz1 : a.part_rz0;
+  double* p_out = (it & 1) ? a.p2 : a.p;
+  const double* p_in = (it & 1) ? a.p : a.p2;
+  const int first = it == 0 ? 1 : 0;
   if (*a.done) return;
   double beta = 0.0;
   if (!first) {
@@ -815,7 +865,12 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(PcgArgs a, const double* rz_o
 }
 
 // alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
-__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* rz_old, const double* p) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_pcg_update(AS as_, int it) {
+  const PcgArgs& a = kargs(as_);
+  // r.z of the last cycle, and the search direction k_pcg_spmvp wrote
+  const double* rz_old = (it & 1) ? a.part_rz1 : a.part_rz0;
+  const double* p = (it & 1) ? a.p2 : a.p;
   if (*a.done) return;
   double ta, tb;
   total_first2(rz_old, a.part_pq, a.nblocks, ta, tb);
@@ -1081,44 +1136,64 @@ __device__ inline void book_residual(const MgArgs& m) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_mg_pre2(MgArgs m) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_pre2(AS ms_) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   book_residual(m);
   ph_pre2<true, GridLoop>(m, m.L[0]);
 }
-__global__ __launch_bounds__(256) void k_mg_resid(MgArgs m) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_resid(AS ms_) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_resid<true, GridLoop>(m, m.L[0]);
 }
 // levels >= 1: sweep from zero and residual in one pass
-__global__ __launch_bounds__(256) void k_mg_resid1(MgArgs m, int l) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_resid1(AS ms_, int l) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_resid1<false, GridLoop>(m, m.L[l]);
 }
-__global__ __launch_bounds__(256) void k_mg_restrict(MgArgs m, int l) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_restrict(AS ms_, int l) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_restrict<GridLoop>(m.L[l], m.L[l + 1]);
 }
-__global__ __launch_bounds__(256) void k_mg_prolong(MgArgs m) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_prolong(AS ms_) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_prolong<GridLoop>(m.L[0], m.L[1], mg_result(m, 1));
 }
-__global__ __launch_bounds__(256) void k_mg_prolong1(MgArgs m, int l) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_prolong1(AS ms_, int l) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_prolong<GridLoop, true>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
 }
 // levels >= 1: one sweep t -> x
-__global__ __launch_bounds__(256) void k_mg_post1(MgArgs m, int l) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_post1(AS ms_, int l) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
 }
-__global__ __launch_bounds__(256) void k_mg_post0a(MgArgs m) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_post0a(AS ms_) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   ph_sweep<true, GridLoop>(m, m.L[0], m.L[0].t, m.L[0].x);
 }
 
 // last sweep of level 0 into z, with the partial r.z of the CG (r = b of level 0)
-__global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, double* z, double* part_rz) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_sweep_last(AS ms_) {
+  const MgArgs& m = kargs(ms_);
+  double* z = m.z;
+  double* part_rz = m.part_rz;
   if (*m.done) return;
   const MgLevelDev& L = m.L[0];
   const double* src = L.x;
@@ -1135,19 +1210,28 @@ __global__ __launch_bounds__(256) void k_mg_sweep_last(MgArgs m, double* z, doub
 }
 
 // coarsest level: x = C^-1 b, one wave per row
-__global__ __launch_bounds__(256) void k_mg_coarse(MgArgs m) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_coarse(AS ms_) {
+  const MgArgs& m = kargs(ms_);
   if (*m.done) return;
   const MgLevelDev& L = m.L[m.nl - 1];
   ph_coarse(m, L.b, L.x, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
 
 // single-level hierarchy: z = C^-1 r over the grid, then the partial r.z
-__global__ __launch_bounds__(256) void k_mg_coarse_z(MgArgs m, double* z) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_coarse_z(AS ms_) {
+  const MgArgs& m = kargs(ms_);
+  double* z = m.z;
   if (*m.done) return;
   book_residual(m);
   ph_coarse(m, m.L[0].b, z, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
-__global__ __launch_bounds__(256) void k_mg_dot_rz(MgArgs m, const double* z, double* part_rz) {
+template <class AS>
+__global__ __launch_bounds__(256) void k_mg_dot_rz(AS ms_) {
+  const MgArgs& m = kargs(ms_);
+  const double* z = m.z;
+  double* part_rz = m.part_rz;
   if (*m.done) return;
   const MgLevelDev& L = m.L[0];
   double rz = 0;
@@ -1194,6 +1278,16 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = 0;
 }
 
+// per device of a batch: dst[q] = src[q] (src null: 0) for q < n (readbacks of
+// many devices gathered into one block, and the clearing of their counters)
+__global__ void k_copy_u64_b(const CopyArgs* t) {
+  const CopyArgs& c = t[blockIdx.y];
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < c.n; q += (size_t)gridDim.x * blockDim.x)
+    c.dst[q] = !c.src ? 0ull
+               : c.src_u32 ? (unsigned long long)((const unsigned*)c.src)[q]
+                           : ((const unsigned long long*)c.src)[q];
+}
+
 }  // namespace emc
 
 // ---------------------------------------------------------------- launchers
@@ -1211,9 +1305,9 @@ static int grid_for(long long n, int bs, int maxg = 65536) {
 // strides over the list, so no empty workgroups of this large kernel are queued.
 hipError_t emc_launch_fly(const FlyArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fly_fast, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_fly_fast<FlyArgs>, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
   const long long slow_blocks = a.wave_slots > 4 ? a.wave_slots / 4 : 1;
-  hipLaunchKernelGGL(k_fly_slow, dim3((unsigned)slow_blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_fly_slow<FlyArgs>, dim3((unsigned)slow_blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_fly_clones(const FlyArgs& a, unsigned long long b, unsigned long long e, hipStream_t s) {
@@ -1236,28 +1330,28 @@ hipError_t emc_launch_merge(const FlyArgs& a, unsigned long long begin, unsigned
 // they flew, with every range read on the device: no host round trip.  ctr[4]
 // and ctr[5] must read zero on entry; afterwards ctr[4] .. ctr[5] holds clones
 // of a third generation still to fly (almost always empty).
-hipError_t emc_launch_clones_dev(const FlyArgs& a, unsigned long long* ctr, hipStream_t s) {
+hipError_t emc_launch_clones_dev(const FlyArgs& a, hipStream_t s) {
   const long long blocks = a.wave_slots > 4 ? a.wave_slots / 4 : 1;
   for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(k_clone_range, dim3(1), dim3(64), 0, s, ctr, a.clone_cap);
-    hipLaunchKernelGGL(k_fly_clones_dev, dim3((unsigned)blocks), dim3(256), 0, s, a, (const unsigned long long*)ctr);
+    hipLaunchKernelGGL(k_clone_range<FlyArgs>, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_fly_clones_dev<FlyArgs>, dim3((unsigned)blocks), dim3(256), 0, s, a);
   }
-  hipLaunchKernelGGL(k_clone_range, dim3(1), dim3(64), 0, s, ctr, a.clone_cap);
-  hipLaunchKernelGGL(k_merge_dev, dim3((unsigned)blocks), dim3(256), 0, s, a, ctr);
+  hipLaunchKernelGGL(k_clone_range<FlyArgs>, dim3(1), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_merge_dev<FlyArgs>, dim3((unsigned)blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 // sort P into Q by tile; toff[ntile] is the live count afterwards
 hipError_t emc_launch_tile_sort(const SortArgs& a, hipStream_t s) {
   // 2048 blocks striding over the population: the per-particle work is short, and
   // one block per 256 particles (12 k blocks at 3 M) costs more in block dispatch
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, a);
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist<SortArgs>, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tile_scan<SortArgs>, dim3(1), dim3(1024), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter<SortArgs>, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_deposit_tile, dim3(a.ntile), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_deposit_gather, dim3(grid_for((long long)a.nnode, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_deposit_tile<DepositArgs>, dim3(a.ntile), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_deposit_gather<DepositArgs>, dim3(grid_for((long long)a.nnode, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_rhs(const RhsArgs& a, hipStream_t s) {
@@ -1270,7 +1364,7 @@ hipError_t emc_launch_qc_potential(const QcArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t emc_launch_field(const FieldArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_field, dim3(grid_for((long long)a.ncell, 256, 1024)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_field<FieldArgs>, dim3(grid_for((long long)a.ncell, 256, 1024)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_sample(const SampleArgs& a, hipStream_t s) {
@@ -1285,46 +1379,68 @@ hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
 }
 int emc_pcg_blocks(size_t n) { return grid_for((long long)n, 256, 1024); }
 
-// z = M r: one V(2,2) cycle; part_rz receives the block partials of r.z (CG grid)
-static void launch_vcycle(const PcgArgs& a, const MgArgs& m, double* part_rz, hipStream_t s) {
+// z = M r: one V(2,2) cycle (m.z receives z, m.part_rz the block partials of
+// r.z on the CG grid).  AS is the argument block (one device) or a device table
+// of K of them (a batch: gridDim.y = K, the same grids as one device).
+template <class AS>
+static void launch_vcycle(const MgArgs& m, AS ms, int nblocks, unsigned K, hipStream_t s) {
   const int nl = m.nl;
   if (nl == 1) {
-    hipLaunchKernelGGL(k_mg_coarse_z, dim3(grid_for((long long)m.nc * 64, 256)), dim3(256), 0, s, m, a.z);
-    hipLaunchKernelGGL(k_mg_dot_rz, dim3(a.nblocks), dim3(256), 0, s, m, (const double*)a.z, part_rz);
+    hipLaunchKernelGGL(k_mg_coarse_z<AS>, dim3(grid_for((long long)m.nc * 64, 256), K), dim3(256), 0, s, ms);
+    hipLaunchKernelGGL(k_mg_dot_rz<AS>, dim3(nblocks, K), dim3(256), 0, s, ms);
     return;
   }
   for (int l = 0; l < nl - 1; ++l) {
-    const dim3 g(grid_for((long long)m.L[l].n, 256));
+    const dim3 g(grid_for((long long)m.L[l].n, 256), K);
     if (l == 0) {
-      hipLaunchKernelGGL(k_mg_pre2, g, dim3(256), 0, s, m);
-      hipLaunchKernelGGL(k_mg_resid, g, dim3(256), 0, s, m);
+      hipLaunchKernelGGL(k_mg_pre2<AS>, g, dim3(256), 0, s, ms);
+      hipLaunchKernelGGL(k_mg_resid<AS>, g, dim3(256), 0, s, ms);
     } else {
-      hipLaunchKernelGGL(k_mg_resid1, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_resid1<AS>, g, dim3(256), 0, s, ms, l);
     }
-    hipLaunchKernelGGL(k_mg_restrict, dim3(grid_for((long long)m.L[l + 1].n, 256)), dim3(256), 0, s, m, l);
+    hipLaunchKernelGGL(k_mg_restrict<AS>, dim3(grid_for((long long)m.L[l + 1].n, 256), K), dim3(256), 0, s, ms, l);
   }
-  hipLaunchKernelGGL(k_mg_coarse, dim3(grid_for((long long)m.nc * 64, 256)), dim3(256), 0, s, m);
+  hipLaunchKernelGGL(k_mg_coarse<AS>, dim3(grid_for((long long)m.nc * 64, 256), K), dim3(256), 0, s, ms);
   for (int l = nl - 2; l >= 0; --l) {
-    const dim3 g(grid_for((long long)m.L[l].n, 256));
+    const dim3 g(grid_for((long long)m.L[l].n, 256), K);
     if (l == 0) {
-      hipLaunchKernelGGL(k_mg_prolong, g, dim3(256), 0, s, m);
-      hipLaunchKernelGGL(k_mg_post0a, g, dim3(256), 0, s, m);
-      hipLaunchKernelGGL(k_mg_sweep_last, dim3(a.nblocks), dim3(256), 0, s, m, a.z, part_rz);
+      hipLaunchKernelGGL(k_mg_prolong<AS>, g, dim3(256), 0, s, ms);
+      hipLaunchKernelGGL(k_mg_post0a<AS>, g, dim3(256), 0, s, ms);
+      hipLaunchKernelGGL(k_mg_sweep_last<AS>, dim3(nblocks, K), dim3(256), 0, s, ms);
     } else {
-      hipLaunchKernelGGL(k_mg_prolong1, g, dim3(256), 0, s, m, l);
-      hipLaunchKernelGGL(k_mg_post1, g, dim3(256), 0, s, m, l);
+      hipLaunchKernelGGL(k_mg_prolong1<AS>, g, dim3(256), 0, s, ms, l);
+      hipLaunchKernelGGL(k_mg_post1<AS>, g, dim3(256), 0, s, ms, l);
     }
   }
 }
 
-hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s) {
-  const int g = a.nblocks;
-  hipLaunchKernelGGL(k_pcg_fix, dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_resid, dim3(g), dim3(256), 0, s, a);
+// the cycle's argument block of the solve's start (r.z partials into part_rz0,
+// no residual booking) and of CG iteration `it` (r.z into the partial array the
+// next iteration reads as rz_new; the first kernel books the residual)
+MgArgs emc_mg_start_args(const PcgArgs& a, const MgArgs& m) {
   MgArgs m0 = m;
   m0.part_rr = nullptr;
-  launch_vcycle(a, m0, a.part_rz0, s);
-  hipLaunchKernelGGL(k_pcg_begin, dim3(1), dim3(256), 0, s, a);
+  m0.z = a.z;
+  m0.part_rz = a.part_rz0;
+  return m0;
+}
+MgArgs emc_mg_iter_args(const PcgArgs& a, const MgArgs& m, int it) {
+  MgArgs mi = m;
+  mi.part_rr = a.part_rr;
+  mi.scal = a.scal;
+  mi.npart = a.nblocks;
+  mi.z = a.z;
+  mi.part_rz = (it & 1) ? a.part_rz0 : a.part_rz1;
+  return mi;
+}
+
+hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s) {
+  const int g = a.nblocks;
+  hipLaunchKernelGGL(k_pcg_fix<PcgArgs>, dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_resid<PcgArgs>, dim3(g), dim3(256), 0, s, a);
+  const MgArgs m0 = emc_mg_start_args(a, m);
+  launch_vcycle<MgArgs>(m0, m0, g, 1, s);
+  hipLaunchKernelGGL(k_pcg_begin<PcgArgs>, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1332,19 +1448,64 @@ hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s
 // part_rz1; the search direction alternates between p (even it) and p2
 hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStream_t s) {
   const int g = a.nblocks;
-  double* rz_old = (it & 1) ? a.part_rz1 : a.part_rz0;
-  double* rz_new = (it & 1) ? a.part_rz0 : a.part_rz1;
-  double* p_out = (it & 1) ? a.p2 : a.p;
-  const double* p_in = (it & 1) ? a.p : a.p2;
-  // the previous iteration's cycle left r.z in rz_old; before that, rz_new held
-  // the r.z of the iteration before (beta = rz_old / rz_new of the previous pair)
-  hipLaunchKernelGGL(k_pcg_spmvp, dim3(g), dim3(256), 0, s, a, (const double*)rz_new, (const double*)rz_old, p_in,
-                     p_out, it == 0 ? 1 : 0);
-  hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(256), 0, s, a, (const double*)rz_old, (const double*)p_out);
-  MgArgs mi = m;
-  mi.part_rr = a.part_rr;
-  mi.scal = a.scal;
-  mi.npart = a.nblocks;
-  launch_vcycle(a, mi, rz_new, s);
+  hipLaunchKernelGGL(k_pcg_spmvp<PcgArgs>, dim3(g), dim3(256), 0, s, a, it);
+  hipLaunchKernelGGL(k_pcg_update<PcgArgs>, dim3(g), dim3(256), 0, s, a, it);
+  const MgArgs mi = emc_mg_iter_args(a, m, it);
+  launch_vcycle<MgArgs>(mi, mi, g, 1, s);
+  return hipGetLastError();
+}
+
+// ---- a batch of K devices sharing the mesh: one launch per kernel for all of
+// them (gridDim.y = K), each device's blocks identical to its own launch.
+hipError_t emc_launch_pcg_start_b(const BatchTables& t, const MgArgs& shape, int nblocks, hipStream_t s) {
+  const unsigned K = (unsigned)t.K;
+  hipLaunchKernelGGL(k_pcg_fix<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
+  hipLaunchKernelGGL(k_pcg_resid<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
+  launch_vcycle<const MgArgs*>(shape, t.mg_start, nblocks, K, s);
+  hipLaunchKernelGGL(k_pcg_begin<const PcgArgs*>, dim3(1, K), dim3(256), 0, s, t.pcg);
+  return hipGetLastError();
+}
+hipError_t emc_launch_pcg_iter_b(const BatchTables& t, const MgArgs& shape, int nblocks, int it, hipStream_t s) {
+  const unsigned K = (unsigned)t.K;
+  hipLaunchKernelGGL(k_pcg_spmvp<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
+  hipLaunchKernelGGL(k_pcg_update<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
+  launch_vcycle<const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
+  return hipGetLastError();
+}
+hipError_t emc_launch_tile_sort_b(const BatchTables& t, long long nmax, hipStream_t s) {
+  const unsigned K = (unsigned)t.K;
+  if (nmax > 0) hipLaunchKernelGGL(k_tile_hist<const SortArgs*>, dim3(grid_for(nmax, 256, 2048), K), dim3(256), 0, s, t.sort);
+  hipLaunchKernelGGL(k_tile_scan<const SortArgs*>, dim3(1, K), dim3(1024), 0, s, t.sort);
+  if (nmax > 0)
+    hipLaunchKernelGGL(k_tile_scatter<const SortArgs*>, dim3(grid_for(nmax, 256, 2048), K), dim3(256), 0, s, t.sort);
+  return hipGetLastError();
+}
+hipError_t emc_launch_deposit_b(const BatchTables& t, int ntile, size_t nnode, hipStream_t s) {
+  const unsigned K = (unsigned)t.K;
+  hipLaunchKernelGGL(k_deposit_tile<const DepositArgs*>, dim3(ntile, K), dim3(256), 0, s, t.dep);
+  hipLaunchKernelGGL(k_deposit_gather<const DepositArgs*>, dim3(grid_for((long long)nnode, 256), K), dim3(256), 0, s,
+                     t.dep);
+  return hipGetLastError();
+}
+hipError_t emc_launch_field_b(const BatchTables& t, size_t ncell, hipStream_t s) {
+  hipLaunchKernelGGL(k_field<const FieldArgs*>, dim3(grid_for((long long)ncell, 256, 1024), (unsigned)t.K), dim3(256), 0,
+                     s, t.field);
+  return hipGetLastError();
+}
+hipError_t emc_launch_fly_b(const BatchTables& t, long long nmax, long long wave_slots, hipStream_t s) {
+  const unsigned K = (unsigned)t.K;
+  if (nmax > 0) hipLaunchKernelGGL(k_fly_fast<const FlyArgs*>, dim3(grid_for(nmax, 256), K), dim3(256), 0, s, t.fly);
+  const long long blocks = wave_slots > 4 ? wave_slots / 4 : 1;
+  hipLaunchKernelGGL(k_fly_slow<const FlyArgs*>, dim3((unsigned)blocks, K), dim3(256), 0, s, t.fly);
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(k_clone_range<const FlyArgs*>, dim3(1, K), dim3(64), 0, s, t.fly);
+    hipLaunchKernelGGL(k_fly_clones_dev<const FlyArgs*>, dim3((unsigned)blocks, K), dim3(256), 0, s, t.fly);
+  }
+  hipLaunchKernelGGL(k_clone_range<const FlyArgs*>, dim3(1, K), dim3(64), 0, s, t.fly);
+  hipLaunchKernelGGL(k_merge_dev<const FlyArgs*>, dim3((unsigned)blocks, K), dim3(256), 0, s, t.fly);
+  return hipGetLastError();
+}
+hipError_t emc_launch_copy_b(const CopyArgs* t, int K, size_t nwords, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_u64_b, dim3(grid_for((long long)nwords, 256, 64), (unsigned)K), dim3(256), 0, s, t);
   return hipGetLastError();
 }

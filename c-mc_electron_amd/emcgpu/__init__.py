@@ -432,6 +432,49 @@ def tallies_dict(t):
     }
 
 
+class Batch:
+    """Engines that share the mesh, stepped together (emc_batch_*): every kernel
+    of the step is one launch for all of them; each engine ends exactly where
+    stepping it alone would."""
+
+    def __init__(self, engines):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.ctx for e in self.engines])
+        st = C.c_int(0)
+        self.b = self.lib.emc_batch_create(arr, len(self.engines), C.byref(st))
+        if st.value != 0:
+            msg = self.lib.emc_batch_last_error(self.b).decode() if self.b else "bad arguments"
+            self.lib.emc_batch_destroy(self.b)
+            self.b = None
+            raise EmcError(st.value, msg)
+
+    def step(self, nsteps=1):
+        """nsteps steps of every engine; one FlyTallies per engine."""
+        acc = (abi.FlyTallies * len(self.engines))()
+        st = self.lib.emc_batch_step(self.b, nsteps, acc)
+        if st != 0:
+            raise EmcError(st, self.lib.emc_batch_last_error(self.b).decode())
+        return list(acc)
+
+    def timing(self):
+        ms = np.zeros(5)
+        it = C.c_int(0)
+        self.lib.emc_batch_timing(self.b, _ptr(ms, C.c_double), C.byref(it))
+        return ms, it.value
+
+    def close(self):
+        if self.b:
+            self.lib.emc_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def write_deck(out_dir, **kw):
     """Write FinFET decks (see decks/make_finfet.py)."""
     import importlib.util

@@ -180,6 +180,11 @@ def declare(lib):
         "emc_heat_begin": (C.c_int, [ctx]),
         "emc_qc_potential": (C.c_int, [ctx, _d, _d]),
         "emc_heat_step": (C.c_int, [ctx, C.c_int, P(FlyTallies)]),
+        "emc_batch_create": (C.c_void_p, [P(C.c_void_p), C.c_int, P(C.c_int)]),
+        "emc_batch_step": (C.c_int, [C.c_void_p, C.c_int, P(FlyTallies)]),
+        "emc_batch_timing": (C.c_int, [C.c_void_p, _d, _i]),
+        "emc_batch_last_error": (C.c_char_p, [C.c_void_p]),
+        "emc_batch_destroy": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -305,6 +305,72 @@ int emc_poisson_solve_noalloc(emc_ctx *ctx, const double *rhs, double *sol, doub
  * the last level.  Works without a GPU. */
 int emc_poisson_mg_level(emc_ctx *ctx, int level, int dims[3], double *a27, double *omega);
 
+/* A batch of devices stepped together.  Replaces running one process per bias
+ * point (run(), src/mcmodel.cpp:498-566, with the bias in the deck's contact
+ * blocks, src/mcmodel.cpp:4190-4584): the contexts share one GPU and the mesh
+ * dimensions (the same geometry; contact voltages, particle counts and seeds may
+ * differ), and every kernel of the step is one launch for all of them.  Each
+ * context keeps its own state and results: n steps of the batch leave every
+ * context exactly where n emc_step calls on it alone would (bit for bit).  No
+ * quantum correction; the heating phase stays per context.  The contexts must
+ * outlive the batch and are not stepped alone while it steps them.
+ * acc (may be NULL) receives one tally block per context.  emc_batch_timing
+ * gives the phase times (ms per step, as emc_step_timing) of the last call and
+ * the largest CG iteration count of its last step. */
+typedef struct emc_batch emc_batch;
+emc_batch *emc_batch_create(emc_ctx *const *ctx, int n, int *status);
+int emc_batch_step(emc_batch *b, int nsteps, emc_fly_tallies *acc);
+int emc_batch_timing(emc_batch *b, double *ms5, int *pcg_iters_max);
+const char *emc_batch_last_error(emc_batch *b);
+void emc_batch_destroy(emc_batch *b);
+
+/* Serving: requests (a deck, a seed and a target precision of the drain current)
+ * arrive at any time and run concurrently on one GPU.  No counterpart in the
+ * reference, which runs one deck per process (src/main.cpp:3-61) and leaves the
+ * current's precision to finfet/scripts/current.py afterwards.  A worker thread
+ * admits queued requests in arrival order while their estimated device-memory
+ * footprint fits hbm_budget_bytes (a request always runs when nothing else
+ * does) and at most max_active run, keeps their state resident on the GPU, and
+ * steps every running request: those with the same mesh dimensions as one batch
+ * (emc_batch), different meshes at the same time on their own streams.  A request
+ * runs windows of window_steps steps; each window gives one drain current, the
+ * mean over contacts 0 and 1 of |Current| as output_stat prints it
+ * (src/mcmodel.cpp:756-842).  After skip_windows windows it is done when the 95 %
+ * Student-t interval of the mean over the later windows is within rel_tol of the
+ * mean (and at least min_windows of them), or after max_windows windows.  The
+ * result depends only on the deck, seed and window settings, never on the other
+ * requests: the same alone and in the mix, bit for bit.  Times are seconds since
+ * emc_server_create. */
+#define EMC_REQ_QUEUED 0
+#define EMC_REQ_RUNNING 1
+#define EMC_REQ_DONE 2
+#define EMC_REQ_FAILED 3
+#define EMC_REQ_MAX_WINDOWS 64
+typedef struct emc_request_opts {
+  uint64_t seed;
+  int window_steps, skip_windows, min_windows, max_windows;
+  double rel_tol;                 /* 0.02: +-2 % at 95 % confidence */
+} emc_request_opts;
+typedef struct emc_request_result {
+  int state, status, converged, windows;
+  double id_mean, id_half_width;  /* A, as data/current prints it              */
+  double id_windows[EMC_REQ_MAX_WINDOWS];
+  int64_t steps, particles, nodes;
+  double footprint_bytes;         /* measured device memory of the request       */
+  double t_submit, t_start, t_done;
+  char error[256];
+} emc_request_result;
+typedef struct emc_server emc_server;
+void emc_default_request(emc_request_opts *r);
+emc_server *emc_server_create(const char *table_dir, int device, double hbm_budget_bytes, int max_active,
+                              int *status);
+int emc_server_submit(emc_server *s, const char *input_txt, const emc_request_opts *r, int64_t *id);
+int emc_server_poll(emc_server *s, int64_t id, emc_request_result *out);
+/* waits for the request to finish (timeout_s < 0: no limit); EMC_ERR_STATE on timeout */
+int emc_server_wait(emc_server *s, int64_t id, double timeout_s, emc_request_result *out);
+const char *emc_server_last_error(emc_server *s);
+void emc_server_destroy(emc_server *s);
+
 /* quantumPotential (src/mcmodel.cpp:2378-2405): the effective potential of a node
  * potential on the GPU; qc_pot[n] = pot[n] outside the quantum region.  Needs a
  * deck with Flag_QuantumCorrection (EMC_ERR_STATE otherwise).  emc_step applies it

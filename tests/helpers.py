@@ -80,3 +80,59 @@ def sort_by_key(p):
     live = np.nonzero(p.i[:p.n] >= 0)[0]
     order = live[np.argsort(p.key[live], kind="stable")]
     return p.subset(order)
+
+
+def bulk_deck(cells=2, cell_nm=4, electrons=8000, holes=8000, dt=2e-15, donor=1e21):
+    """input.txt of a bulk silicon box: cells^3 cells of cell_nm nm, every outer
+    face PERIOD (a carrier leaving a boundary cell re-enters it on the opposite
+    face, CellHit.cpp Period()), donors 1e21 m^-3 (below the Brooks-Herring
+    threshold, so no impurity scattering), surface scattering off.  The one
+    contact only satisfies the grammar: these decks never solve Poisson."""
+    key = ("bulk", cells, cell_nm, electrons, holes, dt, donor)
+    if key in _DECKS:
+        return _DECKS[key]
+    d = tempfile.mkdtemp(prefix="emc_deck_bulk_")
+    L = cells * cell_nm
+    with open(os.path.join(d, "lgrid.txt"), "w") as f:
+        for _ in range(3):
+            f.write("%d\n" % (cells + 1))
+            for c in range(cells + 1):
+                f.write("%.10g\n" % (c * cell_nm * 1e-3))
+    ldg = f"""default_par_number 1 1
+region 0 {L} 0 {L} 0 {L} SILICON
+donor 0 {L} 0 {L} 0 {L} {donor:g}
+acceptor 0 {L} 0 {L} 0 {L} 0
+motioncube 0 {L} 0 {L} 0 {L} PASS PASS PASS PASS PASS PASS
+motionplane 0 0 0 {L} 0 {L} UP PERIOD
+motionplane {L} {L} 0 {L} 0 {L} DOWN PERIOD
+motionplane 0 {L} 0 0 0 {L} LEFT PERIOD
+motionplane 0 {L} {L} {L} 0 {L} RIGHT PERIOD
+motionplane 0 {L} 0 {L} 0 0 FRONT PERIOD
+motionplane 0 {L} 0 {L} {L} {L} BACK PERIOD
+contact
+  1 0
+  0 {L} 0 0 0 {L}
+  0
+end
+"""
+    with open(os.path.join(d, "ldg.txt"), "w") as f:
+        f.write(ldg)
+    with open(os.path.join(d, "input.txt"), "w") as f:
+        f.write(f"""gridFile=lgrid.txt
+device_file_name=ldg.txt
+total_step=1000
+ElectronNumber={electrons}
+HoleNumber={holes}
+Temperature=300
+dt={dt:g}
+stat_step=1000
+stat_heat_step=1000
+heat_steps=0
+Flag_QuantumCorrection=0
+Flag_MultipleRefresh=0
+Flag_compute_heat=0
+Flag_SurfaceScatter=0
+mr_step=10
+""")
+    _DECKS[key] = os.path.join(d, "input.txt")
+    return _DECKS[key]

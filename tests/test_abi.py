@@ -85,3 +85,23 @@ def test_checkpoint_and_refresh_need_a_gpu(tmp_path):
         assert lib.emc_multiple_refresh(eng.ctx, 9, C.byref(b), C.byref(a)) == emcgpu.abi.EMC_ERR_NOGPU
     finally:
         eng.close()
+
+
+EMC_RUN = os.path.join(REPO, "c-mc_electron_amd", "bin", "emc_run")
+
+
+def test_emc_run_executable_finds_its_tables(tmp_path):
+    """`emc_run input.txt` reads the deck from the working directory and the
+    tables from dirname(exe)/../input: without a GPU it gets as far as the run
+    and stops there with the no-GPU status (a missing table would stop it at
+    emc_create with an I/O error first)."""
+    import subprocess
+    assert os.access(EMC_RUN, os.X_OK)
+    emcgpu.write_deck(str(tmp_path), refine=1, electrons=2000, holes=200, vd=0.5, vg=0.7, total_step=2,
+                      stat_step=1)
+    r = subprocess.run([EMC_RUN, "input.txt", "--device", "-1"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "a run needs a GPU" in r.stderr, r.stderr
+    assert os.path.isdir(tmp_path / "data")
+    r = subprocess.run([EMC_RUN], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -9,13 +9,14 @@ cell densities of the step's population, the field of the fixed window-mean
 potential, and the oracle's flight with heat tallies under those fields.
 """
 import ctypes as C
+import os
 import filecmp
 
 import numpy as np
 import pytest
 
 import oracle_lib
-from helpers import cfg1_deck, emcgpu
+from helpers import REPO, cfg1_deck, emcgpu
 from test_gpu_parity import assert_same_population, assert_same_tallies
 
 pytestmark = pytest.mark.gpu
@@ -171,3 +172,36 @@ def test_run_file_writes_the_data_directory(tmp_path, monkeypatch):
     assert not (d / "heat7").exists()
     rows = (d / "par_info_restart7").read_text().split("\n")
     assert rows[-2] == "-1 -1 -1" and len(rows) == m.ncell + 2
+
+
+def test_emc_run_executable_equals_run_file(tmp_path, monkeypatch):
+    """`emc_run input.txt` (the process boundary, tables from dirname(exe)/../input)
+    writes the same data/ files, byte for byte, as emc_run_file called through the
+    library with the package's tables."""
+    import filecmp
+    import subprocess
+    deck = cfg1_deck(stat_step=STAT_STEP, total_step=8, heat=1, heat_steps=4)
+    ddir = os.path.dirname(deck)
+    a, b = tmp_path / "exe", tmp_path / "lib"
+    for d in (a, b):
+        d.mkdir()
+        for f in ("input.txt", "ldg.txt", "lgrid.txt"):
+            (d / f).write_bytes(open(os.path.join(ddir, f), "rb").read())
+    exe = os.path.join(REPO, "c-mc_electron_amd", "bin", "emc_run")
+    print("[gpu] emc_run executable: 8 steps + 4 heat steps", flush=True)
+    r = subprocess.run([exe, "input.txt"], cwd=a, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    monkeypatch.chdir(b)
+    (b / "data").mkdir()
+    lib = emcgpu.load_library()
+    o = emcgpu.abi.Opts()
+    lib.emc_default_opts(C.byref(o))
+    table_dir = emcgpu.TABLE_DIR.encode()
+    o.table_dir = table_dir
+    o.device = 0
+    assert lib.emc_run_file(b"input.txt", C.byref(o)) == 0
+    names = sorted(os.listdir(b / "data"))
+    assert "current" in names and "heat3" in names
+    assert sorted(os.listdir(a / "data")) == names
+    _, mismatch, errors = filecmp.cmpfiles(a / "data", b / "data", names, shallow=False)
+    assert not mismatch and not errors, (mismatch, errors)
