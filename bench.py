@@ -255,6 +255,228 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": mib * 2 ** 20, "ok": bool(dst[-1].item() == 1.0)}))
 
 
+# ---------------------------------------------------------------- bias sweep
+SWEEP_VG = [0.8 * q / 7 for q in range(8)]
+SWEEP_VD = [0.05 + 0.75 * q / 7 for q in range(8)]
+
+
+def sweep_points():
+    """64 bias points (Vg 0-0.8 V x Vd 0.05-0.8 V, 8 x 8) of the 20x40x12-cell
+    n-FinFET; the electron count of a point follows its equilibrium inversion
+    charge, taken linear in Vg: 50 k at Vg = 0 to 500 k at Vg = 0.8 V (holes a
+    tenth; the reference's one carrier of each type per silicon cell on top)."""
+    pts = []
+    for vg in SWEEP_VG:
+        for vd in SWEEP_VD:
+            ne = int(50000 + 450000 * vg / 0.8)
+            pts.append((vg, vd, ne))
+    return pts
+
+
+def sweep_engines(tmp, pts, seed0=100):
+    engs = []
+    for q, (vg, vd, ne) in enumerate(pts):
+        d = os.path.join(tmp, f"pt{q}")
+        inp = emcgpu.write_deck(d, refine=1, electrons=ne, holes=ne // 10, vd=vd, vg=vg, total_step=100000,
+                                stat_step=1000)
+        e = emcgpu.Engine(inp, device=0, seed=seed0 + q)
+        e.particles_init()
+        engs.append(e)
+    return engs
+
+
+def window_current(t, m, w):
+    c = emcgpu.tallies_dict(t)["contact"]
+    cur = 0.0
+    for q in range(2):
+        gen = float(c[q, 1]) / emcgpu.abi.FX_CONTACT
+        cat = float(c[q, 4]) / emcgpu.abi.FX_CONTACT
+        cur += abs((gen - cat) / (m.dt * w) * m.curr0 * 1e-2)
+    return cur / 2
+
+
+def sweep_throughput(steps=60, solo_points=8, solo_steps=15):
+    """The 64-point sweep stepped as one batch against the same points stepped one
+    at a time (the plain GPU path): wall time per step of the whole sweep."""
+    tmp = tempfile.mkdtemp(prefix="emc_sweep_")
+    pts = sweep_points()
+    t0 = time.perf_counter()
+    engs = sweep_engines(tmp, pts)
+    setup = time.perf_counter() - t0
+    nparts = [e.count() for e in engs]
+    b = emcgpu.Batch(engs)
+    b.step(5)
+    t0 = time.perf_counter()
+    b.step(steps)
+    t_batch = (time.perf_counter() - t0) / steps
+    ms, _ = b.timing()
+    b.close()
+    # one at a time: a spread of points (every eighth), each stepped alone
+    sel = list(range(0, len(engs), max(1, len(engs) // solo_points)))[:solo_points]
+    per = []
+    for q in sel:
+        engs[q].step(3)
+        t0 = time.perf_counter()
+        engs[q].step(solo_steps)
+        per.append((time.perf_counter() - t0) / solo_steps)
+    t_solo_sweep = float(np.mean(per)) * len(engs)
+    for e in engs:
+        e.close()
+    return {"points": len(pts), "particles_total": int(sum(nparts)), "particles_min": int(min(nparts)),
+            "particles_max": int(max(nparts)), "setup_s": setup, "batched_ms_per_step": t_batch * 1e3,
+            "one_at_a_time_ms_per_step": t_solo_sweep * 1e3, "speedup": t_solo_sweep / t_batch,
+            "batched_particle_steps_per_s": sum(nparts) / t_batch, "batched_phase_ms": list(ms)}
+
+
+def sweep_converged(window=500, relax_steps=1000, max_windows=40, rel=0.02):
+    """The 64-point sweep run until every point's drain current is known to
+    +-rel at 95 % (Student-t over windows after relaxation; points whose current
+    is below 1 % of the sweep's largest are held to rel of that floor)."""
+    from scipy import stats as st
+    tmp = tempfile.mkdtemp(prefix="emc_sweep_")
+    pts = sweep_points()
+    t_start = time.perf_counter()
+    engs = sweep_engines(tmp, pts)
+    m = engs[0].mesh
+    b = emcgpu.Batch(engs)
+    b.step(relax_steps)
+    wins = [[] for _ in pts]
+    done = [False] * len(pts)
+    nwin = 0
+    while not all(done) and nwin < max_windows:
+        acc = b.step(window)
+        nwin += 1
+        for q, t in enumerate(acc):
+            wins[q].append(window_current(t, m, window))
+        if nwin < 3:
+            continue
+        imax = max(np.mean(w) for w in wins)
+        for q in range(len(pts)):
+            x = np.array(wins[q])
+            hw = st.t.ppf(0.975, len(x) - 1) * x.std(ddof=1) / np.sqrt(len(x))
+            done[q] = hw <= rel * max(abs(x.mean()), 0.01 * imax)
+        print(f"[bench] sweep window {nwin}: {sum(done)} of {len(pts)} points within {rel:.0%}", file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t_start
+    ms, _ = b.timing()
+    b.close()
+    ids = [float(np.mean(w)) for w in wins]
+    for e in engs:
+        e.close()
+    steps = relax_steps + nwin * window
+    return {"points": len(pts), "converged": int(sum(done)), "windows": nwin, "window_steps": window,
+            "relax_steps": relax_steps, "steps_per_point": steps, "wall_s": wall,
+            "bias_points_per_hour": len(pts) * 3600.0 / wall, "id_max": max(ids), "id_min": min(ids)}
+
+
+# ---------------------------------------------------------------- precision
+def precision_run(refine=8, electrons=8000000, holes=800000, relax_steps=100, solves=5, steps=600, window=100):
+    """The 320x96x160-node-step FinFET mesh (160x320x96 cells, 4.99 M nodes) with
+    8 M electrons: the FP32-bulk solve with FP64 iterative refinement against the
+    FP64 solve on the same right-hand sides (max |phi_mixed - phi_fp64| and the
+    time of each), then the drain current of a run with each solve (same seed and
+    initial population; windows of `window` steps, Student-t 95 % interval)."""
+    from scipy import stats as st
+    tmp = tempfile.mkdtemp(prefix="emc_prec_")
+    inp = emcgpu.write_deck(tmp, refine=refine, electrons=electrons, holes=holes, vd=0.5, vg=0.7, total_step=100000,
+                            stat_step=1000)
+    t0 = time.perf_counter()
+    eng = emcgpu.Engine(inp, device=0, seed=1)
+    n0 = eng.particles_init()
+    setup = time.perf_counter() - t0
+    pot0 = eng.mesh.pot0
+    print(f"[bench] precision: {n0} particles, {eng.mesh.nnode} nodes, setup {setup:.1f} s", file=sys.stderr,
+          flush=True)
+    relax(eng, relax_steps, "precision relax")
+    errs, t64, tmx, outer, inner, it64 = [], [], [], [], [], []
+    for q in range(solves):
+        node_q, _, _, _ = eng.deposit()
+        rhs = eng.rhs_from_charge(node_q)
+        x0 = eng.potential()
+        t = time.perf_counter()
+        x64, i64, r64 = eng.poisson_solve(rhs, x0=x0, rtol=1e-10)
+        t64.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        xm, o, i, rm = eng.poisson_solve_mixed(rhs, x0=x0, rtol=1e-10, inner_rtol=1e-5)
+        tmx.append(time.perf_counter() - t)
+        errs.append(float(np.max(np.abs(xm - x64)) * pot0))
+        outer.append(o)
+        inner.append(i)
+        it64.append(i64)
+        eng.step(20)
+        print(f"[bench] precision solve {q}: max err {errs[-1]:.3e} V, fp64 {t64[-1]*1e3:.1f} ms ({i64} it), "
+              f"mixed {tmx[-1]*1e3:.1f} ms ({o} outer, {i} FP32 it)", file=sys.stderr, flush=True)
+    eng.close()
+
+    def id_run(fp32):
+        e = emcgpu.Engine(inp, device=0, seed=1, poisson_fp32=fp32)
+        e.particles_init()
+        m = e.mesh
+        relax(e, relax_steps, "precision id relax")
+        w = []
+        t = time.perf_counter()
+        for k in range(steps // window):
+            w.append(window_current(e.step(window), m, window))
+        el = time.perf_counter() - t
+        ms, it = e.timing()
+        e.close()
+        x = np.array(w)
+        hw = st.t.ppf(0.975, len(x) - 1) * x.std(ddof=1) / np.sqrt(len(x))
+        return {"id_mean": float(x.mean()), "id_half_width": float(hw), "windows": len(x),
+                "ms_per_step": el / steps * 1e3, "poisson_ms": float(ms[1])}
+
+    r64 = id_run(False)
+    rmx = id_run(True)
+    print(f"[bench] precision id: fp64 {r64}, mixed {rmx}", file=sys.stderr, flush=True)
+    return {"particles": int(n0), "setup_s": setup,
+            "max_abs_err_v": max(errs), "errs_v": errs, "fp64_solve_ms": float(np.mean(t64)) * 1e3,
+            "mixed_solve_ms": float(np.mean(tmx)) * 1e3, "fp64_cg_iters": it64, "refine_steps": outer,
+            "fp32_cg_iters": inner, "id_fp64": r64, "id_mixed": rmx,
+            "id_diff_in_combined_half_widths": abs(r64["id_mean"] - rmx["id_mean"]) /
+            max(1e-300, np.hypot(r64["id_half_width"], rmx["id_half_width"]))}
+
+
+# ---------------------------------------------------------------- serving
+def serve_stream(nreq=12, rate=2.0, budget_gb=24.0, window=300, seed=7):
+    """A Poisson stream (mean `rate` requests/s) of bias/geometry requests: gate
+    length 10-30 nm, fin width 4-8 nm, 2 or 3 grid lines per nm (meshes of 45 k
+    to 420 k nodes), Vg 0.3-0.8 V, Vd 0.1-0.8 V, 20 k-100 k electrons; each runs
+    until its drain current is within +-2 % at 95 % (windows of `window` steps,
+    the first dropped) or 40 windows.  Latency = submission to result."""
+    rng = np.random.default_rng(seed)
+    tmp = tempfile.mkdtemp(prefix="emc_serve_")
+    srv = emcgpu.Server(device=0, hbm_budget_gb=budget_gb)
+    ids = []
+    t0 = time.perf_counter()
+    next_t = 0.0
+    for q in range(nreq):
+        next_t += rng.exponential(1.0 / rate)
+        lg, wf, ref = int(rng.integers(10, 31)), int(rng.integers(4, 9)), int(rng.integers(2, 4))
+        vg, vd, ne = float(rng.uniform(0.3, 0.8)), float(rng.uniform(0.1, 0.8)), int(rng.integers(20000, 100001))
+        inp = emcgpu.write_deck(os.path.join(tmp, f"r{q}"), refine=ref, electrons=ne, holes=ne // 10, vd=vd, vg=vg,
+                                lg=lg, wfin=wf, total_step=100000, stat_step=1000)
+        while time.perf_counter() - t0 < next_t:
+            time.sleep(0.005)
+        ids.append(srv.submit(inp, seed=1000 + q, window_steps=window, skip_windows=1, min_windows=4,
+                              max_windows=40, rel_tol=0.02))
+    res = [srv.wait(i, timeout=600) for i in ids]
+    wall = time.perf_counter() - t0
+    srv.close()
+    ok = [r for r in res if r["state"] == emcgpu.abi.REQ_DONE]
+    lat = np.array([r["t_done"] - r["t_submit"] for r in ok])
+    wait = np.array([r["t_start"] - r["t_submit"] for r in ok])
+    return {"requests": nreq, "done": len(ok), "converged": int(sum(r["converged"] for r in ok)),
+            "arrival_rate_per_s": rate, "hbm_budget_gb": budget_gb,
+            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "admission_wait_p50_s": float(np.percentile(wait, 50)) if len(wait) else None,
+            "requests_per_hour": len(ok) * 3600.0 / wall, "wall_s": wall,
+            "nodes_min": int(min(r["nodes"] for r in ok)) if ok else None,
+            "nodes_max": int(max(r["nodes"] for r in ok)) if ok else None,
+            "particles_max": int(max(r["particles"] for r in ok)) if ok else None,
+            "errors": [r["error"] for r in res if r["state"] != emcgpu.abi.REQ_DONE]}
+
+
 def relax(eng, nsteps, tag="relax"):
     done = 0
     while done < nsteps:
@@ -293,6 +515,13 @@ def main():
                     help="quantum correction on (Flag_QuantumCorrection=1 from step 0; not the default workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the 64-point bias sweep run to +-2 %% Id (bias-points/hour), one JSON line")
+    ap.add_argument("--serve", action="store_true",
+                    help="only the serving stream (p50/p99 latency, requests/hour), one JSON line")
+    ap.add_argument("--precision", action="store_true",
+                    help="only the mixed-precision Poisson workload (4.99 M nodes, 8 M electrons), one JSON line")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep throughput measurement")
     ap.add_argument("--save-checkpoint", default="", help="after the relaxation, save the state here and exit")
     ap.add_argument("--load-checkpoint", default="", help="start from this relaxed state instead of sampling")
     a = ap.parse_args()
@@ -302,6 +531,26 @@ def main():
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if a.sweep:
+        r = sweep_converged()
+        print(json.dumps({"metric": "bias-points/hour at +-2% Id (95% CI)", "value": r["bias_points_per_hour"],
+                          "unit": "bias-points/hour", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": "64-point sweep of the 20x40x12-cell n-FinFET, "
+                                                          "one batch", **r}}))
+        return
+    if a.precision:
+        r = precision_run()
+        print(json.dumps({"metric": "max |phi_mixed - phi_fp64|", "value": r["max_abs_err_v"], "unit": "V",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f32 bulk + f64 refinement",
+                          "data": "synthetic", "config": {"workload": "n-FinFET 160x320x96 cells (4.99 M nodes), "
+                                                          "ElectronNumber=8000000", **r}}))
+        return
+    if a.serve:
+        r = serve_stream()
+        print(json.dumps({"metric": "p50/p99 latency", "value": r["latency_p50_s"], "unit": "s", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": "Poisson stream of FinFET bias/geometry requests", **r}}))
+        return
     wl = ["--refine", str(a.refine), "--electrons", str(a.electrons), "--holes", str(a.holes)] + \
         (["--qc"] if a.qc else [])
     traffic = None
@@ -387,6 +636,12 @@ def main():
               flush=True)
         extras["stream_copy_gbs"] = stream_copy_gbs()
         print(f"[bench] stream copy {extras['stream_copy_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    if not a.no_extras and not a.no_sweep:
+        try:
+            extras["sweep"] = sweep_throughput()
+            print(f"[bench] sweep {extras['sweep']}", file=sys.stderr, flush=True)
+        except Exception as ex:  # reported, not fatal: the headline is the single-device step
+            extras["sweep"] = {"error": str(ex)}
     cpu = None
     if not a.no_cpu and a.cpu_seconds > 0:
         thr = a.cpu_threads or cpu_threads()

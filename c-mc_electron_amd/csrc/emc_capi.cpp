@@ -138,6 +138,13 @@ struct Gpu {
   double *part_a = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_rz0 = nullptr, *part_rz1 = nullptr;
   int* done = nullptr;
   MgArgs mg{};                        // device view of the multigrid hierarchy
+  // FP32 copies for the bulk of the mixed-precision solve, made on its first use:
+  // the level-0 operator, the hierarchy's vectors, damped inverse diagonals and
+  // coarsest inverse, and the inner CG's vectors (its b is the FP32 residual, its
+  // x the correction)
+  bool f32_ready = false;
+  PcgArgsT<float> fa{};
+  MgArgsT<float> fmg{};
   double* scal_host = nullptr;        // pinned [2][8]: r.r etc. after iterations (ping-pong)
   hipEvent_t scal_ready[2] = {nullptr, nullptr};
   uint8_t* kind = nullptr;
@@ -585,7 +592,7 @@ static int gpu_setup(emc_ctx* c) {
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
   HIPCHECK(c, hipMemset(g.tcount, 0, g.ntile * sizeof(unsigned)));
   HIPCHECK(c, hipHostMalloc((void**)&g.n_host, sizeof(unsigned), hipHostMallocDefault));
-  HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 24 * sizeof(double), hipHostMallocDefault));
+  HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 32 * sizeof(double), hipHostMallocDefault));
   HIPCHECK(c, hipHostMalloc((void**)&g.tally_host, FLYOUT_N * sizeof(unsigned long long), hipHostMallocDefault));
   g.ctr_host = g.tally_host + EMC_MAX_CONTACT * EMC_CT_NCOL + CNT_N;
   for (auto& e : g.scal_ready) HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -884,6 +891,8 @@ void emc_default_opts(emc_opts* o) {
   o->capacity_factor = 1.25;
   o->poisson_rtol = 1e-10;
   o->poisson_maxit = 20000;
+  o->poisson_fp32 = 0;
+  o->poisson_inner_rtol = 1e-5;
 }
 
 emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
@@ -1376,8 +1385,9 @@ static PcgArgs pcg_args(emc_ctx* c, const double* b, double* x, double rtol) {
 // With `defer` the solve is only queued past its last check: the result (iteration
 // count, residual) is read by finish_pcg() once the stream has passed it.
 static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres);
-static int run_pcg_with(emc_ctx* c, const PcgArgs& a, const MgArgs& mg, double rtol, int maxit, int* iters,
-                        double* relres, bool defer);
+template <class PA, class MA>
+static int run_pcg_with(emc_ctx* c, const PA& a, const MA& mg, double rtol, int maxit, int* iters, double* relres,
+                        bool defer);
 static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres,
                    bool defer = false) {
   Gpu& g = *c->gpu;
@@ -1387,8 +1397,9 @@ static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxi
 // The solve on explicit work vectors: `a` and `mg` name every buffer the
 // kernels write (the solve of the context uses its own, emc_poisson_solve_ws a
 // caller's workspace).
-static int run_pcg_with(emc_ctx* c, const PcgArgs& a, const MgArgs& mg, double rtol, int maxit, int* iters,
-                        double* relres, bool defer) {
+template <class PA, class MA>
+static int run_pcg_with(emc_ctx* c, const PA& a, const MA& mg, double rtol, int maxit, int* iters, double* relres,
+                        bool defer) {
   Gpu& g = *c->gpu;
   // k_pcg_begin sets done when the start already meets the tolerance: the
   // queued iterations are then no-ops and the first check below stops the loop
@@ -1438,6 +1449,92 @@ static int finish_pcg(emc_ctx* c, double rtol, int* iters, double* relres) {
   c->last_pcg_iters = it;
   c->last_relres = rel;
   if (rel > rtol) return fail(c, EMC_ERR_SOLVER, "PCG did not reach the tolerance");
+  return EMC_OK;
+}
+
+// The FP32 copies of the mixed-precision solve (first use).
+static int ensure_f32(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  if (g.f32_ready) return EMC_OK;
+  const Device& d = c->dev;
+  const size_t NN = d.nnode;
+  DevBuf& m = g.mem;
+  auto f32 = [&](const std::vector<double>& v) {
+    std::vector<float> f(v.size());
+    for (size_t q = 0; q < v.size(); ++q) f[q] = (float)v[q];
+    return m.dup(f);
+  };
+  PcgArgsT<float>& a = g.fa;
+  a.n = NN; a.pnx = d.pnx; a.pny = d.pny; a.pnz = d.pnz; a.nblocks = g.pcg_blocks;
+  a.diag = f32(d.diag); a.cx = f32(d.cx); a.cy = f32(d.cy); a.cz = f32(d.cz);
+  a.kind = g.kind;
+  float* b = m.alloc<float>(NN);
+  a.b = b;
+  a.x = m.alloc<float>(NN); a.r = m.alloc<float>(NN); a.p = m.alloc<float>(NN); a.p2 = m.alloc<float>(NN);
+  a.q = m.alloc<float>(NN); a.z = m.alloc<float>(NN);
+  for (double** pp : {&a.part_a, &a.part_pq, &a.part_rr, &a.part_rz0, &a.part_rz1}) *pp = m.alloc<double>(3 * g.pcg_blocks);
+  a.scal = m.alloc<double>(8);
+  a.done = m.alloc<int>(1);
+  const MgHierarchy& h = c->mg;
+  MgArgsT<float>& F = g.fmg;
+  F.nl = g.mg.nl; F.nc = g.mg.nc;
+  F.cinv = f32(h.cinv);
+  F.diag = a.diag; F.cx = a.cx; F.cy = a.cy; F.cz = a.cz;
+  F.done = a.done;
+  for (int l = 0; l < F.nl; ++l) {
+    const MgLevelDev& D = g.mg.L[l];
+    MgLevelDevT<float>& E = F.L[l];
+    E.nx = D.nx; E.ny = D.ny; E.nz = D.nz; E.n = D.n;
+    E.c14 = D.c14;
+    E.wdinv = f32(h.lv[l].wdinv);
+    E.x = m.alloc<float>(D.n); E.t = m.alloc<float>(D.n); E.r = m.alloc<float>(D.n);
+    E.b = l > 0 ? m.alloc<float>(D.n) : a.r;
+  }
+  if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("FP32 solver buffers: ") + hipGetErrorString(m.last));
+  g.f32_ready = true;
+  return EMC_OK;
+}
+
+// Mixed precision (the FP32 bulk with FP64 iterative refinement): r = b - A x in
+// FP64; while ||r|| > rtol ||b||, the multigrid-preconditioned CG solves A d = r
+// with FP32 vectors and operator to inner_rtol, and x += d in FP64.  Each outer
+// step gains about inner_rtol in the FP64 residual, so the result meets the FP64
+// tolerance; FP32 halves the bytes each CG iteration streams.
+static int run_pcg_mixed(emc_ctx* c, const double* b, double* x, double rtol, int maxit_outer, double inner_rtol,
+                         int* outer_out, int* inner_out, double* relres) {
+  Gpu& g = *c->gpu;
+  int st = ensure_f32(c);
+  if (st) return st;
+  RefineArgs ra{};
+  ra.a = pcg_args(c, b, x, rtol);
+  ra.r32 = const_cast<float*>(g.fa.b);
+  ra.d32 = g.fa.x;
+  PcgArgsT<float> fa = g.fa;
+  fa.rtol = inner_rtol;
+  double* sh = g.scal_host + 24;  // [24, 32): the refinement's norms
+  int outer = 0, inner = 0;
+  double rel = 0;
+  for (;;) {
+    HIPCHECK(c, emc_launch_refine_resid(ra, g.stream));
+    HIPCHECK(c, hipMemcpyAsync(sh, ra.a.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    const double bn = std::sqrt(sh[2]) > 0 ? std::sqrt(sh[2]) : 1.0;
+    rel = std::sqrt(std::max(0.0, sh[0])) / bn;
+    if (!std::isfinite(rel)) return fail(c, EMC_ERR_SOLVER, "residual of the refinement is not finite");
+    if (rel <= rtol) break;
+    if (outer >= maxit_outer) return fail(c, EMC_ERR_SOLVER, "iterative refinement did not reach the tolerance");
+    HIPCHECK(c, emc_launch_zero_f32(fa.x, fa.n, g.stream));
+    int it = 0;
+    st = run_pcg_with(c, fa, g.fmg, inner_rtol, c->opts.poisson_maxit, &it, nullptr, false);
+    if (st) return st;
+    inner += it;
+    HIPCHECK(c, emc_launch_refine_add(ra, g.stream));
+    ++outer;
+  }
+  if (outer_out) *outer_out = outer;
+  if (inner_out) *inner_out = inner;
+  if (relres) *relres = rel;
+  c->last_relres = rel;
   return EMC_OK;
 }
 
@@ -2000,6 +2097,20 @@ int emc_poisson_solve_ws(emc_ctx* c, const double* rhs, double* sol, double rtol
 
 extern "C" {
 
+int emc_poisson_solve_mixed(emc_ctx* c, const double* rhs, double* sol, double rtol, int maxit_outer,
+                            double inner_rtol, int* outer, int* inner, double* relres) {
+  if (!c || !rhs || !sol) return EMC_ERR_ARG;
+  if (!c->gpu) return fail(c, EMC_ERR_NOGPU, "no GPU attached");
+  Gpu& g = *c->gpu;
+  const size_t NN = c->dev.nnode;
+  HIPCHECK(c, hipMemcpy(g.rhs, rhs, NN * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(g.pot, sol, NN * 8, hipMemcpyHostToDevice));
+  int st = run_pcg_mixed(c, g.rhs, g.pot, rtol > 0 ? rtol : c->opts.poisson_rtol, maxit_outer > 0 ? maxit_outer : 20,
+                         inner_rtol > 0 ? inner_rtol : c->opts.poisson_inner_rtol, outer, inner, relres);
+  HIPCHECK(c, hipMemcpy(sol, g.pot, NN * 8, hipMemcpyDeviceToHost));
+  return st;
+}
+
 int emc_poisson_mg_level(emc_ctx* c, int level, int dims[3], double* a27, double* omega) {
   if (!c || !dims) return EMC_ERR_ARG;
   const std::string e = ensure_mg(c);
@@ -2113,7 +2224,11 @@ static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double ts
   HIPCHECK(c, emc_launch_deposit(da, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
   if (!heat_phase) {
-    st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr, true);
+    if (c->opts.poisson_fp32)
+      st = run_pcg_mixed(c, g.rhs, g.pot, c->opts.poisson_rtol, 20, c->opts.poisson_inner_rtol, nullptr, nullptr,
+                         nullptr);
+    else
+      st = run_pcg(c, g.rhs, g.pot, c->opts.poisson_rtol, c->opts.poisson_maxit, nullptr, nullptr, true);
     if (st) return st;
   }
   HIPCHECK(c, hipEventRecord(g.ev[2], g.stream));

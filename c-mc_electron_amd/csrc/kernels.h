@@ -130,22 +130,29 @@ struct FieldArgs {
 };
 
 // CG on the eliminated 7-point operator (level 0 of the multigrid hierarchy).
-struct PcgArgs {
+// R is the storage type of the vectors and of the operator: double, or float for
+// the FP32 bulk of the mixed-precision solve (arithmetic is FP64 either way;
+// only the words in memory are narrower).
+template <class R>
+struct PcgArgsT {
+  using real = R;
   size_t n;
   int pnx, pny, pnz, nblocks;
-  const double *diag, *cx, *cy, *cz, *b;
+  const R *diag, *cx, *cy, *cz, *b;
   const uint8_t* kind;
-  double *x, *r, *p, *p2, *q, *z;  // p and p2: search direction of even and odd iterations
+  R *x, *r, *p, *p2, *q, *z;  // p and p2: search direction of even and odd iterations
   double *part_a, *part_pq, *part_rr, *part_rz0, *part_rz1;  // block partial sums: part_a [3*nblocks], the others [nblocks]
   double* scal;  // [0] r.r [1] r.z [2] b.b [3] iterations done [4] r.r at the tolerance
   int* done;     // set once r.r reaches scal[4]: every later CG / V-cycle kernel returns at once
   double rtol;
   int pad0;
 };
+using PcgArgs = PcgArgsT<double>;
 
 constexpr int MG_MAXL = 10;
 
-struct MgLevelDev {
+template <class R>
+struct MgLevelDevT {
   int nx, ny, nz, pad;
   size_t n;
   // levels >= 1: upper half of the symmetric 27-point Galerkin stencil in FP32,
@@ -153,17 +160,20 @@ struct MgLevelDev {
   // st27 for o = 14..26; the coupling to offset o < 13 is entry 13-o of the
   // neighbour.  The preconditioner only needs an SPD operator close to A_l.
   const float* c14;
-  const double* wdinv;   // [n] omega/diag, 0 at fixed nodes
-  double *x, *t, *r, *b; // work vectors; b of level 0 is the CG residual
+  const R* wdinv;   // [n] omega/diag, 0 at fixed nodes
+  R *x, *t, *r, *b; // work vectors; b of level 0 is the CG residual
 };
+using MgLevelDev = MgLevelDevT<double>;
 
 // One multigrid cycle (V(2,2) on level 0, V(1,1) below) of the hierarchy built
 // by mg_build (multigrid.h).
-struct MgArgs {
-  MgLevelDev L[MG_MAXL];
-  int nl, nc;                          // levels, nodes of the coarsest level
-  const double* cinv;                  // [nc*nc] coarsest inverse, row major
-  const double *diag, *cx, *cy, *cz;   // level-0 operator
+template <class R>
+struct MgArgsT {
+  using real = R;
+  MgLevelDevT<R> L[MG_MAXL];
+  int nl, nc;                     // levels, nodes of the coarsest level
+  const R* cinv;                  // [nc*nc] coarsest inverse, row major
+  const R *diag, *cx, *cy, *cz;   // level-0 operator
   int* done;
   // CG iterations: the first kernel of the cycle also books the residual of the
   // iteration (r.r from part_rr, npart block partials) into scal and sets done
@@ -172,8 +182,18 @@ struct MgArgs {
   double* scal;
   int npart, pad;
   // the cycle's result z = M r and the block partials of r.z (CG grid)
-  double* z;
+  R* z;
   double* part_rz;
+};
+using MgArgs = MgArgsT<double>;
+
+// Iterative refinement of the mixed-precision solve: r = b - A x in FP64, its
+// FP32 copy is the right-hand side of the FP32 bulk solve, whose correction d is
+// added to x in FP64.
+struct RefineArgs {
+  PcgArgs a;        // FP64: b, x, r (r = b - A x), part_a, scal
+  float* r32;       // FP32 copy of r
+  const float* d32; // FP32 correction
 };
 
 // Initial population on the device: particle s belongs to spec group g with
@@ -233,6 +253,15 @@ struct BatchTables {
 
 emc::MgArgs emc_mg_start_args(const emc::PcgArgs& a, const emc::MgArgs& m);
 emc::MgArgs emc_mg_iter_args(const emc::PcgArgs& a, const emc::MgArgs& m, int it);
+emc::MgArgsT<float> emc_mg_start_args(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m);
+emc::MgArgsT<float> emc_mg_iter_args(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, int it);
+hipError_t emc_launch_pcg_start(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, hipStream_t s);
+hipError_t emc_launch_pcg_iter(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, int it, hipStream_t s);
+// mixed precision: r = b - A x (FP64) with its norms into a.scal[0] (r.r) and
+// a.scal[2] (b.b), and r32 = r; then x += d32
+hipError_t emc_launch_refine_resid(const emc::RefineArgs& r, hipStream_t s);
+hipError_t emc_launch_refine_add(const emc::RefineArgs& r, hipStream_t s);
+hipError_t emc_launch_zero_f32(float* p, size_t n, hipStream_t s);
 hipError_t emc_launch_pcg_start_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, hipStream_t s);
 hipError_t emc_launch_pcg_iter_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, int it,
                                  hipStream_t s);

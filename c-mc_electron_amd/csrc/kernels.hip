@@ -18,6 +18,8 @@
 // the FP64 results match the CPU restatement bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fly_core.h"
 #include "kernels.h"
 #include "qc.h"
@@ -681,15 +683,17 @@ __global__ void k_field(AS as_) {
 
 // ------------------------------------------------------------------ PCG
 // Operator: y = diag x + sum_dir c_dir(n) x(n+dir) + c_dir(n-dir) x(n-dir).
-__device__ inline double apply_op(const PcgArgs& a, const double* __restrict__ x, size_t n, int i, int j, int k) {
+template <class PA>
+__device__ inline double apply_op(const PA& a, const typename PA::real* __restrict__ x, size_t n, int i, int j,
+                                  int k) {
   const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
-  double v = a.diag[n] * x[n];
-  if (i + 1 < a.pnx) v += a.cx[n] * x[n + sx];
-  if (i > 0) v += a.cx[n - sx] * x[n - sx];
-  if (j + 1 < a.pny) v += a.cy[n] * x[n + sy];
-  if (j > 0) v += a.cy[n - sy] * x[n - sy];
-  if (k + 1 < a.pnz) v += a.cz[n] * x[n + 1];
-  if (k > 0) v += a.cz[n - 1] * x[n - 1];
+  double v = (double)a.diag[n] * (double)x[n];
+  if (i + 1 < a.pnx) v += (double)a.cx[n] * (double)x[n + sx];
+  if (i > 0) v += (double)a.cx[n - sx] * (double)x[n - sx];
+  if (j + 1 < a.pny) v += (double)a.cy[n] * (double)x[n + sy];
+  if (j > 0) v += (double)a.cy[n - sy] * (double)x[n - sy];
+  if (k + 1 < a.pnz) v += (double)a.cz[n] * (double)x[n + 1];
+  if (k > 0) v += (double)a.cz[n - 1] * (double)x[n - 1];
   return v;
 }
 
@@ -773,7 +777,8 @@ __device__ inline void total_first2(const double* pa, const double* pb, int nb, 
   __syncthreads();
 }
 
-__device__ inline void node_ijk(const PcgArgs& a, size_t n, int& i, int& j, int& k) {
+template <class PA>
+__device__ inline void node_ijk(const PA& a, size_t n, int& i, int& j, int& k) {
   k = (int)(n % a.pnz);
   const size_t t = n / a.pnz;
   i = (int)(t % a.pnx);
@@ -785,7 +790,8 @@ __device__ inline void node_ijk(const PcgArgs& a, size_t n, int& i, int& j, int&
 // fixed nodes take their value from b; the solve starts not done
 template <class AS>
 __global__ __launch_bounds__(256) void k_pcg_fix(AS as_) {
-  const PcgArgs& a = kargs(as_);
+  const auto& a = kargs(as_);
+  using R = typename std::decay_t<decltype(a)>::real;
   GRID_LOOP(n, a.n) if (a.kind[n] != 0) a.x[n] = a.b[n];
   if (blockIdx.x == 0 && threadIdx.x == 0) { *a.done = 0; a.scal[3] = 0; }
 }
@@ -793,15 +799,16 @@ __global__ __launch_bounds__(256) void k_pcg_fix(AS as_) {
 // r = b - A x ; partials (r.r, b.b) -> part_a
 template <class AS>
 __global__ __launch_bounds__(256) void k_pcg_resid(AS as_) {
-  const PcgArgs& a = kargs(as_);
+  const auto& a = kargs(as_);
+  using R = typename std::decay_t<decltype(a)>::real;
   double rr = 0, bb = 0;
   GRID_LOOP(n, a.n) {
     int i, j, k;
     node_ijk(a, n, i, j, k);
-    const double r = a.b[n] - apply_op(a, a.x, n, i, j, k);
-    a.r[n] = r;
+    const double r = (double)a.b[n] - apply_op(a, a.x, n, i, j, k);
+    a.r[n] = (R)r;
     rr += r * r;
-    bb += a.b[n] * a.b[n];
+    bb += (double)a.b[n] * (double)a.b[n];
   }
   block_reduce3(rr, bb, 0.0, a.part_a);
 }
@@ -810,7 +817,8 @@ __global__ __launch_bounds__(256) void k_pcg_resid(AS as_) {
 // (one block)
 template <class AS>
 __global__ __launch_bounds__(256) void k_pcg_begin(AS as_) {
-  const PcgArgs& a = kargs(as_);
+  const auto& a = kargs(as_);
+  using R = typename std::decay_t<decltype(a)>::real;
   double ta[3];
   total3(a.part_a, a.nblocks, ta);
   const double tz = total1(a.part_rz0, a.nblocks);
@@ -827,13 +835,14 @@ __global__ __launch_bounds__(256) void k_pcg_begin(AS as_) {
 // changes during the pass.
 template <class AS>
 __global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
-  const PcgArgs& a = kargs(as_);
+  const auto& a = kargs(as_);
+  using R = typename std::decay_t<decltype(a)>::real;
   // iteration `it`: rz_new holds r.z of the last cycle (part_rz0 for even it),
   // rz_old that of the cycle before; p alternates between p and p2
   const double* rz_old = (it & 1) ? a.part_rz0 : a.part_rz1;
   const double* rz_new = (it & 1) ? a.part_rz1 : a.part_rz0;
-  double* p_out = (it & 1) ? a.p2 : a.p;
-  const double* p_in = (it & 1) ? a.p : a.p2;
+  R* p_out = (it & 1) ? a.p2 : a.p;
+  const R* p_in = (it & 1) ? a.p : a.p2;
   const int first = it == 0 ? 1 : 0;
   if (*a.done) return;
   double beta = 0.0;
@@ -842,23 +851,23 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
     total_first2(rz_old, rz_new, a.nblocks, to, tn);
     beta = (to != 0.0) ? tn / to : 0.0;
   }
-  const double* __restrict__ z = a.z;
-  auto pv = [&](size_t m) { return first ? z[m] : z[m] + beta * p_in[m]; };
+  const R* __restrict__ z = a.z;
+  auto pv = [&](size_t m) -> double { return first ? (double)z[m] : (double)z[m] + beta * (double)p_in[m]; };
   const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
   double pq = 0;
   GRID_LOOP(n, a.n) {
     int i, j, k;
     node_ijk(a, n, i, j, k);
     const double pn = pv(n);
-    double q = a.diag[n] * pn;
-    if (i + 1 < a.pnx) q += a.cx[n] * pv(n + sx);
-    if (i > 0) q += a.cx[n - sx] * pv(n - sx);
-    if (j + 1 < a.pny) q += a.cy[n] * pv(n + sy);
-    if (j > 0) q += a.cy[n - sy] * pv(n - sy);
-    if (k + 1 < a.pnz) q += a.cz[n] * pv(n + 1);
-    if (k > 0) q += a.cz[n - 1] * pv(n - 1);
-    p_out[n] = pn;
-    a.q[n] = q;
+    double q = (double)a.diag[n] * pn;
+    if (i + 1 < a.pnx) q += (double)a.cx[n] * pv(n + sx);
+    if (i > 0) q += (double)a.cx[n - sx] * pv(n - sx);
+    if (j + 1 < a.pny) q += (double)a.cy[n] * pv(n + sy);
+    if (j > 0) q += (double)a.cy[n - sy] * pv(n - sy);
+    if (k + 1 < a.pnz) q += (double)a.cz[n] * pv(n + 1);
+    if (k > 0) q += (double)a.cz[n - 1] * pv(n - 1);
+    p_out[n] = (R)pn;
+    a.q[n] = (R)q;
     pq += pn * q;
   }
   block_reduce1(pq, a.part_pq);
@@ -867,19 +876,20 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
 // alpha = r.z / p.q ; x += alpha p ; r -= alpha q ; partial r.r
 template <class AS>
 __global__ __launch_bounds__(256) void k_pcg_update(AS as_, int it) {
-  const PcgArgs& a = kargs(as_);
+  const auto& a = kargs(as_);
+  using R = typename std::decay_t<decltype(a)>::real;
   // r.z of the last cycle, and the search direction k_pcg_spmvp wrote
   const double* rz_old = (it & 1) ? a.part_rz1 : a.part_rz0;
-  const double* p = (it & 1) ? a.p2 : a.p;
+  const R* p = (it & 1) ? a.p2 : a.p;
   if (*a.done) return;
   double ta, tb;
   total_first2(rz_old, a.part_pq, a.nblocks, ta, tb);
   const double alpha = (tb != 0.0) ? ta / tb : 0.0;
   double rr = 0;
   GRID_LOOP(n, a.n) {
-    a.x[n] += alpha * p[n];
-    const double r = a.r[n] - alpha * a.q[n];
-    a.r[n] = r;
+    a.x[n] = (R)((double)a.x[n] + alpha * (double)p[n]);
+    const double r = (double)a.r[n] - alpha * (double)a.q[n];
+    a.r[n] = (R)r;
     rr += r * r;
   }
   block_reduce1(rr, a.part_rr);
@@ -894,7 +904,8 @@ struct GridLoop {
   __device__ static size_t stride() { return (size_t)gridDim.x * blockDim.x; }
 };
 
-__device__ inline void lvl_ijk(const MgLevelDev& L, size_t n, int& i, int& j, int& k) {
+template <class LV>
+__device__ inline void lvl_ijk(const LV& L, size_t n, int& i, int& j, int& k) {
   k = (int)(n % L.nz);
   const size_t t = n / L.nz;
   i = (int)(t % L.nx);
@@ -914,8 +925,8 @@ __device__ inline void mem_fence_compiler() {
 // clamped address and a missing one adds 0.0, so the loads carry no branches and
 // issue together; the sum is the same, term for term, as over the present
 // neighbours only.
-template <bool FINE, class XA>
-__device__ inline double mg_Ax(const MgArgs& m, const MgLevelDev& L, const XA& xa, size_t n, int i, int j, int k) {
+template <bool FINE, class MA, class LV, class XA>
+__device__ inline double mg_Ax(const MA& m, const LV& L, const XA& xa, size_t n, int i, int j, int k) {
   if (FINE) {
     const size_t sx = L.nz, sy = (size_t)L.nx * L.nz;
     const bool ip = i + 1 < L.nx, im = i > 0, jp = j + 1 < L.ny, jm = j > 0, kp = k + 1 < L.nz, km = k > 0;
@@ -923,7 +934,7 @@ __device__ inline double mg_Ax(const MgArgs& m, const MgLevelDev& L, const XA& x
     const size_t nkp = kp ? n + 1 : n, nkm = km ? n - 1 : n;
     const double c_ip = m.cx[n], c_im = m.cx[nim], c_jp = m.cy[n], c_jm = m.cy[njm], c_kp = m.cz[n], c_km = m.cz[nkm];
     const double x0 = xa(n), x_ip = xa(nip), x_im = xa(nim), x_jp = xa(njp), x_jm = xa(njm), x_kp = xa(nkp),
-                 x_km = xa(nkm), dg = m.diag[n];
+                 x_km = xa(nkm), dg = (double)m.diag[n];
     mem_fence_compiler();
     double v = dg * x0;
     v += ip ? c_ip * x_ip : 0.0;
@@ -960,47 +971,47 @@ __device__ inline double mg_Ax(const MgArgs& m, const MgLevelDev& L, const XA& x
 }
 
 // t = x1 + wdinv (b - A x1) with x1 = wdinv b: two Jacobi sweeps from zero
-template <bool FINE, class LP>
-__device__ inline void ph_pre2(const MgArgs& m, const MgLevelDev& L) {
-  auto x1 = [&](size_t q) { return L.wdinv[q] * L.b[q]; };
+template <bool FINE, class LP, class MA, class LV>
+__device__ inline void ph_pre2(const MA& m, const LV& L) {
+  auto x1 = [&](size_t q) -> double { return (double)L.wdinv[q] * (double)L.b[q]; };
   for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    L.t[n] = x1(n) + L.wdinv[n] * (L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k));
+    L.t[n] = x1(n) + (double)L.wdinv[n] * ((double)L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k));
   }
 }
 
 // dst = src + wdinv (b - A src)
-template <bool FINE, class LP>
-__device__ inline void ph_sweep(const MgArgs& m, const MgLevelDev& L, const double* src, double* dst) {
-  auto xs = [&](size_t q) { return src[q]; };
+template <bool FINE, class LP, class MA, class LV, class R>
+__device__ inline void ph_sweep(const MA& m, const LV& L, const R* src, R* dst) {
+  auto xs = [&](size_t q) -> double { return src[q]; };
   for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    dst[n] = src[n] + L.wdinv[n] * (L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k));
+    dst[n] = (double)src[n] + (double)L.wdinv[n] * ((double)L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k));
   }
 }
 
 // r = b - A t
-template <bool FINE, class LP>
-__device__ inline void ph_resid(const MgArgs& m, const MgLevelDev& L) {
-  auto xs = [&](size_t q) { return L.t[q]; };
+template <bool FINE, class LP, class MA, class LV>
+__device__ inline void ph_resid(const MA& m, const LV& L) {
+  auto xs = [&](size_t q) -> double { return L.t[q]; };
   for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    L.r[n] = L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k);
+    L.r[n] = (double)L.b[n] - mg_Ax<FINE>(m, L, xs, n, i, j, k);
   }
 }
 
 // r = b - A x1 with x1 = wdinv b: one Jacobi sweep from zero and its residual
 // (levels >= 1: the restriction stored x1 in x)
-template <bool FINE, class LP>
-__device__ inline void ph_resid1(const MgArgs& m, const MgLevelDev& L) {
-  auto x1 = [&](size_t q) { return L.x[q]; };
+template <bool FINE, class LP, class MA, class LV>
+__device__ inline void ph_resid1(const MA& m, const LV& L) {
+  auto x1 = [&](size_t q) -> double { return L.x[q]; };
   for (size_t n = LP::start(); n < L.n; n += LP::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    L.r[n] = L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k);
+    L.r[n] = (double)L.b[n] - mg_Ax<FINE>(m, L, x1, n, i, j, k);
   }
 }
 
@@ -1010,8 +1021,8 @@ __device__ inline double pw1(int f, int C) { return f == 2 * C ? 1.0 : 0.5; }
 // (residuals of fixed fine nodes are zero, so the zeroed rows of P need no test);
 // also x_C = wdinv_C b_C, the first sweep from zero on the coarse level.  Fine
 // nodes off the mesh are read at a clamped address and add 0.0.
-template <class LP>
-__device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
+template <class LP, class LV>
+__device__ inline void ph_restrict(const LV& F, const LV& C) {
   for (size_t I = LP::start(); I < C.n; I += LP::stride()) {
     int ci, cj, ck;
     lvl_ijk(C, I, ci, cj, ck);
@@ -1042,7 +1053,7 @@ __device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
         }
       }
     C.b[I] = v;
-    C.x[I] = C.wdinv[I] * v;
+    C.x[I] = (double)C.wdinv[I] * v;
   }
 }
 
@@ -1050,8 +1061,8 @@ __device__ inline void ph_restrict(const MgLevelDev& F, const MgLevelDev& C) {
 // before the correction is the sweep from zero, t_F = wdinv b + P e (fixed
 // nodes 0).  Fine node f takes coarse node f/2 when f is even, (f-1)/2 and
 // (f+1)/2 with weight 1/2 each when odd; absent terms add 0.0.
-template <class LP, bool FROM_ZERO = false>
-__device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, const double* e) {
+template <class LP, bool FROM_ZERO = false, class LV, class R>
+__device__ inline void ph_prolong(const LV& F, const LV& C, const R* e) {
   for (size_t n = LP::start(); n < F.n; n += LP::stride()) {
     const double wd = F.wdinv[n];
     if (wd == 0.0) {
@@ -1084,8 +1095,8 @@ __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, cons
         for (int c = 0; c < 2; ++c)
           v += (okj[a] && oki[b] && okk[c]) ? w * pw1(k, ck[c]) * ev[(a * 2 + b) * 2 + c] : 0.0;
       }
-    if (FROM_ZERO) F.t[n] = wd * F.b[n] + v;
-    else F.t[n] += v;
+    if (FROM_ZERO) F.t[n] = wd * (double)F.b[n] + v;
+    else F.t[n] = (double)F.t[n] + v;
   }
 }
 
@@ -1093,12 +1104,13 @@ __device__ inline void ph_prolong(const MgLevelDev& F, const MgLevelDev& C, cons
 // the waves taking part.  A lane sums its columns lane, lane + 64, ... in that
 // order; the loads of eight of them are issued before the first multiply, so a
 // row costs a few memory round trips instead of one per 64 columns.
-__device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out, size_t wave, size_t nwaves) {
+template <class MA, class R>
+__device__ inline void ph_coarse(const MA& m, const R* in, R* out, size_t wave, size_t nwaves) {
   const int nc = m.nc;
   const int lane = threadIdx.x & 63;
   constexpr int U = 8;
   for (size_t r = wave; r < (size_t)nc; r += nwaves) {
-    const double* row = m.cinv + r * nc;
+    const R* row = m.cinv + r * nc;
     double v = 0;
     int c = lane;
     for (; c + 64 * (U - 1) < nc; c += 64 * U) {
@@ -1111,7 +1123,7 @@ __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out,
 #pragma unroll
       for (int u = 0; u < U; ++u) v += rv[u] * xv[u];
     }
-    for (; c < nc; c += 64) v += row[c] * in[c];
+    for (; c < nc; c += 64) v += (double)row[c] * (double)in[c];
     v = wave_sum(v);
     if (lane == 0) out[r] = v;
   }
@@ -1120,13 +1132,15 @@ __device__ inline void ph_coarse(const MgArgs& m, const double* in, double* out,
 // Cycle shape: V(2,2) on level 0, V(1,1) on the coarser levels (the same CG
 // iteration count on the FinFET meshes, two launches fewer per level).  The
 // result of every level >= 1 ends in x (the coarsest is solved into x).
-__device__ __host__ inline const double* mg_result(const MgArgs& m, int l) { return m.L[l].x; }
+template <class MA>
+__device__ __host__ inline const typename MA::real* mg_result(const MA& m, int l) { return m.L[l].x; }
 
 // In a CG iteration the first kernel of the cycle books the residual of the
 // iteration that has just updated r (= b of level 0): block 0 writes r.r and the
 // iteration count into scal and sets done at the tolerance (the rest of this
 // launch still runs; every later kernel returns at once).
-__device__ inline void book_residual(const MgArgs& m) {
+template <class MA>
+__device__ inline void book_residual(const MA& m) {
   if (!m.part_rr || blockIdx.x != 0) return;
   const double tr = total1(m.part_rr, m.npart);
   if (threadIdx.x == 0) {
@@ -1138,52 +1152,60 @@ __device__ inline void book_residual(const MgArgs& m) {
 
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_pre2(AS ms_) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   book_residual(m);
   ph_pre2<true, GridLoop>(m, m.L[0]);
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_resid(AS ms_) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_resid<true, GridLoop>(m, m.L[0]);
 }
 // levels >= 1: sweep from zero and residual in one pass
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_resid1(AS ms_, int l) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_resid1<false, GridLoop>(m, m.L[l]);
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_restrict(AS ms_, int l) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_restrict<GridLoop>(m.L[l], m.L[l + 1]);
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_prolong(AS ms_) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_prolong<GridLoop>(m.L[0], m.L[1], mg_result(m, 1));
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_prolong1(AS ms_, int l) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_prolong<GridLoop, true>(m.L[l], m.L[l + 1], mg_result(m, l + 1));
 }
 // levels >= 1: one sweep t -> x
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_post1(AS ms_, int l) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_sweep<false, GridLoop>(m, m.L[l], m.L[l].t, m.L[l].x);
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_post0a(AS ms_) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
   ph_sweep<true, GridLoop>(m, m.L[0], m.L[0].t, m.L[0].x);
 }
@@ -1191,20 +1213,21 @@ __global__ __launch_bounds__(256) void k_mg_post0a(AS ms_) {
 // last sweep of level 0 into z, with the partial r.z of the CG (r = b of level 0)
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_sweep_last(AS ms_) {
-  const MgArgs& m = kargs(ms_);
-  double* z = m.z;
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
+  R* z = m.z;
   double* part_rz = m.part_rz;
   if (*m.done) return;
-  const MgLevelDev& L = m.L[0];
-  const double* src = L.x;
-  auto xs = [&](size_t q) { return src[q]; };
+  const auto& L = m.L[0];
+  const R* src = L.x;
+  auto xs = [&](size_t q) -> double { return src[q]; };
   double rz = 0;
   for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) {
     int i, j, k;
     lvl_ijk(L, n, i, j, k);
-    const double v = src[n] + L.wdinv[n] * (L.b[n] - mg_Ax<true>(m, L, xs, n, i, j, k));
-    z[n] = v;
-    rz += L.b[n] * v;
+    const double v = (double)src[n] + (double)L.wdinv[n] * ((double)L.b[n] - mg_Ax<true>(m, L, xs, n, i, j, k));
+    z[n] = (R)v;
+    rz += (double)L.b[n] * v;
   }
   block_reduce1(rz, part_rz);
 }
@@ -1212,30 +1235,33 @@ __global__ __launch_bounds__(256) void k_mg_sweep_last(AS ms_) {
 // coarsest level: x = C^-1 b, one wave per row
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_coarse(AS ms_) {
-  const MgArgs& m = kargs(ms_);
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
   if (*m.done) return;
-  const MgLevelDev& L = m.L[m.nl - 1];
+  const auto& L = m.L[m.nl - 1];
   ph_coarse(m, L.b, L.x, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
 
 // single-level hierarchy: z = C^-1 r over the grid, then the partial r.z
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_coarse_z(AS ms_) {
-  const MgArgs& m = kargs(ms_);
-  double* z = m.z;
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
+  R* z = m.z;
   if (*m.done) return;
   book_residual(m);
   ph_coarse(m, m.L[0].b, z, ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6, ((size_t)gridDim.x * blockDim.x) >> 6);
 }
 template <class AS>
 __global__ __launch_bounds__(256) void k_mg_dot_rz(AS ms_) {
-  const MgArgs& m = kargs(ms_);
-  const double* z = m.z;
+  const auto& m = kargs(ms_);
+  using R = typename std::decay_t<decltype(m)>::real;
+  const R* z = m.z;
   double* part_rz = m.part_rz;
   if (*m.done) return;
-  const MgLevelDev& L = m.L[0];
+  const auto& L = m.L[0];
   double rz = 0;
-  for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) rz += L.b[n] * z[n];
+  for (size_t n = GridLoop::start(); n < L.n; n += GridLoop::stride()) rz += (double)L.b[n] * (double)z[n];
   block_reduce1(rz, part_rz);
 }
 
@@ -1276,6 +1302,40 @@ __global__ __launch_bounds__(256) void k_sample_population(SampleArgs a) {
 // ------------------------------------------------------------------ helpers
 __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = 0;
+}
+
+// ---------------------------------------------------------------- mixed precision
+// r = b - A x in FP64 (0 at the fixed nodes, which k_pcg_fix set to b), its
+// FP32 copy, and the partials of r.r and b.b
+__global__ __launch_bounds__(256) void k_refine_resid(RefineArgs ra) {
+  const PcgArgs& a = ra.a;
+  double rr = 0, bb = 0;
+  GRID_LOOP(n, a.n) {
+    int i, j, k;
+    node_ijk(a, n, i, j, k);
+    const double r = a.kind[n] != 0 ? 0.0 : a.b[n] - apply_op(a, a.x, n, i, j, k);
+    a.r[n] = r;
+    ra.r32[n] = (float)r;
+    rr += r * r;
+    bb += a.b[n] * a.b[n];
+  }
+  block_reduce3(rr, bb, 0.0, a.part_a);
+}
+// scal[0] = r.r, scal[2] = b.b (one block)
+__global__ __launch_bounds__(256) void k_refine_norms(RefineArgs ra) {
+  double t[3];
+  total3(ra.a.part_a, ra.a.nblocks, t);
+  if (threadIdx.x == 0) {
+    ra.a.scal[0] = t[0];
+    ra.a.scal[2] = t[1];
+  }
+}
+// x += d (the FP32 correction, in FP64)
+__global__ __launch_bounds__(256) void k_refine_add(RefineArgs ra) {
+  GRID_LOOP(n, ra.a.n) ra.a.x[n] += (double)ra.d32[n];
+}
+__global__ void k_zero_f32(float* p, size_t n) {
+  GRID_LOOP(q, n) p[q] = 0.0f;
 }
 
 // per device of a batch: dst[q] = src[q] (src null: 0) for q < n (readbacks of
@@ -1382,8 +1442,8 @@ int emc_pcg_blocks(size_t n) { return grid_for((long long)n, 256, 1024); }
 // z = M r: one V(2,2) cycle (m.z receives z, m.part_rz the block partials of
 // r.z on the CG grid).  AS is the argument block (one device) or a device table
 // of K of them (a batch: gridDim.y = K, the same grids as one device).
-template <class AS>
-static void launch_vcycle(const MgArgs& m, AS ms, int nblocks, unsigned K, hipStream_t s) {
+template <class SHAPE, class AS>
+static void launch_vcycle(const SHAPE& m, AS ms, int nblocks, unsigned K, hipStream_t s) {
   const int nl = m.nl;
   if (nl == 1) {
     hipLaunchKernelGGL(k_mg_coarse_z<AS>, dim3(grid_for((long long)m.nc * 64, 256), K), dim3(256), 0, s, ms);
@@ -1417,15 +1477,17 @@ static void launch_vcycle(const MgArgs& m, AS ms, int nblocks, unsigned K, hipSt
 // the cycle's argument block of the solve's start (r.z partials into part_rz0,
 // no residual booking) and of CG iteration `it` (r.z into the partial array the
 // next iteration reads as rz_new; the first kernel books the residual)
-MgArgs emc_mg_start_args(const PcgArgs& a, const MgArgs& m) {
-  MgArgs m0 = m;
+template <class R>
+static MgArgsT<R> mg_start_args(const PcgArgsT<R>& a, const MgArgsT<R>& m) {
+  MgArgsT<R> m0 = m;
   m0.part_rr = nullptr;
   m0.z = a.z;
   m0.part_rz = a.part_rz0;
   return m0;
 }
-MgArgs emc_mg_iter_args(const PcgArgs& a, const MgArgs& m, int it) {
-  MgArgs mi = m;
+template <class R>
+static MgArgsT<R> mg_iter_args(const PcgArgsT<R>& a, const MgArgsT<R>& m, int it) {
+  MgArgsT<R> mi = m;
   mi.part_rr = a.part_rr;
   mi.scal = a.scal;
   mi.npart = a.nblocks;
@@ -1434,24 +1496,65 @@ MgArgs emc_mg_iter_args(const PcgArgs& a, const MgArgs& m, int it) {
   return mi;
 }
 
-hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s) {
+MgArgs emc_mg_start_args(const PcgArgs& a, const MgArgs& m) { return mg_start_args(a, m); }
+MgArgs emc_mg_iter_args(const PcgArgs& a, const MgArgs& m, int it) { return mg_iter_args(a, m, it); }
+MgArgsT<float> emc_mg_start_args(const PcgArgsT<float>& a, const MgArgsT<float>& m) { return mg_start_args(a, m); }
+MgArgsT<float> emc_mg_iter_args(const PcgArgsT<float>& a, const MgArgsT<float>& m, int it) {
+  return mg_iter_args(a, m, it);
+}
+
+template <class R>
+static hipError_t launch_pcg_start(const PcgArgsT<R>& a, const MgArgsT<R>& m, hipStream_t s) {
+  using PA = PcgArgsT<R>;
+  using MA = MgArgsT<R>;
   const int g = a.nblocks;
-  hipLaunchKernelGGL(k_pcg_fix<PcgArgs>, dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_resid<PcgArgs>, dim3(g), dim3(256), 0, s, a);
-  const MgArgs m0 = emc_mg_start_args(a, m);
-  launch_vcycle<MgArgs>(m0, m0, g, 1, s);
-  hipLaunchKernelGGL(k_pcg_begin<PcgArgs>, dim3(1), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_fix<PA>, dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_resid<PA>, dim3(g), dim3(256), 0, s, a);
+  const MA m0 = mg_start_args(a, m);
+  launch_vcycle<MA, MA>(m0, m0, g, 1, s);
+  hipLaunchKernelGGL(k_pcg_begin<PA>, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s) { return launch_pcg_start(a, m, s); }
+hipError_t emc_launch_pcg_start(const PcgArgsT<float>& a, const MgArgsT<float>& m, hipStream_t s) {
+  return launch_pcg_start(a, m, s);
 }
 
 // iteration `it`: r.z of the previous iteration is in part_rz0 (even it) or
 // part_rz1; the search direction alternates between p (even it) and p2
-hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStream_t s) {
+template <class R>
+static hipError_t launch_pcg_iter(const PcgArgsT<R>& a, const MgArgsT<R>& m, int it, hipStream_t s) {
+  using PA = PcgArgsT<R>;
+  using MA = MgArgsT<R>;
   const int g = a.nblocks;
-  hipLaunchKernelGGL(k_pcg_spmvp<PcgArgs>, dim3(g), dim3(256), 0, s, a, it);
-  hipLaunchKernelGGL(k_pcg_update<PcgArgs>, dim3(g), dim3(256), 0, s, a, it);
-  const MgArgs mi = emc_mg_iter_args(a, m, it);
-  launch_vcycle<MgArgs>(mi, mi, g, 1, s);
+  hipLaunchKernelGGL(k_pcg_spmvp<PA>, dim3(g), dim3(256), 0, s, a, it);
+  hipLaunchKernelGGL(k_pcg_update<PA>, dim3(g), dim3(256), 0, s, a, it);
+  const MA mi = mg_iter_args(a, m, it);
+  launch_vcycle<MA, MA>(mi, mi, g, 1, s);
+  return hipGetLastError();
+}
+hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStream_t s) {
+  return launch_pcg_iter(a, m, it, s);
+}
+hipError_t emc_launch_pcg_iter(const PcgArgsT<float>& a, const MgArgsT<float>& m, int it, hipStream_t s) {
+  return launch_pcg_iter(a, m, it, s);
+}
+
+// ---- iterative refinement of the mixed-precision solve
+hipError_t emc_launch_refine_resid(const RefineArgs& r, hipStream_t s) {
+  const int g = r.a.nblocks;
+  hipLaunchKernelGGL(k_pcg_fix<PcgArgs>, dim3(g), dim3(256), 0, s, r.a);
+  hipLaunchKernelGGL(k_refine_resid, dim3(g), dim3(256), 0, s, r);
+  hipLaunchKernelGGL(k_refine_norms, dim3(1), dim3(256), 0, s, r);
+  return hipGetLastError();
+}
+hipError_t emc_launch_refine_add(const RefineArgs& r, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_add, dim3(grid_for((long long)r.a.n, 256, 2048)), dim3(256), 0, s, r);
+  return hipGetLastError();
+}
+hipError_t emc_launch_zero_f32(float* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_f32, dim3(grid_for((long long)n, 256, 2048)), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 
@@ -1461,7 +1564,7 @@ hipError_t emc_launch_pcg_start_b(const BatchTables& t, const MgArgs& shape, int
   const unsigned K = (unsigned)t.K;
   hipLaunchKernelGGL(k_pcg_fix<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
   hipLaunchKernelGGL(k_pcg_resid<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
-  launch_vcycle<const MgArgs*>(shape, t.mg_start, nblocks, K, s);
+  launch_vcycle<MgArgs, const MgArgs*>(shape, t.mg_start, nblocks, K, s);
   hipLaunchKernelGGL(k_pcg_begin<const PcgArgs*>, dim3(1, K), dim3(256), 0, s, t.pcg);
   return hipGetLastError();
 }
@@ -1469,7 +1572,7 @@ hipError_t emc_launch_pcg_iter_b(const BatchTables& t, const MgArgs& shape, int 
   const unsigned K = (unsigned)t.K;
   hipLaunchKernelGGL(k_pcg_spmvp<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
   hipLaunchKernelGGL(k_pcg_update<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
-  launch_vcycle<const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
+  launch_vcycle<MgArgs, const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
   return hipGetLastError();
 }
 hipError_t emc_launch_tile_sort_b(const BatchTables& t, long long nmax, hipStream_t s) {

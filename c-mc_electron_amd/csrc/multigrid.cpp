@@ -264,7 +264,25 @@ std::string mg_build(const Device& d, MgHierarchy& h, int min_nodes, int threads
               a[n * nc + ((size_t)gj * Lc.nx + gi) * Lc.nz + gk] = op.at(n, i, j, k, di, dj, dk);
             }
       }
-  if (!spd_inverse(a, nc, threads)) return "coarsest multigrid operator is not positive definite";
+  // The Galerkin product is positive semi-definite; where the interpolation of
+  // two coarse nodes sees the same unknown fine nodes only (ghost nodes past an
+  // even-sized edge next to fixed planes) it is singular in exact arithmetic, and
+  // the factorisation meets a zero pivot.  The coarsest solve only preconditions:
+  // it is then done on A_c + delta I with the smallest delta (1e-12, 1e-9, 1e-6 of
+  // the largest diagonal entry) that factorises, which keeps it SPD.
+  double dmax = 0.0;
+  for (int q = 0; q < nc; ++q) dmax = std::max(dmax, std::fabs(a[(size_t)q * nc + q]));
+  std::vector<double> a0 = a;
+  h.coarse_shift = 0.0;
+  bool ok = spd_inverse(a, nc, threads);
+  for (double f : {1e-12, 1e-9, 1e-6}) {
+    if (ok) break;
+    a = a0;
+    for (int q = 0; q < nc; ++q) a[(size_t)q * nc + q] += f * dmax;
+    h.coarse_shift = f * dmax;
+    ok = spd_inverse(a, nc, threads);
+  }
+  if (!ok) return "coarsest multigrid operator is not positive definite";
   h.cinv.swap(a);
   return "";
 }

@@ -38,6 +38,7 @@ struct MgLevel {
 struct MgHierarchy {
   std::vector<MgLevel> lv;
   std::vector<double> cinv;  // dense inverse of the coarsest operator, [n*n] row major
+  double coarse_shift = 0.0; // diagonal shift that made it factorise (0: none needed)
 };
 
 // Builds the hierarchy from the device's eliminated 7-point operator; `threads`

@@ -173,6 +173,8 @@ void start_request(emc_server* s, Req* r) {
     return;
   }
   (void)hipMemGetInfo(&f1, &tot);
+  // reported only: the budget is kept on the estimates, which do not move when
+  // memory another request released is returned late
   r->bytes = f0 > f1 ? (double)(f0 - f1) : r->est_bytes;
   r->particles = n;
 }
@@ -203,14 +205,12 @@ void run_worker(emc_server* s) {
       r->t_start = now_s() - s->t0;
       start_request(s, r);
       std::lock_guard<std::mutex> lk(s->mu);
-      s->used += r->bytes - r->est_bytes;
-      r->est_bytes = r->bytes;
       if (r->status) {
         r->state = EMC_REQ_FAILED;
         r->t_done = now_s() - s->t0;
         if (r->c) emc_destroy(r->c);
         r->c = nullptr;
-        s->used -= r->bytes;
+        s->used -= r->est_bytes;
         s->cv.notify_all();
       } else {
         r->state = EMC_REQ_RUNNING;
@@ -297,7 +297,7 @@ void run_worker(emc_server* s) {
     for (Req* r : done) {
       emc_destroy(r->c);
       r->c = nullptr;
-      s->used -= r->bytes;
+      s->used -= r->est_bytes;
       r->t_done = now_s() - s->t0;
       r->state = r->status ? EMC_REQ_FAILED : EMC_REQ_DONE;
       s->active.erase(std::find(s->active.begin(), s->active.end(), r));

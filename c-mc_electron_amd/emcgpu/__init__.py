@@ -102,7 +102,7 @@ class Engine:
     """One device description (decks) plus its GPU state."""
 
     def __init__(self, input_txt, device=0, seed=1, table_dir=TABLE_DIR, capacity_factor=1.25,
-                 poisson_rtol=1e-10, poisson_maxit=20000):
+                 poisson_rtol=1e-10, poisson_maxit=20000, poisson_fp32=False, poisson_inner_rtol=1e-5):
         self.lib = load_library()
         o = abi.Opts()
         self.lib.emc_default_opts(C.byref(o))
@@ -113,6 +113,8 @@ class Engine:
         o.capacity_factor = capacity_factor
         o.poisson_rtol = poisson_rtol
         o.poisson_maxit = poisson_maxit
+        o.poisson_fp32 = 1 if poisson_fp32 else 0
+        o.poisson_inner_rtol = poisson_inner_rtol
         st = C.c_int(0)
         self.ctx = self.lib.emc_create(os.fsencode(input_txt), C.byref(o), C.byref(st))
         if st.value != 0:
@@ -241,6 +243,17 @@ class Engine:
         self._check(self.lib.emc_poisson_solve(self.ctx, _ptr(rhs, C.c_double), _ptr(sol, C.c_double), rtol, maxit,
                                                C.byref(it), C.byref(rr)))
         return sol, it.value, rr.value
+
+    def poisson_solve_mixed(self, rhs, x0=None, rtol=1e-10, maxit_outer=20, inner_rtol=1e-5):
+        """The solve with an FP32 bulk and FP64 iterative refinement: (sol, outer
+        steps, FP32 CG iterations, relative residual)."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        sol = np.zeros(self.mesh.nnode) if x0 is None else np.array(x0, np.float64)
+        o, i = C.c_int(0), C.c_int(0)
+        rr = C.c_double(0)
+        self._check(self.lib.emc_poisson_solve_mixed(self.ctx, _ptr(rhs, C.c_double), _ptr(sol, C.c_double), rtol,
+                                                     maxit_outer, inner_rtol, C.byref(o), C.byref(i), C.byref(rr)))
+        return sol, o.value, i.value, rr.value
 
     def poisson_solve_noalloc(self, rhs, x0=None, rtol=1e-10, maxit=20000):
         """poisson_solve through emc_poisson_solve_noalloc (work vectors in the
@@ -467,6 +480,66 @@ class Batch:
         if self.b:
             self.lib.emc_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request_dict(r):
+    return {"state": r.state, "status": r.status, "converged": bool(r.converged), "windows": r.windows,
+            "id_mean": r.id_mean, "id_half_width": r.id_half_width,
+            "id_windows": [r.id_windows[q] for q in range(min(r.windows, abi.REQ_MAX_WINDOWS))],
+            "steps": r.steps, "particles": r.particles, "nodes": r.nodes, "footprint_bytes": r.footprint_bytes,
+            "t_submit": r.t_submit, "t_start": r.t_start, "t_done": r.t_done, "error": r.error.decode()}
+
+
+class Server:
+    """The serving loop (emc_server_*): submit decks, they run concurrently on the
+    GPU until their drain current reaches the requested precision."""
+
+    def __init__(self, device=0, hbm_budget_gb=32.0, max_active=64, table_dir=TABLE_DIR):
+        self.lib = load_library()
+        self._table_dir = table_dir.encode()
+        st = C.c_int(0)
+        self.s = self.lib.emc_server_create(self._table_dir, device, hbm_budget_gb * 2.0 ** 30, max_active,
+                                            C.byref(st))
+        if st.value != 0:
+            self.s = None
+            raise EmcError(st.value, "emc_server_create")
+
+    def submit(self, input_txt, seed=1, window_steps=1000, skip_windows=1, min_windows=4, max_windows=64,
+               rel_tol=0.02):
+        o = abi.RequestOpts()
+        self.lib.emc_default_request(C.byref(o))
+        o.seed, o.window_steps, o.skip_windows = seed, window_steps, skip_windows
+        o.min_windows, o.max_windows, o.rel_tol = min_windows, max_windows, rel_tol
+        rid = C.c_int64(0)
+        st = self.lib.emc_server_submit(self.s, os.fsencode(input_txt), C.byref(o), C.byref(rid))
+        if st != 0:
+            raise EmcError(st, self.lib.emc_server_last_error(self.s).decode())
+        return rid.value
+
+    def poll(self, rid):
+        r = abi.RequestResult()
+        st = self.lib.emc_server_poll(self.s, rid, C.byref(r))
+        if st != 0:
+            raise EmcError(st, "no such request")
+        return request_dict(r)
+
+    def wait(self, rid, timeout=-1.0):
+        r = abi.RequestResult()
+        st = self.lib.emc_server_wait(self.s, rid, timeout, C.byref(r))
+        if st not in (0, abi.EMC_ERR_STATE):
+            raise EmcError(st, "no such request")
+        return request_dict(r)
+
+    def close(self):
+        if self.s:
+            self.lib.emc_server_destroy(self.s)
+            self.s = None
 
     def __del__(self):
         try:

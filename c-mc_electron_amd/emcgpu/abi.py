@@ -128,6 +128,27 @@ class Opts(C.Structure):
     _fields_ = [
         ("table_dir", C.c_char_p), ("device", C.c_int), ("seed", C.c_uint64),
         ("capacity_factor", C.c_double), ("poisson_rtol", C.c_double), ("poisson_maxit", C.c_int),
+        ("poisson_fp32", C.c_int), ("poisson_inner_rtol", C.c_double),
+    ]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+REQ_MAX_WINDOWS = 64
+
+
+class RequestOpts(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("window_steps", C.c_int), ("skip_windows", C.c_int), ("min_windows", C.c_int),
+        ("max_windows", C.c_int), ("rel_tol", C.c_double),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("state", C.c_int), ("status", C.c_int), ("converged", C.c_int), ("windows", C.c_int),
+        ("id_mean", C.c_double), ("id_half_width", C.c_double), ("id_windows", C.c_double * REQ_MAX_WINDOWS),
+        ("steps", C.c_int64), ("particles", C.c_int64), ("nodes", C.c_int64), ("footprint_bytes", C.c_double),
+        ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double), ("error", C.c_char * 256),
     ]
 
 
@@ -161,6 +182,7 @@ def declare(lib):
         "emc_deposit": (C.c_int, [ctx, _i64, _i64, _i64, _i64]),
         "emc_poisson_solve": (C.c_int, [ctx, _d, _d, C.c_double, C.c_int, _i, _d]),
         "emc_poisson_solve_noalloc": (C.c_int, [ctx, _d, _d, C.c_double, C.c_int, _i, _d]),
+        "emc_poisson_solve_mixed": (C.c_int, [ctx, _d, _d, C.c_double, C.c_int, C.c_double, _i, _i, _d]),
         "emc_poisson_mg_level": (C.c_int, [ctx, C.c_int, _i, _d, _d]),
         "emc_compute_field": (C.c_int, [ctx, _d, _d]),
         "emc_step": (C.c_int, [ctx, C.c_int, P(FlyTallies)]),
@@ -185,6 +207,13 @@ def declare(lib):
         "emc_batch_timing": (C.c_int, [C.c_void_p, _d, _i]),
         "emc_batch_last_error": (C.c_char_p, [C.c_void_p]),
         "emc_batch_destroy": (None, [C.c_void_p]),
+        "emc_default_request": (None, [P(RequestOpts)]),
+        "emc_server_create": (C.c_void_p, [C.c_char_p, C.c_int, C.c_double, C.c_int, P(C.c_int)]),
+        "emc_server_submit": (C.c_int, [C.c_void_p, C.c_char_p, P(RequestOpts), P(C.c_int64)]),
+        "emc_server_poll": (C.c_int, [C.c_void_p, C.c_int64, P(RequestResult)]),
+        "emc_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, P(RequestResult)]),
+        "emc_server_last_error": (C.c_char_p, [C.c_void_p]),
+        "emc_server_destroy": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

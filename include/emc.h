@@ -202,6 +202,8 @@ typedef struct emc_opts {
   double capacity_factor;    /* particle capacity = factor * initial count      */
   double poisson_rtol;       /* relative residual target of the PCG            */
   int poisson_maxit;
+  int poisson_fp32;          /* 1: emc_step solves with emc_poisson_solve_mixed  */
+  double poisson_inner_rtol; /* tolerance of the FP32 bulk solve (1e-5)          */
 } emc_opts;
 
 typedef struct emc_ctx emc_ctx;
@@ -296,6 +298,16 @@ int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol,
  * them. */
 int emc_poisson_solve_noalloc(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
                               int *iters, double *relres);
+
+/* The same solve in mixed precision: the multigrid-preconditioned CG runs with
+ * FP32 vectors and operator (the FP32 bulk) to inner_rtol on the residual
+ * equation, and FP64 iterative refinement (r = b - A x, x += d) repeats it until
+ * the FP64 residual meets ||r|| <= rtol ||b|| (at most maxit_outer times).  The
+ * result meets the same tolerance as emc_poisson_solve.  outer: refinement
+ * steps, inner: FP32 CG iterations in all.  No counterpart in the reference
+ * (KLU in FP64, src/PoissonSolver.cpp:48-78). */
+int emc_poisson_solve_mixed(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit_outer,
+                            double inner_rtol, int *outer, int *inner, double *relres);
 
 /* Level `level` of the multigrid hierarchy used by emc_poisson_solve (no reference
  * counterpart: the reference factorises with KLU).  dims[3] = nodes along i, j, k;
