@@ -135,6 +135,12 @@ def cpu_baseline(eng, seconds, threads):
 FLY_KERNELS = ("k_fly_fast", "k_fly_slow")
 
 
+def _kernel_base_name(full):
+    """'void emc::k_fly_fast<emc::FlyArgs>(emc::FlyArgs)' -> 'k_fly_fast'."""
+    name = full.split("(")[0].split("<")[0].strip()
+    return name.split("::")[-1].split()[-1] if name else full
+
+
 def _pmc_pass(counter, args, outdir):
     """One rocprofv3 counter pass over a child run (this process has not touched
     the GPU yet).  `counter` names one counter or several separated by spaces.
@@ -157,7 +163,7 @@ def _pmc_pass(counter, args, outdir):
     out = {c: {} for c in names}
     for row in csv.DictReader(open(files[0])):
         if row["Counter_Name"] in out:
-            out[row["Counter_Name"]].setdefault(row["Kernel_Name"].split("(")[0].split("::")[-1], []).append(
+            out[row["Counter_Name"]].setdefault(_kernel_base_name(row["Kernel_Name"]), []).append(
                 float(row["Counter_Value"]))
     return out[names[0]] if len(names) == 1 else out
 
@@ -223,9 +229,25 @@ def measure_traffic(bench_args, nsteps):
     return out
 
 
-def stream_copy_gbs(mib=1024, reps=20):
+def stream_copy_gbs(mib=1024):
     """Measured device-to-device copy bandwidth (read + write bytes / time) of a
-    plain copy kernel: what HBM gives a streaming kernel on this GPU."""
+    plain copy: what HBM gives a streaming kernel on this GPU.  Timed in a child
+    process (the engine library and torch each bring a HIP runtime; the copy uses
+    torch's), see copy_probe."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-probe", str(mib)],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        print(f"[bench] stream copy child failed (rc {r.returncode}): {r.stderr[-400:]}", file=sys.stderr, flush=True)
+        return None
+    return json.loads(lines[-1])["gbs"]
+
+
+def copy_probe(mib, reps=20):
+    """A device-to-device copy of known size, repeated: the bytes the HBM counters
+    report for it calibrate the traffic counted for the engine's kernels, and its
+    time gives the streaming bandwidth (read + write bytes / time)."""
     import torch
     src = torch.ones(mib * 2 ** 20 // 8, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
@@ -238,21 +260,8 @@ def stream_copy_gbs(mib=1024, reps=20):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    torch.cuda.empty_cache()
-    return 2.0 * mib * 2 ** 20 / (ms * 1e-3) / 1e9
-
-
-def copy_probe(mib):
-    """A device-to-device copy of known size: the bytes the HBM counters should report for
-    it calibrate the traffic counted for the engine's kernels."""
-    import torch
-    src = torch.ones(mib * 2 ** 20 // 8, dtype=torch.float64, device="cuda:0")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": mib * 2 ** 20, "ok": bool(dst[-1].item() == 1.0)}))
+    print(json.dumps({"copy_probe_bytes": mib * 2 ** 20, "ok": bool(dst[-1].item() == 1.0),
+                      "gbs": 2.0 * mib * 2 ** 20 / (ms * 1e-3) / 1e9}))
 
 
 # ---------------------------------------------------------------- bias sweep
@@ -635,7 +644,8 @@ def main():
         print(f"[bench] heating phase {extras['heat_phase_ms_per_step']:.3f} ms per step", file=sys.stderr,
               flush=True)
         extras["stream_copy_gbs"] = stream_copy_gbs()
-        print(f"[bench] stream copy {extras['stream_copy_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+        if extras["stream_copy_gbs"]:
+            print(f"[bench] stream copy {extras['stream_copy_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     if not a.no_extras and not a.no_sweep:
         try:
             extras["sweep"] = sweep_throughput()

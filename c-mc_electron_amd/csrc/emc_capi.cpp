@@ -648,10 +648,20 @@ static void release_part(DevBuf& m, PartDev& P) {
 static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   Gpu& g = *c->gpu;
   if (g.cap >= cap) return EMC_OK;
-  if (g.cap > 0) return fail(c, EMC_ERR_CAPACITY, "particle arrays are already allocated with a smaller capacity");
   // particle slots are listed as 32-bit indices by the flight kernels
   if (cap > 0x7fffffffLL) return fail(c, EMC_ERR_CAPACITY, "particle capacity above 2^31-1");
   DevBuf& m = g.mem;
+  if (g.cap > 0) {
+    // a larger population replaces the current one (every caller writes all of
+    // it): the old buffers go once no queued kernel reads them
+    HIPCHECK(c, hipStreamSynchronize(g.stream));
+    release_part(m, g.P);
+    release_part(m, g.P2);
+    m.release(g.clones); m.release(g.dead); m.release(g.slow);
+    g.clones = nullptr; g.dead = nullptr; g.slow = nullptr;
+    g.n = 0;
+    g.n_pending = false;
+  }
   g.cap = cap;
   alloc_part(m, g.P, cap);
   alloc_part(m, g.P2, cap);
@@ -1636,7 +1646,36 @@ extern "C" {
 
 // ---- checkpoint: the device state a later load continues from exactly
 static const char CKPT_MAGIC[8] = {'E', 'M', 'C', 'C', 'K', 'P', 'T', '1'};
-static const uint32_t CKPT_VERSION = 3;
+static const uint32_t CKPT_VERSION = 4;
+
+// FNV-1a over what a resumed run must share with the saved one: the mesh, the
+// doping and materials, the contacts and their voltages, the time step, the
+// temperature, the refresh, statistics and physics switches and the run seed
+// (total_step and the output cadence of the heating phase may change).
+static uint64_t deck_fingerprint(const emc_ctx* c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t q = 0; q < n; ++q) { h ^= b[q]; h *= 1099511628211ull; }
+  };
+  const Device& d = c->dev;
+  const Config& f = d.cfg;
+  for (const auto* v : {&d.lx, &d.ly, &d.lz, &d.donor, &d.acceptor}) mix(v->data(), v->size() * sizeof(double));
+  mix(d.material.data(), d.material.size() * sizeof(int));
+  mix(d.motion.data(), d.motion.size());
+  for (const ContactDef& k : d.contacts) {
+    mix(&k.phims, sizeof(double));
+    mix(&k.vapp, sizeof(double));
+    mix(&k.nplane, sizeof(int));
+  }
+  const double sc[2] = {f.dt, f.temperature};
+  mix(sc, sizeof(sc));
+  const int fl[9] = {f.flag_mr, f.mr_step, f.flag_surface, f.flag_rough, f.flag_sphonon, f.flag_simp, f.flag_qc,
+                     f.stat_step, f.quantum_start_step};
+  mix(fl, sizeof(fl));
+  mix(&c->opts.seed, sizeof(c->opts.seed));
+  return h;
+}
 
 int emc_checkpoint_save(emc_ctx* c, const char* path) {
   if (!c || !path) return EMC_ERR_ARG;
@@ -1662,7 +1701,8 @@ int emc_checkpoint_save(emc_ctx* c, const char* path) {
   const uint32_t version = CKPT_VERSION;
   const int32_t dims[4] = {d.cnx, d.cny, d.cnz, 0};
   const uint64_t seed = c->opts.seed, next = c->next_step;
-  put(CKPT_MAGIC, 8); put(&version, 4); put(dims, 16); put(&seed, 8); put(&next, 8); put(&nl, 8);
+  const uint64_t fp = deck_fingerprint(c);
+  put(CKPT_MAGIC, 8); put(&version, 4); put(dims, 16); put(&seed, 8); put(&next, 8); put(&nl, 8); put(&fp, 8);
   // live particles, compacted in slot order
   auto put_live = [&](const void* dptr, size_t width) {
     std::vector<unsigned char> all((size_t)n * width), sel((size_t)nl * width);
@@ -1720,6 +1760,8 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   const Device& d = c->dev;
   const size_t NN = d.nnode;
+  // The whole file is read and checked first; the context changes only once all
+  // of it is there (a truncated or foreign file leaves the context as it was).
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(c, EMC_ERR_IO, std::string("cannot read ") + path);
   bool ok = true;
@@ -1727,38 +1769,52 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
   char magic[8];
   uint32_t version = 0;
   int32_t dims[4];
-  uint64_t seed = 0, next = 0;
+  uint64_t seed = 0, next = 0, fp = 0;
   long long nl = 0;
-  get(magic, 8); get(&version, 4); get(dims, 16); get(&seed, 8); get(&next, 8); get(&nl, 8);
+  get(magic, 8); get(&version, 4); get(dims, 16); get(&seed, 8); get(&next, 8); get(&nl, 8); get(&fp, 8);
   if (!ok || std::memcmp(magic, CKPT_MAGIC, 8) != 0 || version != CKPT_VERSION) {
     std::fclose(f);
     return fail(c, EMC_ERR_IO, std::string(path) + " is not a checkpoint of this format");
   }
-  if (dims[0] != d.cnx || dims[1] != d.cny || dims[2] != d.cnz || nl < 0) {
+  if (dims[0] != d.cnx || dims[1] != d.cny || dims[2] != d.cnz || nl < 0 || nl > 0x7fffffffLL) {
     std::fclose(f);
     return fail(c, EMC_ERR_DECK, "checkpoint mesh does not match the deck");
   }
-  const long long want = (long long)std::ceil(nl * std::max(1.0, c->opts.capacity_factor)) + 1024;
-  st = gpu_alloc_particles(c, want);
-  if (st) { std::fclose(f); return st; }
-  if (nl > g.cap) { std::fclose(f); return fail(c, EMC_ERR_CAPACITY, "checkpoint population exceeds the device capacity"); }
-  auto get_dev = [&](void* dptr, size_t bytes) {
-    std::vector<unsigned char> h(bytes);
-    get(h.data(), bytes);
-    if (ok && hipMemcpy(dptr, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
+  if (fp != deck_fingerprint(c)) {
+    std::fclose(f);
+    return fail(c, EMC_ERR_DECK, "checkpoint was saved under another deck or seed (mesh, doping, contacts, dt, "
+                                 "temperature, switches)");
+  }
+  // device destination (null: host only) and the bytes read for it, in file order
+  struct Part {
+    void** dev;
+    size_t bytes;
+    std::vector<unsigned char> h;
   };
-  for (void* a : {(void*)g.P.x, (void*)g.P.y, (void*)g.P.z, (void*)g.P.kx, (void*)g.P.ky, (void*)g.P.kz,
-                  (void*)g.P.energy, (void*)g.P.charge})
-    get_dev(a, (size_t)nl * 8);
-  get_dev(g.P.pk, (size_t)nl * 4); get_dev(g.P.itet, (size_t)nl * 4); get_dev(g.P.isym, (size_t)nl * 4);
-  get_dev(g.P.key, (size_t)nl * 8);
-  get_dev(g.pot, NN * 8);
-  get_dev(g.node_stat, 10 * NN * 8);
-  get_dev(g.heat_e, NN * 8); get_dev(g.heat_h, NN * 8);
-  get_dev(g.rcur, (size_t)d.pny * 8); get_dev(g.lcur, (size_t)d.pny * 8);
-  get_dev(g.stat_pot, NN * 8);
-  get_dev(g.stat_enum, d.ncell * 8);
-  get_dev(g.stat_heat_e, NN * 8); get_dev(g.stat_heat_h, NN * 8);
+  std::vector<Part> parts;
+  double* pot_avg_tmp = nullptr;
+  double* qc_avg_tmp = nullptr;
+  auto take = [&](void** dev, size_t bytes) {
+    Part p{dev, bytes, std::vector<unsigned char>(bytes)};
+    get(p.h.data(), bytes);
+    parts.push_back(std::move(p));
+  };
+  PartDev Pn = g.P;  // pointers bound below, once the capacity is settled
+  void* slots[12];
+  const size_t w8 = (size_t)nl * 8, w4 = (size_t)nl * 4;
+  for (int q = 0; q < 8; ++q) take(&slots[q], w8);
+  for (int q = 8; q < 11; ++q) take(&slots[q], w4);
+  take(&slots[11], w8);
+  void* dv[12];
+  dv[0] = g.pot; dv[1] = g.node_stat; dv[2] = g.heat_e; dv[3] = g.heat_h; dv[4] = g.rcur; dv[5] = g.lcur;
+  dv[6] = g.stat_pot; dv[7] = g.stat_enum; dv[8] = g.stat_heat_e; dv[9] = g.stat_heat_h;
+  take(&dv[0], NN * 8);
+  take(&dv[1], 10 * NN * 8);
+  take(&dv[2], NN * 8); take(&dv[3], NN * 8);
+  take(&dv[4], (size_t)d.pny * 8); take(&dv[5], (size_t)d.pny * 8);
+  take(&dv[6], NN * 8);
+  take(&dv[7], d.ncell * 8);
+  take(&dv[8], NN * 8); take(&dv[9], NN * 8);
   emc_fly_tallies win;
   int64_t flags[4] = {0, 0, 0, 0};
   get(&win, sizeof(win));
@@ -1768,30 +1824,46 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
     v->resize(NN);
     get(v->data(), NN * 8);
   }
-  if (ok && flags[2]) {
-    if (!g.pot_avg) g.pot_avg = g.mem.alloc<double>(NN);
-    if (!g.pot_avg) ok = false;
-    else get_dev(g.pot_avg, NN * 8);
-  }
+  void* avg_dst = nullptr;
+  if (ok && flags[2]) take(&avg_dst, NN * 8);
   int64_t has_qc = 0;
   get(&has_qc, 8);
   if (ok && has_qc != (g.qc_pot ? 1 : 0)) {
     std::fclose(f);
     return fail(c, EMC_ERR_DECK, "checkpoint and deck disagree on Flag_QuantumCorrection");
   }
+  void* qdv[3] = {g.qc_pot, g.stat_qc_pot, nullptr};
   if (ok && has_qc) {
-    get_dev(g.qc_pot, NN * 8);
-    get_dev(g.stat_qc_pot, NN * 8);
+    take(&qdv[0], NN * 8);
+    take(&qdv[1], NN * 8);
     saved.qc_pot.resize(NN);
     get(saved.qc_pot.data(), NN * 8);
-    if (ok && flags[2]) {
-      if (!g.qc_avg) g.qc_avg = g.mem.alloc<double>(NN);
-      if (!g.qc_avg) ok = false;
-      else get_dev(g.qc_avg, NN * 8);
-    }
+    if (ok && flags[2]) take(&qdv[2], NN * 8);
   }
+  const bool at_end = ok && std::fgetc(f) == EOF;
   std::fclose(f);
-  if (!ok) return fail(c, EMC_ERR_IO, std::string("reading ") + path + " failed");
+  if (!ok || !at_end) return fail(c, EMC_ERR_IO, std::string(path) + " is truncated or longer than its header says");
+  // the file is whole: now the device
+  const long long want = (long long)std::ceil(nl * std::max(1.0, c->opts.capacity_factor)) + 1024;
+  st = gpu_alloc_particles(c, want);
+  if (st) return st;
+  if (nl > g.cap) return fail(c, EMC_ERR_CAPACITY, "checkpoint population exceeds the device capacity");
+  Pn = g.P;
+  void* pdst[12] = {Pn.x, Pn.y, Pn.z, Pn.kx, Pn.ky, Pn.kz, Pn.energy, Pn.charge, Pn.pk, Pn.itet, Pn.isym, Pn.key};
+  for (int q = 0; q < 12; ++q) slots[q] = pdst[q];
+  if (flags[2]) {
+    if (!g.pot_avg) g.pot_avg = pot_avg_tmp = g.mem.alloc<double>(NN);
+    avg_dst = g.pot_avg;
+    if (has_qc) {
+      if (!g.qc_avg) g.qc_avg = qc_avg_tmp = g.mem.alloc<double>(NN);
+      qdv[2] = g.qc_avg;
+    }
+    if (!g.pot_avg || (has_qc && !g.qc_avg)) return fail(c, EMC_ERR_HIP, "heat-phase buffers");
+  }
+  (void)pot_avg_tmp;
+  (void)qc_avg_tmp;
+  for (const Part& p : parts)
+    if (p.bytes) HIPCHECK(c, hipMemcpy(*p.dev, p.h.data(), p.bytes, hipMemcpyHostToDevice));
   g.n = nl;
   g.n_pending = false;
   c->next_step = next;

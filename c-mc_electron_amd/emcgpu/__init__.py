@@ -578,6 +578,15 @@ def read_current(path):
     return out
 
 
+# two-sided 95 % Student-t quantiles, 1 .. 30 degrees of freedom
+_T975 = (12.706, 4.303, 3.182, 2.776, 2.571, 2.447, 2.365, 2.306, 2.262, 2.228, 2.201, 2.179, 2.160, 2.145, 2.131,
+         2.120, 2.110, 2.101, 2.093, 2.086, 2.080, 2.074, 2.069, 2.064, 2.060, 2.056, 2.052, 2.048, 2.045, 2.042)
+
+
+def _t975(df):
+    return _T975[df - 1] if df <= 30 else 1.959963984540054 + 2.4 / df
+
+
 def current_estimate(path, contacts=(0, 1), skip=1, level=0.95):
     """Drain-current estimate from data/current with a confidence interval.
 
@@ -600,8 +609,10 @@ def current_estimate(path, contacts=(0, 1), skip=1, level=0.95):
     try:
         from scipy.stats import t as student_t
         q = float(student_t.ppf(0.5 + level / 2, n - 1))
-    except ImportError:  # normal quantile for 95 %
-        q = 1.959963984540054
+    except ImportError:
+        if level != 0.95:
+            raise ValueError("without scipy only the 95 % interval is available")
+        q = _t975(n - 1)
     d = x - mean
     lag1 = float((d[1:] * d[:-1]).sum() / (d * d).sum()) if n > 2 and (d * d).sum() > 0 else float("nan")
     return {"mean": mean, "half_width": q * sd / np.sqrt(n), "windows": n, "lag1": lag1}
