@@ -1307,6 +1307,29 @@ int orc_get_state(const emc_band_view *b, uint64_t key, uint64_t step, double en
   return s->Flag_SelfScatter;
 }
 
+/* Test hook: one EleImpScUpdateK (src/ElectronImpurityScatter.cpp:188-306) on an
+ * electron at wedge-frame k with symmetry isym, energy `energy` and cell charge
+ * density rho, draws from the stream of (key, step).  Returns 1 when the draw is
+ * accepted (k, isym, itet, energy take the new state) and 0 on self-scattering. */
+int orc_imp_update_k(const emc_band_view *b, const emc_mesh_view *m, uint64_t key, uint64_t step, double k[3],
+                     int *isym, int *itet, double *energy, double rho) {
+  emc_fly_tallies t;
+  memset(&t, 0, sizeof(t));
+  FS st, *s = &st;
+  memset(s, 0, sizeof(*s));
+  s->b = b; s->m = m; s->t = &t;
+  rng_init(&s->rng, key, step);
+  s->kx = k[0]; s->ky = k[1]; s->kz = k[2];
+  s->isym = *isym; s->itet = *itet; s->energy = *energy; s->Rho = rho;
+  s->par_type = PELEC;
+  s->iband = B->ibt[*itet];
+  EleImpScUpdateK(s);
+  if (s->Flag_SelfScatter || s->Flag_Catch) return 0;
+  k[0] = s->kx; k[1] = s->ky; k[2] = s->kz;
+  *isym = s->isym; *itet = s->itet; *energy = s->energy;
+  return 1;
+}
+
 int orc_getisym(const emc_band_view *b, const double k[3], double energy, int iband, int itet_hint, int *isym,
                 int *itet) {
   emc_fly_tallies t;

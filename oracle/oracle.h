@@ -65,6 +65,11 @@ int orc_get_state(const emc_band_view *b, uint64_t key, uint64_t step, double en
                   int isym, double k[3], int *itet, int *ndraws);
 int orc_getisym(const emc_band_view *b, const double k[3], double energy, int iband, int itet_hint,
                 int *isym, int *itet);
+/* EleImpScUpdateK (src/ElectronImpurityScatter.cpp:188-306) on one electron with
+ * cell charge density rho; returns 1 and the new state when accepted, 0 when the
+ * draw self-scatters. */
+int orc_imp_update_k(const emc_band_view *b, const emc_mesh_view *m, uint64_t key, uint64_t step, double k[3],
+                     int *isym, int *itet, double *energy, double rho);
 
 /* ---- deposit: statistic + particle_to_density + compute_cell_charge
  * (src/mcmodel.cpp:1167-1408, 348-417, 2664-2712); outputs as emc_deposit. */

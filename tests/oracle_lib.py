@@ -41,6 +41,8 @@ def load():
         "orc_get_state": (C.c_int, [P(abi.BandView), C.c_uint64, C.c_uint64, C.c_double, C.c_int, C.c_int, d,
                                     P(C.c_int), P(C.c_int)]),
         "orc_getisym": (C.c_int, [P(abi.BandView), d, C.c_double, C.c_int, C.c_int, P(C.c_int), P(C.c_int)]),
+        "orc_imp_update_k": (C.c_int, [P(abi.BandView), P(abi.MeshView), C.c_uint64, C.c_uint64, d, P(C.c_int),
+                                       P(C.c_int), P(C.c_double), C.c_double]),
         "orc_multiple_refresh": (C.c_int64, [P(abi.MeshView), P(abi.Particles), C.c_int64, P(C.c_int32),
                                              P(C.c_int32), C.c_uint64, P(abi.Particles), C.c_int64]),
         "orc_mr_stream_key": (C.c_uint64, [C.c_int64, C.c_int]),

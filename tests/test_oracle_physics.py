@@ -338,3 +338,86 @@ def test_bulk_mobility_and_saturation_velocity(bulk):
     assert 400 < mu_h < 560, mu_h
     assert 0.85e7 < -res[1e7][0] < 1.1e7, res[1e7]
     assert 5.5e6 < res[1e7][1] < 8.0e6, res[1e7]
+
+
+def test_impurity_final_state_follows_the_screened_coulomb_matrix_element(band):
+    """EleImpScUpdateK (src/ElectronImpurityScatter.cpp:188-306) picks the final
+    state of an elastic Brooks-Herring impurity collision in a Delta valley.  In
+    the Herring-Vogt frame of the valley (k' = ((kx - 0.85 a0pi) sqrt(md/ml),
+    ky sqrt(md/mt), kz sqrt(md/mt))) the constant-energy surface is the sphere
+    |k'| = const, so:
+    - the collision keeps |k'| (elastic);
+    - the final direction n on that sphere has the density of the screened
+      Coulomb matrix element, |M|^2 ~ 1/(beta^2 + q^2)^2, with q the wave-vector
+      transfer in the crystal frame, q^2 = (ml/md) dk'x^2 + (mt/md)(dk'y^2 + dk'z^2)
+      (ml, mt the longitudinal and transverse masses), uniform measure on the
+      sphere.  Chi-square over 10 x 8 bins of (n_x, azimuth about x), p > 1e-4.
+    The same counts reject the isotropic-mass density (q^2 = (mt/md)|dk'|^2, what
+    the draw gives without its rejection step) at p < 1e-8."""
+    lib = oracle_lib.load()
+    v = band["_view"]
+    eng = band["_eng"]
+    m = eng.mesh
+    ml, mt, md, a0pi = v.mell, v.melt, v.meld, v.a0pi
+    E, rho = 3.0, 1e-3  # 78 meV; 2e23 m^-3 of charge: forward-peaked, anisotropic
+    beta2 = m.frickel * 4 * np.pi * rho / m.eps_si
+    k0 = np.zeros(3)
+    it0, nd = C.c_int(0), C.c_int(0)
+    assert lib.orc_get_state(C.byref(v), 0xC0FFEE, 3, E, 0, 0, k0.ctypes.data_as(C.POINTER(C.c_double)),
+                             C.byref(it0), C.byref(nd)) == 0
+    sc = np.array([np.sqrt(md / ml), np.sqrt(md / mt), np.sqrt(md / mt)])
+
+    def hv(k):
+        return (np.asarray(k) - [0.85 * a0pi, 0, 0]) * sc
+
+    kp0 = hv(k0)
+    r0 = np.linalg.norm(kp0)
+    assert abs(kp0[0]) < 0.3 * a0pi * sc[0], "initial state outside the +x Delta valley"
+    dirs = []
+    k = np.zeros(3)
+    kp = k.ctypes.data_as(C.POINTER(C.c_double))
+    isym, itet, en = C.c_int(0), C.c_int(0), C.c_double(0)
+    s = 0
+    while len(dirs) < 20000:
+        k[:] = k0
+        isym.value, itet.value, en.value = 0, it0.value, E
+        ok = lib.orc_imp_update_k(C.byref(v), C.byref(m), 0x5EED0000 + s, 11, kp, C.byref(isym), C.byref(itet),
+                                  C.byref(en), rho)
+        s += 1
+        if not ok:
+            continue
+        kn = hv(k)
+        assert abs(np.linalg.norm(kn) - r0) <= 1e-12 * r0
+        dirs.append(kn / r0)
+    dirs = np.array(dirs)
+    n0 = kp0 / r0
+    nu, npsi = 10, 8
+    iu = np.minimum(((dirs[:, 0] + 1) / 2 * nu).astype(int), nu - 1)
+    ipsi = np.minimum(((np.arctan2(dirs[:, 2], dirs[:, 1]) + np.pi) / (2 * np.pi) * npsi).astype(int), npsi - 1)
+    counts = np.bincount(iu * npsi + ipsi, minlength=nu * npsi).astype(float)
+
+    def expected(mlong):
+        # integrate the density over each (n_x, azimuth) bin on a fine grid
+        fu, fp = 400, 320
+        u = -1 + (np.arange(nu * fu) + 0.5) * 2 / (nu * fu)
+        psi = -np.pi + (np.arange(npsi * fp) + 0.5) * 2 * np.pi / (npsi * fp)
+        U, P = np.meshgrid(u, psi, indexing="ij")
+        st = np.sqrt(1 - U * U)
+        n = np.stack([U, st * np.cos(P), st * np.sin(P)], axis=-1)
+        d = r0 * (n0 - n)
+        q2 = mlong / md * d[..., 0] ** 2 + mt / md * (d[..., 1] ** 2 + d[..., 2] ** 2)
+        f = 1.0 / (beta2 + q2) ** 2
+        w = f.reshape(nu, fu, npsi, fp).sum(axis=(1, 3)).ravel()
+        return len(dirs) * w / w.sum()
+
+    def pval(exp):
+        keep = exp >= 5
+        o, e = counts[keep], exp[keep]
+        o = np.append(o, counts[~keep].sum())
+        e = np.append(e, exp[~keep].sum())
+        if e[-1] == 0:
+            o, e = o[:-1], e[:-1]
+        return stats.chi2.sf(((o - e) ** 2 / e).sum(), len(o) - 1)
+
+    assert pval(expected(ml)) > 1e-4
+    assert pval(expected(mt)) < 1e-8
