@@ -674,6 +674,47 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
   return EMC_OK;
 }
 
+// Keeps room for a step's clones: once the population comes within one clone
+// buffer of its slots, the particles move into buffers of 1.5x the population
+// (the reference's cell lists have no limit).  Between steps only: the flight
+// appends a step's surviving clones up to the capacity and counts the rest as
+// drops (reason 8).  The stream must have passed every kernel that reads the
+// particles, and g.n must be settled.
+static int grow_if_full(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  if (g.n <= g.cap - (long long)g.clone_cap) return EMC_OK;
+  const long long cap = (long long)std::ceil(1.5 * (double)g.n) + 4096;
+  if (cap > 0x7fffffffLL) return fail(c, EMC_ERR_CAPACITY, "particle capacity above 2^31-1");
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  DevBuf& m = g.mem;
+  PartDev out{}, spare{};
+  alloc_part(m, out, cap);
+  alloc_part(m, spare, cap);
+  const unsigned long long clone_cap = (unsigned long long)std::max<long long>(4096, cap / 16);
+  CloneRec* clones = m.alloc<CloneRec>(clone_cap);
+  long long* dead = m.alloc<long long>(clone_cap);
+  uint32_t* slow = m.alloc<uint32_t>(cap);
+  if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("particle allocation: ") + hipGetErrorString(m.last));
+  const size_t n = (size_t)g.n;
+  const std::pair<const void*, void*> f8[] = {{g.P.x, out.x}, {g.P.y, out.y}, {g.P.z, out.z}, {g.P.kx, out.kx},
+                                              {g.P.ky, out.ky}, {g.P.kz, out.kz}, {g.P.energy, out.energy},
+                                              {g.P.charge, out.charge}, {g.P.key, out.key}};
+  for (const auto& pr : f8) HIPCHECK(c, hipMemcpyAsync(pr.second, pr.first, n * 8, hipMemcpyDeviceToDevice, g.stream));
+  const std::pair<const void*, void*> f4[] = {{g.P.pk, out.pk}, {g.P.itet, out.itet}, {g.P.isym, out.isym}};
+  for (const auto& pr : f4) HIPCHECK(c, hipMemcpyAsync(pr.second, pr.first, n * 4, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHECK(c, hipStreamSynchronize(g.stream));
+  release_part(m, g.P);
+  release_part(m, g.P2);
+  m.release(g.clones); m.release(g.dead); m.release(g.slow);
+  g.P = out;
+  g.P2 = spare;
+  g.cap = cap;
+  g.clones = clones; g.clone_cap = clone_cap;
+  g.dead = dead; g.dead_cap = clone_cap;
+  g.slow = slow;
+  return EMC_OK;
+}
+
 // MultipleRefresh target counts per cell (src/particle_init.cpp:44-70): the
 // deck's per-cell numbers, raised in silicon cells to the initial particle
 // count of the cell and to the default_par_number floor.
@@ -2283,6 +2324,12 @@ static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double ts
   Gpu& g = *c->gpu;
   const size_t NN = c->dev.nnode;
   const Config& cf = c->dev.cfg;
+  {
+    int st = settle_n(c);
+    if (st) return st;
+    st = grow_if_full(c);
+    if (st) return st;
+  }
   const uint64_t count = heat_phase ? c->heat_next++ : c->next_step++;
   const uint64_t step = heat_phase ? HEAT_STEP0 + count : count;
   HIPCHECK(c, hipEventRecord(g.ev[0], g.stream));
@@ -2553,6 +2600,8 @@ static int batch_one_step(emc_batch* b, emc_fly_tallies* acc, double tsum[5], in
     Gpu& g = *c->gpu;
     step[q] = c->next_step++;
     int st = settle_n(c);
+    if (st) return bfail(b, st, c->err);
+    st = grow_if_full(c);
     if (st) return bfail(b, st, c->err);
     SortArgs& a = hs[q];
     a = SortArgs{};

@@ -122,15 +122,16 @@ double estimate_bytes(const emc_mesh_view& m, const std::string& deck, double ca
 }
 
 void finish_window(Req* r) {
-  double cur = 0;
-  int nc = 0;
-  for (int q = 0; q < 2; ++q) {
+  // output_stat's Current of contact q (src/mcmodel.cpp:811)
+  auto current = [&](int q) {
     const double gen = (double)r->acc.contact[q][EMC_CT_QGEN] / EMC_FX_CONTACT;
     const double cat = (double)r->acc.contact[q][EMC_CT_QCATCH] / EMC_FX_CONTACT;
-    cur += std::fabs((gen - cat) / (r->dt * r->o.window_steps) * r->curr0 * 1e-2);
-    ++nc;
-  }
-  r->win.push_back(cur / nc);
+    return (gen - cat) / (r->dt * r->o.window_steps) * r->curr0 * 1e-2;
+  };
+  if (r->o.drain_contact >= 0)
+    r->win.push_back(current(r->o.drain_contact));
+  else
+    r->win.push_back(0.5 * (std::fabs(current(0)) + std::fabs(current(1))));
   std::memset(&r->acc, 0, sizeof(r->acc));
   r->win_step = 0;
   const int skip = std::max(0, r->o.skip_windows);
@@ -144,7 +145,7 @@ void finish_window(Req* r) {
   const double sd = n > 1 ? std::sqrt(v / (n - 1)) : INFINITY;
   r->mean = m;
   r->half = n > 1 ? t975(n - 1) * sd / std::sqrt((double)n) : INFINITY;
-  r->converged = n >= std::max(2, r->o.min_windows) && r->half <= r->o.rel_tol * std::fabs(m);
+  r->converged = n >= std::max(2, r->o.min_windows) && r->half <= std::max(r->o.rel_tol * std::fabs(m), r->o.abs_tol);
 }
 
 void fail_req(Req* r, int st, const std::string& why) {
@@ -319,6 +320,9 @@ void emc_default_request(emc_request_opts* r) {
   r->min_windows = 4;
   r->max_windows = 64;
   r->rel_tol = 0.02;
+  r->abs_tol = 0.0;
+  r->drain_contact = 1;
+  r->pad0 = 0;
 }
 
 emc_server* emc_server_create(const char* table_dir, int device, double hbm_budget_bytes, int max_active,
@@ -347,7 +351,8 @@ int emc_server_submit(emc_server* s, const char* input_txt, const emc_request_op
   emc_request_opts o;
   if (ro) o = *ro;
   else emc_default_request(&o);
-  if (o.window_steps < 1 || o.max_windows < 1 || o.rel_tol < 0) return EMC_ERR_ARG;
+  if (o.window_steps < 1 || o.max_windows < 1 || o.rel_tol < 0 || o.abs_tol < 0 || o.drain_contact >= EMC_MAX_CONTACT)
+    return EMC_ERR_ARG;
   // the deck is read now, on the host (mesh and footprint), and the request queued
   emc_opts ho = s->base;
   ho.device = -1;

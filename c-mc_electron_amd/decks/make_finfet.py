@@ -28,9 +28,12 @@ def grid_axis(lo, hi, refine):
 
 def write_decks(out, refine=1, electrons=20000, holes=2000, vs=0.0, vd=0.5, vg=0.7,
                 total_step=2000, dt=1e-16, stat_step=1000, heat=0, heat_steps=0,
-                surface=1, qc=0, quantum_start_step=20, lg=15, wfin=5):
+                surface=1, qc=0, quantum_start_step=20, lg=15, wfin=5, phims=-0.46, mr=0, mr_step=10):
     """lg: gate length and wfin: fin width, whole nm (defaults 15 and 5: the
-    device above); the source stays 13 nm and the drain 12 nm long."""
+    device above); the source stays 13 nm and the drain 12 nm long.  phims: the
+    gate's work-function difference, the second number of the gate contact (the
+    reference's FinFET decks use -0.46 V).  mr, mr_step: Flag_MultipleRefresh
+    and its period."""
     os.makedirs(out, exist_ok=True)
     y0, g0 = -20, -7
     g1 = g0 + lg
@@ -99,7 +102,7 @@ contact
   19 19 {y0} {y1} -3 {zt}
   0
 contact
-  3 -0.46
+  3 {phims:g}
   -1 -1 {g0} {g1} -1 {w1}
   -1 10 {g0} {g1} -1 -1
   -1 10 {g0} {g1} {w1} {w1}
@@ -120,10 +123,10 @@ stat_heat_step={stat_step}
 heat_steps={heat_steps}
 Flag_QuantumCorrection={qc}
 quantum_start_step={quantum_start_step}
-Flag_MultipleRefresh=0
+Flag_MultipleRefresh={mr}
 Flag_compute_heat={heat}
 Flag_SurfaceScatter={surface}
-mr_step=10
+mr_step={mr_step}
 """
     with open(os.path.join(out, "input.txt"), "w") as f:
         f.write(inp)

@@ -511,11 +511,12 @@ class Server:
             raise EmcError(st.value, "emc_server_create")
 
     def submit(self, input_txt, seed=1, window_steps=1000, skip_windows=1, min_windows=4, max_windows=64,
-               rel_tol=0.02):
+               rel_tol=0.02, abs_tol=0.0, drain_contact=1):
         o = abi.RequestOpts()
         self.lib.emc_default_request(C.byref(o))
         o.seed, o.window_steps, o.skip_windows = seed, window_steps, skip_windows
         o.min_windows, o.max_windows, o.rel_tol = min_windows, max_windows, rel_tol
+        o.abs_tol, o.drain_contact = abs_tol, drain_contact
         rid = C.c_int64(0)
         st = self.lib.emc_server_submit(self.s, os.fsencode(input_txt), C.byref(o), C.byref(rid))
         if st != 0:

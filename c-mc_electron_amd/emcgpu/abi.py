@@ -139,7 +139,8 @@ REQ_MAX_WINDOWS = 64
 class RequestOpts(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("window_steps", C.c_int), ("skip_windows", C.c_int), ("min_windows", C.c_int),
-        ("max_windows", C.c_int), ("rel_tol", C.c_double),
+        ("max_windows", C.c_int), ("rel_tol", C.c_double), ("abs_tol", C.c_double), ("drain_contact", C.c_int),
+        ("pad0", C.c_int),
     ]
 
 

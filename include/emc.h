@@ -346,10 +346,13 @@ void emc_batch_destroy(emc_batch *b);
  * steps every running request: those with the same mesh dimensions as one batch
  * (emc_batch), different meshes at the same time on their own streams.  A request
  * runs windows of window_steps steps; each window gives one drain current, the
- * mean over contacts 0 and 1 of |Current| as output_stat prints it
- * (src/mcmodel.cpp:756-842).  After skip_windows windows it is done when the 95 %
- * Student-t interval of the mean over the later windows is within rel_tol of the
- * mean (and at least min_windows of them), or after max_windows windows.  The
+ * Current output_stat prints for contact drain_contact (src/mcmodel.cpp:756-842;
+ * contact 1, the drain of the reference's FinFET decks, whose finfet/Idrain.png
+ * plots it), or with drain_contact < 0 the mean over contacts 0 and 1 of
+ * |Current| (finfet/scripts/current.py).  After skip_windows windows it is done
+ * when the 95 % Student-t half-width of the mean over the later windows is within
+ * max(rel_tol |mean|, abs_tol) (and at least min_windows of them), or after
+ * max_windows windows.  The
  * result depends only on the deck, seed and window settings, never on the other
  * requests: the same alone and in the mix, bit for bit.  Times are seconds since
  * emc_server_create. */
@@ -362,6 +365,9 @@ typedef struct emc_request_opts {
   uint64_t seed;
   int window_steps, skip_windows, min_windows, max_windows;
   double rel_tol;                 /* 0.02: +-2 % at 95 % confidence */
+  double abs_tol;                 /* A; 0: relative precision only    */
+  int drain_contact;              /* 1; < 0: current.py's mean of contacts 0, 1 */
+  int pad0;
 } emc_request_opts;
 typedef struct emc_request_result {
   int state, status, converged, windows;

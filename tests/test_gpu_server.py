@@ -19,7 +19,8 @@ W, NWIN = 40, 4
 
 
 def run_windows_direct(inp, seed):
-    """Window currents of the deck stepped through the engine (one device)."""
+    """Window currents of the deck stepped through the engine (one device): the
+    drain contact's (contact 1) Current as output_stat prints it."""
     e = emcgpu.Engine(inp, device=0, seed=seed)
     e.particles_init()
     m = e.mesh
@@ -27,12 +28,9 @@ def run_windows_direct(inp, seed):
     for _ in range(NWIN):
         t = e.step(W)
         c = emcgpu.tallies_dict(t)["contact"]
-        cur = 0.0
-        for q in range(2):
-            gen = float(c[q, 1]) / emcgpu.abi.FX_CONTACT
-            cat = float(c[q, 4]) / emcgpu.abi.FX_CONTACT
-            cur += abs((gen - cat) / (m.dt * W) * m.curr0 * 1e-2)
-        out.append(cur / 2)
+        gen = float(c[1, 1]) / emcgpu.abi.FX_CONTACT
+        cat = float(c[1, 4]) / emcgpu.abi.FX_CONTACT
+        out.append((gen - cat) / (m.dt * W) * m.curr0 * 1e-2)
     e.close()
     return out
 
