@@ -349,28 +349,31 @@ def sweep_throughput(steps=60, solo_points=8, solo_steps=15):
             "batched_particle_steps_per_s": sum(nparts) / t_batch, "batched_phase_ms": list(ms)}
 
 
-def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=16, rel=0.02, floor_frac=0.25,
-                    solo_points=8, solo_steps=15):
+def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=30, rel=0.02, floor_frac=0.25,
+                    per_stream=10, max_streams=12, solo_points=8, solo_steps=15):
     """The 64-point sweep run to a drain current known to +-rel at 95 %.
 
-    A point's Id is the mean over its windows of `window` steps, after
-    `relax_steps` and `skip` more windows, of the drain contact's current
-    (window_current).  The point is settled once the Student-t 95 % half-width of
-    that mean is within rel of max(|Id|, floor_frac x the sweep's largest Id): a
-    point below a quarter of the on-current is held to 0.5 % of the on-current,
-    the precision that resolves it on the sweep's transfer curve (the off-state
-    current is thermal noise around a value far below it).  The batch steps the
-    unsettled points together and drops each point once it has settled; a point
-    that has not settled after max_windows windows is given the window count its
-    measured window spread needs, ceil((1.96 sd / (rel scale))^2), at the step
-    time of the last batch.
+    A point's Id is the mean of the drain contact's current (window_current) over
+    windows of `window` steps, each window taken after the stream's relaxation
+    (`relax_steps`) and `skip` more windows.  The point is settled once the
+    Student-t 95 % half-width of that mean is within rel of
+    max(|Id|, floor_frac x the sweep's largest Id): a point below a quarter of the
+    on-current is held to 0.5 % of it.
 
-    Each point evolves bit-identically in the batch and alone, so the plain
-    one-at-a-time GPU path needs the same windows per point and carries the same
-    populations; its time is the sum over points and windows of the window's
-    steps times the solo step time at the point's particle count then (recorded
-    each window), the solo step time measured on a spread of points after the
-    run and fitted linear in the particle count."""
+    The batch steps the unsettled points' streams together; a point's streams
+    leave the batch once it has settled.  A point whose window spread asks for
+    many more windows (an off-state point, whose current is thermal noise around
+    zero) gets independent replicas: the same deck with other seeds, sampled and
+    relaxed in the batch, about one stream per `per_stream` windows still needed
+    (at most max_streams); its windows are pooled over its streams.  A point not
+    settled after max_windows batch windows is given the windows its spread asks
+    for, at the step time of the last batch.
+
+    The plain path runs each point alone, one stream, for the same pooled number
+    of windows after its relaxation and skipped windows; its time sums, per point
+    and window, the window's steps times the solo step time at the point's
+    particle count then (the solo step time measured on a spread of points after
+    the run, fitted linear in the particle count)."""
     from scipy import stats as st
     tmp = tempfile.mkdtemp(prefix="emc_sweep_")
     pts = sweep_points()
@@ -380,19 +383,24 @@ def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=16, rel=0
     nparts = [e.count() for e in engs]
     print(f"[bench] sweep: {len(engs)} points set up in {time.perf_counter() - t_start:.1f} s", file=sys.stderr,
           flush=True)
+    relax_w = relax_steps // window
+    # a stream: its engine, its point, and the batch window at which its windows start counting
+    streams = [{"e": engs[q], "q": q, "count_from": skip + 1, "origin": True} for q in range(len(pts))]
     b = emcgpu.Batch(engs)
     t_run = time.perf_counter()
     for r0 in range(0, relax_steps, 500):
         b.step(min(500, relax_steps - r0))
         print(f"[bench] sweep relax: {min(relax_steps, r0 + 500)} of {relax_steps} steps", file=sys.stderr, flush=True)
     wins = [[] for _ in pts]
-    counts = [[nparts[q], engs[q].count()] for q in range(len(pts))]  # at the start, after the relaxation
+    counts = [[nparts[q], engs[q].count()] for q in range(len(pts))]
     settled_at = [None] * len(pts)
-    active = list(range(len(pts)))
+    nrep = [0] * len(pts)
+    active = list(streams)
     nwin = 0
     last_ms = None
+    imax = None
 
-    def half_width(q, imax):
+    def spread(q):
         x = np.array(wins[q])
         sd = x.std(ddof=1)
         scale = max(abs(x.mean()), floor_frac * imax)
@@ -403,39 +411,67 @@ def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=16, rel=0
         acc = b.step(window)
         last_ms = (time.perf_counter() - t0) * 1e3 / window
         nwin += 1
-        if nwin <= skip:
-            print(f"[bench] sweep window {nwin} (skipped)", file=sys.stderr, flush=True)
-            continue
-        for q in active:
-            counts[q].append(engs[q].count())
-        for q, t in zip(active, acc):
-            wins[q].append(window_current(t, m, window))
+        for sm, t in zip(active, acc):
+            if sm["origin"]:
+                counts[sm["q"]].append(sm["e"].count())
+            if nwin >= sm["count_from"]:
+                wins[sm["q"]].append(window_current(t, m, window))
         if nwin < skip + 3:
             print(f"[bench] sweep window {nwin}", file=sys.stderr, flush=True)
             continue
-        imax = max(abs(np.mean(w)) for w in wins)
-        for q in active:
-            hw, scale, _ = half_width(q, imax)
-            if hw <= rel * scale:
-                settled_at[q] = nwin
-        still = [q for q in active if settled_at[q] is None]
-        print(f"[bench] sweep window {nwin}: {len(pts) - len(still)} of {len(pts)} points within {rel:.0%}, "
-              f"{last_ms:.2f} ms per batched step of {len(active)}", file=sys.stderr, flush=True)
-        if still and len(still) < len(active):
+        imax = max(abs(np.mean(w)) for w in wins if len(w))
+        pending = sorted({sm["q"] for sm in active})
+        for q in pending:
+            if len(wins[q]) >= 3:
+                hw, scale, _ = spread(q)
+                if hw <= rel * scale:
+                    settled_at[q] = nwin
+        still = [sm for sm in active if settled_at[sm["q"]] is None]
+        # replicas for the points whose spread asks for many more windows
+        added = []
+        for q in sorted({sm["q"] for sm in still}):
+            if nrep[q] or len(wins[q]) < 3:
+                continue
+            _, scale, sd = spread(q)
+            more = int(np.ceil((1.96 * sd / (rel * scale)) ** 2)) - len(wins[q])
+            want = min(max_streams, int(np.ceil(more / per_stream))) - 1
+            vg, vd, ne = pts[q]
+            for r in range(max(0, want)):
+                inp = os.path.join(tmp, f"pt{q}", "input.txt")
+                e = emcgpu.Engine(inp, device=0, seed=100 + 1000 * (r + 1) + q)
+                e.particles_init()
+                added.append({"e": e, "q": q, "count_from": nwin + relax_w + skip + 1, "origin": False})
+            nrep[q] = max(0, want)
+        n_points = len({sm["q"] for sm in still})
+        print(f"[bench] sweep window {nwin}: {len(pts) - n_points} of {len(pts)} points within {rel:.0%}, "
+              f"{last_ms:.2f} ms per batched step of {len(active)} streams, {len(added)} replicas added",
+              file=sys.stderr, flush=True)
+        if still != active or added:
             b.close()
-            b = emcgpu.Batch([engs[q] for q in still])
-        active = still
-    b.close()
+            for sm in active:
+                if settled_at[sm["q"]] is not None and not sm["origin"]:
+                    sm["e"].close()
+            active = still + added
+            if active:
+                b = emcgpu.Batch([sm["e"] for sm in active])
+    if active:
+        b.close()
     wall_batched = time.perf_counter() - t_run
-    imax = max(abs(np.mean(w)) for w in wins)
-    need = []
+    imax = max(abs(np.mean(w)) for w in wins if len(w))
+    need = []   # pooled windows each point needs
+    extra_s = 0.0
     for q in range(len(pts)):
         if settled_at[q] is not None:
-            need.append(settled_at[q])
+            need.append(len(wins[q]))
         else:
-            _, scale, sd = half_width(q, imax)
-            need.append(skip + max(nwin - skip + 1, int(np.ceil((1.96 * sd / (rel * scale)) ** 2))))
-    extra_s = (max(need) - nwin) * window * last_ms * 1e-3 if active else 0.0
+            _, scale, sd = spread(q)
+            nq = max(len(wins[q]) + 1, int(np.ceil((1.96 * sd / (rel * scale)) ** 2)))
+            need.append(nq)
+            nstreams = 1 + nrep[q]
+            extra_s = max(extra_s, (nq - len(wins[q])) / nstreams * window * last_ms * 1e-3)
+    for sm in active:
+        if not sm["origin"]:
+            sm["e"].close()
     # one at a time: a spread of points, each stepped alone
     sel = list(range(0, len(engs), max(1, len(engs) // solo_points)))[:solo_points]
     solo = []
@@ -449,28 +485,28 @@ def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=16, rel=0
         e.close()
 
     def solo_s(q):
-        # relaxation at the mean of its first and last count, each counted
-        # window at the count it ended with, the extrapolated ones at the last
+        # relaxation at the mean of its first and last count, the skipped windows at
+        # the count after relaxation, then the pooled windows one after another at
+        # the counts the point's own stream had (its last count beyond them)
         c = counts[q]
         t = relax_steps * float(np.polyval(fit, 0.5 * (c[0] + c[1])))
-        run = c[2:]
-        t += window * sum(float(np.polyval(fit, n)) for n in run)
         t += window * skip * float(np.polyval(fit, c[1]))
-        t += window * max(0, need[q] - skip - len(run)) * float(np.polyval(fit, c[-1]))
+        run = c[2 + skip:]
+        for w in range(need[q]):
+            t += window * float(np.polyval(fit, run[w] if w < len(run) else c[-1]))
         return t * 1e-3
 
     ids = [float(np.mean(w)) for w in wins]
     t_batch_s = wall_batched + extra_s
     t_solo_s = sum(solo_s(q) for q in range(len(pts)))
-    solo_ms = [float(np.polyval(fit, counts[q][-1])) for q in range(len(pts))]
     settled = sum(v is not None for v in settled_at)
     for q, (vg, vd, ne) in enumerate(pts):
-        print(f"[bench] point vg={vg:.3f} vd={vd:.3f}: Id {ids[q]:.4e} A, windows {need[q]}"
-              f"{'' if settled_at[q] is not None else ' (extrapolated)'}", file=sys.stderr, flush=True)
+        print(f"[bench] point vg={vg:.3f} vd={vd:.3f}: Id {ids[q]:.4e} A, {need[q]} windows over {1 + nrep[q]} "
+              f"streams{'' if settled_at[q] is not None else ' (extrapolated)'}", file=sys.stderr, flush=True)
     return {"points": len(pts), "settled_in_run": settled, "extrapolated_points": len(pts) - settled,
-            "windows_run": nwin, "window_steps": window, "relax_steps": relax_steps, "skip_windows": skip,
-            "windows_needed_max": int(max(need)), "windows_needed_median": float(np.median(need)),
-            "floor_frac": floor_frac, "solo_ms_per_step_total": float(sum(solo_ms)),
+            "batch_windows_run": nwin, "window_steps": window, "relax_steps": relax_steps, "skip_windows": skip,
+            "replicas": int(sum(nrep)), "pooled_windows_max": int(max(need)),
+            "pooled_windows_median": float(np.median(need)), "floor_frac": floor_frac,
             "batched_run_s": wall_batched, "batched_extrapolated_s": extra_s,
             "batched_sweep_s": t_batch_s, "one_at_a_time_sweep_s": t_solo_s,
             "bias_points_per_hour": len(pts) * 3600.0 / t_batch_s,
