@@ -145,6 +145,11 @@ struct Gpu {
   bool f32_ready = false;
   PcgArgsT<float> fa{};
   MgArgsT<float> fmg{};
+  // the FP32 cycle as the preconditioner of the FP64 CG: its input r32 (the
+  // residual in FP32), its output z32, and the hierarchy above with level 0
+  // reading r32 and the FP64 CG's done flag
+  float *r32 = nullptr, *z32 = nullptr;
+  MgArgsT<float> pmg{};
   double* scal_host = nullptr;        // pinned [2][8]: r.r etc. after iterations (ping-pong)
   hipEvent_t scal_ready[2] = {nullptr, nullptr};
   uint8_t* kind = nullptr;
@@ -932,6 +937,8 @@ static int gpu_sample_population(emc_ctx* c, const RestartCells* rs) {
 }
 
 // ---------------------------------------------------------------- public API
+static int ensure_f32(emc_ctx* c);
+
 extern "C" {
 
 void emc_default_opts(emc_opts* o) {
@@ -942,7 +949,7 @@ void emc_default_opts(emc_opts* o) {
   o->capacity_factor = 1.25;
   o->poisson_rtol = 1e-10;
   o->poisson_maxit = 20000;
-  o->poisson_fp32 = 0;
+  o->poisson_fp32 = 2;
   o->poisson_inner_rtol = 1e-5;
 }
 
@@ -955,6 +962,8 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
   if (!input_txt) return done(fail(c, EMC_ERR_ARG, "input path is NULL"));
   if (o) c->opts = *o;
   else emc_default_opts(&c->opts);
+  if (c->opts.poisson_fp32 < 0 || c->opts.poisson_fp32 > 2)
+    return done(fail(c, EMC_ERR_ARG, "poisson_fp32 must be 0 (FP64), 1 (FP32 bulk with FP64 refinement) or 2 (FP32 cycle)"));
   std::string err = device_parse(c->dev, input_txt);
   if (!err.empty()) return done(fail(c, err.find("cannot open") != std::string::npos ? EMC_ERR_IO : EMC_ERR_DECK, err));
   std::string tdir = c->opts.table_dir ? c->opts.table_dir : "";
@@ -983,6 +992,11 @@ emc_ctx* emc_create(const char* input_txt, const emc_opts* o, int* status) {
     // per device, with the device current
     st = emc_noalloc_setup(c->gpu->device);
     if (st) return done(fail(c, st, "emc_poisson_solve_noalloc: the scratch array's device address is not available"));
+    // the FP32 cycle's operator and hierarchy, so that no solve allocates
+    if (c->opts.poisson_fp32 == 2) {
+      st = ensure_f32(c);
+      if (st) return done(st);
+    }
   }
   return done(EMC_OK);
 }
@@ -1442,6 +1456,15 @@ static int run_pcg_with(emc_ctx* c, const PA& a, const MA& mg, double rtol, int 
 static int run_pcg(emc_ctx* c, const double* b, double* x, double rtol, int maxit, int* iters, double* relres,
                    bool defer = false) {
   Gpu& g = *c->gpu;
+  if (c->opts.poisson_fp32 == 2) {
+    // FP64 CG, FP32 cycle
+    const int st = ensure_f32(c);
+    if (st) return st;
+    PcgArgs a = pcg_args(c, b, x, rtol);
+    a.r32 = g.r32;
+    a.z32 = g.z32;
+    return run_pcg_with(c, a, g.pmg, rtol, maxit, iters, relres, defer);
+  }
   return run_pcg_with(c, pcg_args(c, b, x, rtol), g.mg, rtol, maxit, iters, relres, defer);
 }
 
@@ -1541,6 +1564,11 @@ static int ensure_f32(emc_ctx* c) {
     E.x = m.alloc<float>(D.n); E.t = m.alloc<float>(D.n); E.r = m.alloc<float>(D.n);
     E.b = l > 0 ? m.alloc<float>(D.n) : a.r;
   }
+  g.r32 = m.alloc<float>(NN);
+  g.z32 = m.alloc<float>(NN);
+  g.pmg = F;
+  g.pmg.L[0].b = g.r32;
+  g.pmg.done = g.done;
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("FP32 solver buffers: ") + hipGetErrorString(m.last));
   g.f32_ready = true;
   return EMC_OK;
@@ -2166,7 +2194,13 @@ size_t emc_poisson_ws_doubles(emc_ctx* c) {
   const Gpu& g = *c->gpu;
   const size_t NN = c->dev.nnode;
   size_t need = 7 * align32(NN) + 5 * align32(3 * (size_t)g.pcg_blocks) + align32(8) + align32(1);
-  for (int l = 0; l < g.mg.nl; ++l) need += (l == 0 ? 3 : 4) * align32(g.mg.L[l].n);
+  if (c->opts.poisson_fp32 == 2) {
+    // the FP32 cycle: r32 and z32, then x t r b per level (level 0 reads r32), in FP32
+    need += 2 * align32((NN + 1) / 2);
+    for (int l = 0; l < g.mg.nl; ++l) need += (l == 0 ? 3 : 4) * align32((g.mg.L[l].n + 1) / 2);
+  } else {
+    for (int l = 0; l < g.mg.nl; ++l) need += (l == 0 ? 3 : 4) * align32(g.mg.L[l].n);
+  }
   return need;
 }
 
@@ -2193,16 +2227,32 @@ int emc_poisson_solve_ws(emc_ctx* c, const double* rhs, double* sol, double rtol
   for (double** pp : {&a.part_a, &a.part_pq, &a.part_rr, &a.part_rz0, &a.part_rz1}) *pp = take(3 * (size_t)g.pcg_blocks);
   a.scal = take(8);
   a.done = (int*)take(1);
-  MgArgs mg = g.mg;
-  mg.done = a.done;
-  for (int l = 0; l < mg.nl; ++l) {
-    MgLevelDev& D = mg.L[l];
-    D.x = take(D.n); D.t = take(D.n); D.r = take(D.n);
-    D.b = l > 0 ? take(D.n) : a.r;
-  }
   HIPCHECK(c, hipMemcpyAsync(b, rhs, NN * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHECK(c, hipMemcpyAsync(x, sol, NN * 8, hipMemcpyHostToDevice, g.stream));
-  int st = run_pcg_with(c, a, mg, rtol, maxit, iters, relres, false);
+  int st;
+  if (c->opts.poisson_fp32 == 2) {
+    if (!g.f32_ready) return fail(c, EMC_ERR_STATE, "the FP32 cycle is not set up");
+    auto takef = [&](size_t n) { return (float*)take((n + 1) / 2); };
+    a.r32 = takef(NN);
+    a.z32 = takef(NN);
+    MgArgsT<float> mg = g.pmg;
+    mg.done = a.done;
+    for (int l = 0; l < mg.nl; ++l) {
+      MgLevelDevT<float>& D = mg.L[l];
+      D.x = takef(D.n); D.t = takef(D.n); D.r = takef(D.n);
+      D.b = l > 0 ? takef(D.n) : a.r32;
+    }
+    st = run_pcg_with(c, a, mg, rtol, maxit, iters, relres, false);
+  } else {
+    MgArgs mg = g.mg;
+    mg.done = a.done;
+    for (int l = 0; l < mg.nl; ++l) {
+      MgLevelDev& D = mg.L[l];
+      D.x = take(D.n); D.t = take(D.n); D.r = take(D.n);
+      D.b = l > 0 ? take(D.n) : a.r;
+    }
+    st = run_pcg_with(c, a, mg, rtol, maxit, iters, relres, false);
+  }
   HIPCHECK(c, hipMemcpyAsync(sol, x, NN * 8, hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(c, hipStreamSynchronize(g.stream));
   return st;
@@ -2343,7 +2393,7 @@ static int one_step(emc_ctx* c, bool heat_phase, emc_fly_tallies* acc, double ts
   HIPCHECK(c, emc_launch_deposit(da, g.stream));
   HIPCHECK(c, hipEventRecord(g.ev[1], g.stream));
   if (!heat_phase) {
-    if (c->opts.poisson_fp32)
+    if (c->opts.poisson_fp32 == 1)
       st = run_pcg_mixed(c, g.rhs, g.pot, c->opts.poisson_rtol, 20, c->opts.poisson_inner_rtol, nullptr, nullptr,
                          nullptr);
     else
@@ -2453,7 +2503,8 @@ struct emc_batch {
   // are uploaded each step, the rest once
   unsigned char* d_tab = nullptr;
   unsigned char* h_tab = nullptr;     // pinned host image of d_tab
-  size_t off_sort = 0, off_dep = 0, off_pcg = 0, off_mg = 0, off_field = 0, off_fly = 0, off_cp = 0, tab_bytes = 0;
+  size_t off_sort = 0, off_dep = 0, off_pcg = 0, off_mg = 0, off_fmg = 0, off_field = 0, off_fly = 0, off_cp = 0,
+         tab_bytes = 0;
   BatchTables bt{};
   const CopyArgs *cp_n = nullptr, *cp_scal = nullptr, *cp_tally = nullptr, *cp_zero = nullptr;
   unsigned long long* d_n = nullptr;   // [K] live counts after the sort
@@ -2503,6 +2554,8 @@ static int batch_setup(emc_batch* b) {
     if (d.cfg.flag_qc) return bfail(b, EMC_ERR_ARG, "batches run without the quantum correction");
     if (c->opts.poisson_rtol != c0->opts.poisson_rtol || c->opts.poisson_maxit != c0->opts.poisson_maxit)
       return bfail(b, EMC_ERR_ARG, "the contexts of a batch share the solver tolerance");
+    if (c->opts.poisson_fp32 == 1 || c->opts.poisson_fp32 != c0->opts.poisson_fp32)
+      return bfail(b, EMC_ERR_ARG, "the contexts of a batch share one FP64 CG (poisson_fp32 0 or 2 for all)");
     if (c->heat_phase) return bfail(b, EMC_ERR_STATE, "a context of the batch is in its heating phase");
     for (emc_ctx* o : b->c)
       if (o != c && o->gpu.get() == c->gpu.get()) return bfail(b, EMC_ERR_ARG, "a context appears twice");
@@ -2530,6 +2583,8 @@ static int batch_setup(emc_batch* b) {
   b->off_fly = place(K * sizeof(FlyArgs));
   b->off_pcg = place(K * sizeof(PcgArgs));
   b->off_mg = place(3 * K * sizeof(MgArgs));
+  const bool f32 = c0->opts.poisson_fp32 == 2;
+  b->off_fmg = place(f32 ? 3 * K * sizeof(MgArgsT<float>) : 0);
   b->off_field = place(K * sizeof(FieldArgs));
   b->off_cp = place(5 * K * sizeof(CopyArgs));
   b->tab_bytes = off;
@@ -2552,6 +2607,12 @@ static int batch_setup(emc_batch* b) {
   b->bt.mg_start = mg;
   b->bt.mg_even = mg + K;
   b->bt.mg_odd = mg + 2 * K;
+  if (f32) {
+    const MgArgsT<float>* fm = (const MgArgsT<float>*)(b->d_tab + b->off_fmg);
+    b->bt.fmg_start = fm;
+    b->bt.fmg_even = fm + K;
+    b->bt.fmg_odd = fm + 2 * K;
+  }
   b->bt.field = (const FieldArgs*)(b->d_tab + b->off_field);
   const CopyArgs* cp = (const CopyArgs*)(b->d_tab + b->off_cp);
   b->cp_n = cp;
@@ -2561,6 +2622,7 @@ static int batch_setup(emc_batch* b) {
   // the tables that stay: solver, cycle, fields, readbacks
   PcgArgs* hp = (PcgArgs*)(b->h_tab + b->off_pcg);
   MgArgs* hm = (MgArgs*)(b->h_tab + b->off_mg);
+  MgArgsT<float>* hfm = (MgArgsT<float>*)(b->h_tab + b->off_fmg);
   FieldArgs* hf = (FieldArgs*)(b->h_tab + b->off_field);
   CopyArgs* hc = (CopyArgs*)(b->h_tab + b->off_cp);
   for (int q = 0; q < K; ++q) {
@@ -2570,6 +2632,14 @@ static int batch_setup(emc_batch* b) {
     hm[q] = emc_mg_start_args(hp[q], g.mg);
     hm[K + q] = emc_mg_iter_args(hp[q], g.mg, 0);
     hm[2 * K + q] = emc_mg_iter_args(hp[q], g.mg, 1);
+    if (f32) {
+      if (!g.f32_ready) return bfail(b, EMC_ERR_STATE, "a context of the batch has no FP32 cycle");
+      hp[q].r32 = g.r32;
+      hp[q].z32 = g.z32;
+      hfm[q] = emc_mg_start_args(hp[q], g.pmg);
+      hfm[K + q] = emc_mg_iter_args(hp[q], g.pmg, 0);
+      hfm[2 * K + q] = emc_mg_iter_args(hp[q], g.pmg, 1);
+    }
     hf[q] = field_args(c, g.pot, true);
     int st = sync_fly_const(c);
     if (st) return bfail(b, st, c->err);

@@ -146,6 +146,10 @@ struct PcgArgsT {
   int* done;     // set once r.r reaches scal[4]: every later CG / V-cycle kernel returns at once
   double rtol;
   int pad0;
+  // FP32 preconditioner of the FP64 CG (null otherwise): the residual is also
+  // written in FP32 to r32, the cycle's input, and z is read from z32, its output
+  float* r32;
+  const float* z32;
 };
 using PcgArgs = PcgArgsT<double>;
 
@@ -245,6 +249,8 @@ struct BatchTables {
   const DepositArgs* dep;
   const PcgArgs* pcg;
   const MgArgs *mg_start, *mg_even, *mg_odd;
+  // the FP32 cycle's tables when the contexts solve with it (null otherwise)
+  const MgArgsT<float> *fmg_start, *fmg_even, *fmg_odd;
   const FieldArgs* field;
   const FlyArgs* fly;
 };
@@ -254,6 +260,8 @@ struct BatchTables {
 emc::MgArgs emc_mg_start_args(const emc::PcgArgs& a, const emc::MgArgs& m);
 emc::MgArgs emc_mg_iter_args(const emc::PcgArgs& a, const emc::MgArgs& m, int it);
 emc::MgArgsT<float> emc_mg_start_args(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m);
+emc::MgArgsT<float> emc_mg_start_args(const emc::PcgArgs& a, const emc::MgArgsT<float>& m);
+emc::MgArgsT<float> emc_mg_iter_args(const emc::PcgArgs& a, const emc::MgArgsT<float>& m, int it);
 emc::MgArgsT<float> emc_mg_iter_args(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, int it);
 hipError_t emc_launch_pcg_start(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, hipStream_t s);
 hipError_t emc_launch_pcg_iter(const emc::PcgArgsT<float>& a, const emc::MgArgsT<float>& m, int it, hipStream_t s);
@@ -289,3 +297,6 @@ hipError_t emc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 int emc_pcg_blocks(size_t n);
 hipError_t emc_launch_pcg_start(const emc::PcgArgs& a, const emc::MgArgs& m, hipStream_t s);
 hipError_t emc_launch_pcg_iter(const emc::PcgArgs& a, const emc::MgArgs& m, int it, hipStream_t s);
+// FP64 CG preconditioned by the FP32 cycle (a.r32 and a.z32 set)
+hipError_t emc_launch_pcg_start(const emc::PcgArgs& a, const emc::MgArgsT<float>& m, hipStream_t s);
+hipError_t emc_launch_pcg_iter(const emc::PcgArgs& a, const emc::MgArgsT<float>& m, int it, hipStream_t s);

@@ -807,6 +807,7 @@ __global__ __launch_bounds__(256) void k_pcg_resid(AS as_) {
     node_ijk(a, n, i, j, k);
     const double r = (double)a.b[n] - apply_op(a, a.x, n, i, j, k);
     a.r[n] = (R)r;
+    if (a.r32) a.r32[n] = (float)r;
     rr += r * r;
     bb += (double)a.b[n] * (double)a.b[n];
   }
@@ -851,25 +852,29 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
     total_first2(rz_old, rz_new, a.nblocks, to, tn);
     beta = (to != 0.0) ? tn / to : 0.0;
   }
-  const R* __restrict__ z = a.z;
-  auto pv = [&](size_t m) -> double { return first ? (double)z[m] : (double)z[m] + beta * (double)p_in[m]; };
   const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
   double pq = 0;
-  GRID_LOOP(n, a.n) {
-    int i, j, k;
-    node_ijk(a, n, i, j, k);
-    const double pn = pv(n);
-    double q = (double)a.diag[n] * pn;
-    if (i + 1 < a.pnx) q += (double)a.cx[n] * pv(n + sx);
-    if (i > 0) q += (double)a.cx[n - sx] * pv(n - sx);
-    if (j + 1 < a.pny) q += (double)a.cy[n] * pv(n + sy);
-    if (j > 0) q += (double)a.cy[n - sy] * pv(n - sy);
-    if (k + 1 < a.pnz) q += (double)a.cz[n] * pv(n + 1);
-    if (k > 0) q += (double)a.cz[n - 1] * pv(n - 1);
-    p_out[n] = (R)pn;
-    a.q[n] = (R)q;
-    pq += pn * q;
-  }
+  // z in the CG's precision, or the FP32 cycle's output
+  auto pass = [&](const auto* __restrict__ z) {
+    auto pv = [&](size_t m) -> double { return first ? (double)z[m] : (double)z[m] + beta * (double)p_in[m]; };
+    GRID_LOOP(n, a.n) {
+      int i, j, k;
+      node_ijk(a, n, i, j, k);
+      const double pn = pv(n);
+      double q = (double)a.diag[n] * pn;
+      if (i + 1 < a.pnx) q += (double)a.cx[n] * pv(n + sx);
+      if (i > 0) q += (double)a.cx[n - sx] * pv(n - sx);
+      if (j + 1 < a.pny) q += (double)a.cy[n] * pv(n + sy);
+      if (j > 0) q += (double)a.cy[n - sy] * pv(n - sy);
+      if (k + 1 < a.pnz) q += (double)a.cz[n] * pv(n + 1);
+      if (k > 0) q += (double)a.cz[n - 1] * pv(n - 1);
+      p_out[n] = (R)pn;
+      a.q[n] = (R)q;
+      pq += pn * q;
+    }
+  };
+  if (a.z32) pass(a.z32);
+  else pass((const R*)a.z);
   block_reduce1(pq, a.part_pq);
 }
 
@@ -890,6 +895,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(AS as_, int it) {
     a.x[n] = (R)((double)a.x[n] + alpha * (double)p[n]);
     const double r = (double)a.r[n] - alpha * (double)a.q[n];
     a.r[n] = (R)r;
+    if (a.r32) a.r32[n] = (float)r;
     rr += r * r;
   }
   block_reduce1(rr, a.part_rr);
@@ -1477,21 +1483,27 @@ static void launch_vcycle(const SHAPE& m, AS ms, int nblocks, unsigned K, hipStr
 // the cycle's argument block of the solve's start (r.z partials into part_rz0,
 // no residual booking) and of CG iteration `it` (r.z into the partial array the
 // next iteration reads as rz_new; the first kernel books the residual)
-template <class R>
-static MgArgsT<R> mg_start_args(const PcgArgsT<R>& a, const MgArgsT<R>& m) {
-  MgArgsT<R> m0 = m;
+// the cycle's output: the CG's z, or z32 for the FP32 cycle of the FP64 CG
+template <class RM, class RA>
+static RM* cycle_out(const PcgArgsT<RA>& a) {
+  if constexpr (std::is_same<RA, RM>::value) return a.z;
+  else return const_cast<float*>(a.z32);
+}
+template <class RA, class RM>
+static MgArgsT<RM> mg_start_args(const PcgArgsT<RA>& a, const MgArgsT<RM>& m) {
+  MgArgsT<RM> m0 = m;
   m0.part_rr = nullptr;
-  m0.z = a.z;
+  m0.z = cycle_out<RM>(a);
   m0.part_rz = a.part_rz0;
   return m0;
 }
-template <class R>
-static MgArgsT<R> mg_iter_args(const PcgArgsT<R>& a, const MgArgsT<R>& m, int it) {
-  MgArgsT<R> mi = m;
+template <class RA, class RM>
+static MgArgsT<RM> mg_iter_args(const PcgArgsT<RA>& a, const MgArgsT<RM>& m, int it) {
+  MgArgsT<RM> mi = m;
   mi.part_rr = a.part_rr;
   mi.scal = a.scal;
   mi.npart = a.nblocks;
-  mi.z = a.z;
+  mi.z = cycle_out<RM>(a);
   mi.part_rz = (it & 1) ? a.part_rz0 : a.part_rz1;
   return mi;
 }
@@ -1502,11 +1514,13 @@ MgArgsT<float> emc_mg_start_args(const PcgArgsT<float>& a, const MgArgsT<float>&
 MgArgsT<float> emc_mg_iter_args(const PcgArgsT<float>& a, const MgArgsT<float>& m, int it) {
   return mg_iter_args(a, m, it);
 }
+MgArgsT<float> emc_mg_start_args(const PcgArgs& a, const MgArgsT<float>& m) { return mg_start_args(a, m); }
+MgArgsT<float> emc_mg_iter_args(const PcgArgs& a, const MgArgsT<float>& m, int it) { return mg_iter_args(a, m, it); }
 
-template <class R>
-static hipError_t launch_pcg_start(const PcgArgsT<R>& a, const MgArgsT<R>& m, hipStream_t s) {
-  using PA = PcgArgsT<R>;
-  using MA = MgArgsT<R>;
+template <class RA, class RM>
+static hipError_t launch_pcg_start(const PcgArgsT<RA>& a, const MgArgsT<RM>& m, hipStream_t s) {
+  using PA = PcgArgsT<RA>;
+  using MA = MgArgsT<RM>;
   const int g = a.nblocks;
   hipLaunchKernelGGL(k_pcg_fix<PA>, dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_pcg_resid<PA>, dim3(g), dim3(256), 0, s, a);
@@ -1519,13 +1533,14 @@ hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgs& m, hipStream_t s
 hipError_t emc_launch_pcg_start(const PcgArgsT<float>& a, const MgArgsT<float>& m, hipStream_t s) {
   return launch_pcg_start(a, m, s);
 }
+hipError_t emc_launch_pcg_start(const PcgArgs& a, const MgArgsT<float>& m, hipStream_t s) { return launch_pcg_start(a, m, s); }
 
 // iteration `it`: r.z of the previous iteration is in part_rz0 (even it) or
 // part_rz1; the search direction alternates between p (even it) and p2
-template <class R>
-static hipError_t launch_pcg_iter(const PcgArgsT<R>& a, const MgArgsT<R>& m, int it, hipStream_t s) {
-  using PA = PcgArgsT<R>;
-  using MA = MgArgsT<R>;
+template <class RA, class RM>
+static hipError_t launch_pcg_iter(const PcgArgsT<RA>& a, const MgArgsT<RM>& m, int it, hipStream_t s) {
+  using PA = PcgArgsT<RA>;
+  using MA = MgArgsT<RM>;
   const int g = a.nblocks;
   hipLaunchKernelGGL(k_pcg_spmvp<PA>, dim3(g), dim3(256), 0, s, a, it);
   hipLaunchKernelGGL(k_pcg_update<PA>, dim3(g), dim3(256), 0, s, a, it);
@@ -1537,6 +1552,9 @@ hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgs& m, int it, hipStr
   return launch_pcg_iter(a, m, it, s);
 }
 hipError_t emc_launch_pcg_iter(const PcgArgsT<float>& a, const MgArgsT<float>& m, int it, hipStream_t s) {
+  return launch_pcg_iter(a, m, it, s);
+}
+hipError_t emc_launch_pcg_iter(const PcgArgs& a, const MgArgsT<float>& m, int it, hipStream_t s) {
   return launch_pcg_iter(a, m, it, s);
 }
 
@@ -1564,7 +1582,8 @@ hipError_t emc_launch_pcg_start_b(const BatchTables& t, const MgArgs& shape, int
   const unsigned K = (unsigned)t.K;
   hipLaunchKernelGGL(k_pcg_fix<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
   hipLaunchKernelGGL(k_pcg_resid<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg);
-  launch_vcycle<MgArgs, const MgArgs*>(shape, t.mg_start, nblocks, K, s);
+  if (t.fmg_start) launch_vcycle<MgArgs, const MgArgsT<float>*>(shape, t.fmg_start, nblocks, K, s);
+  else launch_vcycle<MgArgs, const MgArgs*>(shape, t.mg_start, nblocks, K, s);
   hipLaunchKernelGGL(k_pcg_begin<const PcgArgs*>, dim3(1, K), dim3(256), 0, s, t.pcg);
   return hipGetLastError();
 }
@@ -1572,7 +1591,8 @@ hipError_t emc_launch_pcg_iter_b(const BatchTables& t, const MgArgs& shape, int 
   const unsigned K = (unsigned)t.K;
   hipLaunchKernelGGL(k_pcg_spmvp<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
   hipLaunchKernelGGL(k_pcg_update<const PcgArgs*>, dim3(nblocks, K), dim3(256), 0, s, t.pcg, it);
-  launch_vcycle<MgArgs, const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
+  if (t.fmg_start) launch_vcycle<MgArgs, const MgArgsT<float>*>(shape, (it & 1) ? t.fmg_odd : t.fmg_even, nblocks, K, s);
+  else launch_vcycle<MgArgs, const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
   return hipGetLastError();
 }
 hipError_t emc_launch_tile_sort_b(const BatchTables& t, long long nmax, hipStream_t s) {

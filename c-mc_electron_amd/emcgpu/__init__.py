@@ -102,7 +102,7 @@ class Engine:
     """One device description (decks) plus its GPU state."""
 
     def __init__(self, input_txt, device=0, seed=1, table_dir=TABLE_DIR, capacity_factor=1.25,
-                 poisson_rtol=1e-10, poisson_maxit=20000, poisson_fp32=False, poisson_inner_rtol=1e-5):
+                 poisson_rtol=1e-10, poisson_maxit=20000, poisson_fp32=None, poisson_inner_rtol=1e-5):
         self.lib = load_library()
         o = abi.Opts()
         self.lib.emc_default_opts(C.byref(o))
@@ -113,7 +113,10 @@ class Engine:
         o.capacity_factor = capacity_factor
         o.poisson_rtol = poisson_rtol
         o.poisson_maxit = poisson_maxit
-        o.poisson_fp32 = 1 if poisson_fp32 else 0
+        if poisson_fp32 is not None:
+            # 0 / False: FP64 throughout; 1 / True: FP32 bulk with FP64 refinement;
+            # 2: FP64 CG with the FP32 cycle (the library's default)
+            o.poisson_fp32 = int(poisson_fp32)
         o.poisson_inner_rtol = poisson_inner_rtol
         st = C.c_int(0)
         self.ctx = self.lib.emc_create(os.fsencode(input_txt), C.byref(o), C.byref(st))

@@ -202,7 +202,12 @@ typedef struct emc_opts {
   double capacity_factor;    /* particle capacity = factor * initial count      */
   double poisson_rtol;       /* relative residual target of the PCG            */
   int poisson_maxit;
-  int poisson_fp32;          /* 1: emc_step solves with emc_poisson_solve_mixed  */
+  int poisson_fp32;          /* precision of emc_step's and emc_poisson_solve's
+                                solve: 0 FP64 CG with the FP64 multigrid cycle;
+                                1 emc_step solves with emc_poisson_solve_mixed;
+                                2 (default) FP64 CG with the cycle in FP32: the
+                                CG vectors, residual and stopping test stay FP64,
+                                the preconditioner streams half the bytes         */
   double poisson_inner_rtol; /* tolerance of the FP32 bulk solve (1e-5)          */
 } emc_opts;
 
@@ -281,8 +286,9 @@ int emc_deposit(emc_ctx *ctx, int64_t *node_q, int64_t *node_stat, int64_t *cell
 
 /* Poisson solve on the eliminated operator: rhs, sol of length nnode.  rhs holds
  * the reference right-hand side (compute_rhs); sol is the initial guess on entry.
- * CG preconditioned by one geometric multigrid cycle (V(2,2) on the mesh, V(1,1)
- * on the coarser levels), to ||r|| <= rtol ||b||. */
+ * FP64 CG preconditioned by one geometric multigrid cycle (V(2,2) on the mesh,
+ * V(1,1) on the coarser levels), to ||r|| <= rtol ||b||; the cycle runs in FP32
+ * when emc_opts.poisson_fp32 is 2 (the default), else in FP64. */
 int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
                       int *iters, double *relres);
 
@@ -291,9 +297,10 @@ int emc_poisson_solve(emc_ctx *ctx, const double *rhs, double *sol, double rtol,
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
  * solve's work vectors (staged rhs and sol, the CG vectors and partial sums,
  * the multigrid work vectors) live in one static 80 MiB device array per GPU,
- * held until the process ends: meshes up to about 980 k nodes (10.6 doubles
- * per node); a larger mesh returns EMC_ERR_CAPACITY.  Same arguments, results
- * and iteration counts as emc_poisson_solve.  Every call on one GPU shares that
+ * held until the process ends: meshes up to about 1.07 M nodes with the FP32
+ * cycle (9.8 doubles per node), 980 k with the FP64 cycle (10.6); a larger mesh
+ * returns EMC_ERR_CAPACITY.  Same arguments, precision, results and iteration
+ * counts as emc_poisson_solve.  Every call on one GPU shares that
  * GPU's array: calls that overlap on one GPU race, so the caller serialises
  * them. */
 int emc_poisson_solve_noalloc(emc_ctx *ctx, const double *rhs, double *sol, double rtol, int maxit,
@@ -325,7 +332,8 @@ int emc_poisson_mg_level(emc_ctx *ctx, int level, int dims[3], double *a27, doub
  * context keeps its own state and results: n steps of the batch leave every
  * context exactly where n emc_step calls on it alone would (bit for bit).  No
  * quantum correction; the heating phase stays per context.  The contexts must
- * outlive the batch and are not stepped alone while it steps them.
+ * outlive the batch and are not stepped alone while it steps them.  They share
+ * the solver options (poisson_fp32 0 or 2, not the mixed solve's 1).
  * acc (may be NULL) receives one tally block per context.  emc_batch_timing
  * gives the phase times (ms per step, as emc_step_timing) of the last call and
  * the largest CG iteration count of its last step. */

@@ -3,7 +3,8 @@ every kernel one launch for all of them (emc_batch_*), against each device
 stepped alone with emc_step: after the same steps, every device's potential,
 population (sorted by key), tallies and statistics window are identical bit for
 bit.  The devices differ in drain and gate bias and in particle count (uneven
-populations, as the equilibrium charge of each bias point sets them)."""
+populations, as the equilibrium charge of each bias point sets them).  Both
+precisions of the multigrid cycle (FP32, the default, and FP64) are run."""
 import numpy as np
 import pytest
 
@@ -15,10 +16,10 @@ POINTS = [(0.5, 0.7, 6000), (0.05, 0.0, 9000), (0.8, 0.4, 3000), (0.3, 0.8, 1200
 STEPS = 6
 
 
-def engines():
+def engines(mode):
     out = []
     for vd, vg, ne in POINTS:
-        e = emcgpu.Engine(cfg1_deck(vd=vd, vg=vg, electrons=ne, holes=ne // 10), device=0, seed=7)
+        e = emcgpu.Engine(cfg1_deck(vd=vd, vg=vg, electrons=ne, holes=ne // 10), device=0, seed=7, poisson_fp32=mode)
         e.particles_init()
         out.append(e)
     return out
@@ -30,9 +31,10 @@ def snapshot(e):
     return {"pot": e.potential(), "p": p, "win": w}
 
 
-def test_batch_equals_each_device_alone():
-    print("[gpu] batch of 4 bias points", flush=True)
-    bat_eng = engines()
+@pytest.mark.parametrize("mode", [2, 0])
+def test_batch_equals_each_device_alone(mode):
+    print(f"[gpu] batch of 4 bias points, poisson_fp32 {mode}", flush=True)
+    bat_eng = engines(mode)
     b = emcgpu.Batch(bat_eng)
     acc_b = [b.step(2) for _ in range(STEPS // 2)]
     ms, imax = b.timing()
@@ -40,7 +42,7 @@ def test_batch_equals_each_device_alone():
     got = [snapshot(e) for e in bat_eng]
     b.close()
     print("[gpu] the same points one at a time", flush=True)
-    solo_eng = engines()
+    solo_eng = engines(mode)
     acc_s = [[e.step(2) for _ in range(STEPS // 2)] for e in solo_eng]
     want = [snapshot(e) for e in solo_eng]
     for q in range(len(POINTS)):
@@ -70,5 +72,13 @@ def test_batch_refuses_mixed_meshes():
     with pytest.raises(emcgpu.EmcError) as e:
         emcgpu.Batch([a, c])
     assert e.value.code == emcgpu.abi.EMC_ERR_ARG
-    a.close()
     c.close()
+    # one FP64 CG for all: the cycle's precision is shared, the mixed solve not batched
+    for mode in (0, 1):
+        d = emcgpu.Engine(cfg1_deck(), device=0, poisson_fp32=mode)
+        d.particles_init()
+        with pytest.raises(emcgpu.EmcError) as e:
+            emcgpu.Batch([a, d])
+        assert e.value.code == emcgpu.abi.EMC_ERR_ARG
+        d.close()
+    a.close()
