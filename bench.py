@@ -517,23 +517,26 @@ def sweep_converged(window=1000, relax_steps=2000, skip=2, max_windows=30, rel=0
 # ---------------------------------------------------------------- precision
 def precision_run(refine=8, electrons=8000000, holes=800000, relax_steps=100, solves=5, steps=600, window=100):
     """The 320x96x160-node-step FinFET mesh (160x320x96 cells, 4.99 M nodes) with
-    8 M electrons: the FP32-bulk solve with FP64 iterative refinement against the
-    FP64 solve on the same right-hand sides (max |phi_mixed - phi_fp64| and the
-    time of each), then the drain current of a run with each solve (same seed and
+    8 M electrons: the FP32-bulk solve with FP64 iterative refinement and the FP64
+    CG with the FP32 multigrid cycle (the default solve) against the FP64 solve
+    (FP64 cycle) on the same right-hand sides (max |phi - phi_fp64| and the time
+    of each), then the drain current of a run with each solve (same seed and
     initial population; windows of `window` steps, Student-t 95 % interval)."""
     from scipy import stats as st
     tmp = tempfile.mkdtemp(prefix="emc_prec_")
     inp = emcgpu.write_deck(tmp, refine=refine, electrons=electrons, holes=holes, vd=0.5, vg=0.7, total_step=100000,
                             stat_step=1000)
     t0 = time.perf_counter()
-    eng = emcgpu.Engine(inp, device=0, seed=1)
+    eng = emcgpu.Engine(inp, device=0, seed=1, poisson_fp32=0)
     n0 = eng.particles_init()
     setup = time.perf_counter() - t0
     pot0 = eng.mesh.pot0
+    e32 = emcgpu.Engine(inp, device=0, seed=1, poisson_fp32=2)  # solves only
     print(f"[bench] precision: {n0} particles, {eng.mesh.nnode} nodes, setup {setup:.1f} s", file=sys.stderr,
           flush=True)
     relax(eng, relax_steps, "precision relax")
     errs, t64, tmx, outer, inner, it64 = [], [], [], [], [], []
+    errs32, t32, it32 = [], [], []
     for q in range(solves):
         node_q, _, _, _ = eng.deposit()
         rhs = eng.rhs_from_charge(node_q)
@@ -544,14 +547,21 @@ def precision_run(refine=8, electrons=8000000, holes=800000, relax_steps=100, so
         t = time.perf_counter()
         xm, o, i, rm = eng.poisson_solve_mixed(rhs, x0=x0, rtol=1e-10, inner_rtol=1e-5)
         tmx.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        xc, ic, rc = e32.poisson_solve(rhs, x0=x0, rtol=1e-10)
+        t32.append(time.perf_counter() - t)
         errs.append(float(np.max(np.abs(xm - x64)) * pot0))
+        errs32.append(float(np.max(np.abs(xc - x64)) * pot0))
         outer.append(o)
         inner.append(i)
         it64.append(i64)
+        it32.append(ic)
         eng.step(20)
-        print(f"[bench] precision solve {q}: max err {errs[-1]:.3e} V, fp64 {t64[-1]*1e3:.1f} ms ({i64} it), "
-              f"mixed {tmx[-1]*1e3:.1f} ms ({o} outer, {i} FP32 it)", file=sys.stderr, flush=True)
+        print(f"[bench] precision solve {q}: fp64 {t64[-1]*1e3:.1f} ms ({i64} it); mixed max err {errs[-1]:.3e} V, "
+              f"{tmx[-1]*1e3:.1f} ms ({o} outer, {i} FP32 it); fp32 cycle max err {errs32[-1]:.3e} V, "
+              f"{t32[-1]*1e3:.1f} ms ({ic} it)", file=sys.stderr, flush=True)
     eng.close()
+    e32.close()
 
     def id_run(fp32):
         e = emcgpu.Engine(inp, device=0, seed=1, poisson_fp32=fp32)
@@ -570,15 +580,20 @@ def precision_run(refine=8, electrons=8000000, holes=800000, relax_steps=100, so
         return {"id_mean": float(x.mean()), "id_half_width": float(hw), "windows": len(x),
                 "ms_per_step": el / steps * 1e3, "poisson_ms": float(ms[1])}
 
-    r64 = id_run(False)
-    rmx = id_run(True)
-    print(f"[bench] precision id: fp64 {r64}, mixed {rmx}", file=sys.stderr, flush=True)
+    r64 = id_run(0)
+    rmx = id_run(1)
+    r32 = id_run(2)
+    print(f"[bench] precision id: fp64 {r64}, mixed {rmx}, fp32 cycle {r32}", file=sys.stderr, flush=True)
     return {"particles": int(n0), "setup_s": setup,
             "max_abs_err_v": max(errs), "errs_v": errs, "fp64_solve_ms": float(np.mean(t64)) * 1e3,
             "mixed_solve_ms": float(np.mean(tmx)) * 1e3, "fp64_cg_iters": it64, "refine_steps": outer,
             "fp32_cg_iters": inner, "id_fp64": r64, "id_mixed": rmx,
             "id_diff_in_combined_half_widths": abs(r64["id_mean"] - rmx["id_mean"]) /
-            max(1e-300, np.hypot(r64["id_half_width"], rmx["id_half_width"]))}
+            max(1e-300, np.hypot(r64["id_half_width"], rmx["id_half_width"])),
+            "fp32_cycle": {"max_abs_err_v": max(errs32), "errs_v": errs32, "solve_ms": float(np.mean(t32)) * 1e3,
+                           "cg_iters": it32, "id": r32,
+                           "id_diff_in_combined_half_widths": abs(r64["id_mean"] - r32["id_mean"]) /
+                           max(1e-300, np.hypot(r64["id_half_width"], r32["id_half_width"]))}}
 
 
 # ---------------------------------------------------------------- serving
@@ -852,6 +867,7 @@ def main():
                                f"HoleNumber={a.holes}), dt 0.1 fs, full step: deposit + PCG Poisson + fields + flight"
                                + (" + quantum-correction potential" if a.qc else ""),
                    "quantum_correction": bool(a.qc), "relax_steps": relaxed,
+                   "poisson_solve": "FP64 CG to ||r|| <= 1e-10 ||b||, preconditioned by a multigrid cycle in FP32",
                    "particles": int(n_start), "nodes": int(nodes),
                    "phase_ms": {"deposit_rhs": phase[0], "poisson": phase[1], "field": phase[2],
                                 "flight_total": phase[3], "k_fly": phase[4]},

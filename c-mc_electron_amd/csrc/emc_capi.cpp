@@ -97,7 +97,11 @@ struct Gpu {
   long long n = 0, cap = 0;
   // tile sort / deposit
   int ntx = 0, nty = 0, ntz = 0, ntile = 0;
-  unsigned *tcount = nullptr, *toff = nullptr, *tcursor = nullptr;
+  unsigned *tcount = nullptr, *toff = nullptr, *tcursor = nullptr, *tstay = nullptr, *toff_old = nullptr;
+  // the population is in the tile segments of the last sort (toff), with the
+  // particles added since after them from slot sorted_n on
+  bool seg_valid = false;
+  long long sorted_n = 0;
   unsigned long long* blk = nullptr;
   unsigned* n_host = nullptr;         // pinned: live count after the last sort
   hipEvent_t n_ready = nullptr;
@@ -551,6 +555,8 @@ static int gpu_setup(emc_ctx* c) {
   g.tcount = m.alloc<unsigned>(g.ntile);
   g.tcursor = m.alloc<unsigned>(g.ntile);
   g.toff = m.alloc<unsigned>(g.ntile + 1);
+  g.tstay = m.alloc<unsigned>(g.ntile);
+  g.toff_old = m.alloc<unsigned>(g.ntile + 1);
   g.blk = m.alloc<unsigned long long>((size_t)g.ntile * DEP_NQ * TILE_NODES);
   g.T.contact = g.contact; g.T.counters = g.counters; g.T.rcur = g.rcur; g.T.lcur = g.lcur;
   g.T.heat_e = g.heat_e; g.T.heat_h = g.heat_h;
@@ -596,6 +602,7 @@ static int gpu_setup(emc_ctx* c) {
   if (m.last != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(m.last));
   HIPCHECK(c, hipMemset(g.pot, 0, NN * sizeof(double)));
   HIPCHECK(c, hipMemset(g.tcount, 0, g.ntile * sizeof(unsigned)));
+  HIPCHECK(c, hipMemset(g.toff, 0, (g.ntile + 1) * sizeof(unsigned)));
   HIPCHECK(c, hipHostMalloc((void**)&g.n_host, sizeof(unsigned), hipHostMallocDefault));
   HIPCHECK(c, hipHostMalloc((void**)&g.scal_host, 32 * sizeof(double), hipHostMallocDefault));
   HIPCHECK(c, hipHostMalloc((void**)&g.tally_host, FLYOUT_N * sizeof(unsigned long long), hipHostMallocDefault));
@@ -667,6 +674,7 @@ static int gpu_alloc_particles(emc_ctx* c, long long cap) {
     g.n = 0;
     g.n_pending = false;
   }
+  g.seg_valid = false;
   g.cap = cap;
   alloc_part(m, g.P, cap);
   alloc_part(m, g.P2, cap);
@@ -887,6 +895,7 @@ static int gpu_upload_host(emc_ctx* c) {
   HIPCHECK(c, up(g.P.pk, pk.data(), n * 4)); HIPCHECK(c, up(g.P.itet, hp.itet.data(), n * 4));
   HIPCHECK(c, up(g.P.isym, hp.isym.data(), n * 4)); HIPCHECK(c, up(g.P.key, hp.key.data(), n * 8));
   g.n = n;
+  g.seg_valid = false;
   return EMC_OK;
 }
 
@@ -932,6 +941,7 @@ static int gpu_sample_population(emc_ctx* c, const RestartCells* rs) {
   if (e != hipSuccess) return fail(c, EMC_ERR_HIP, std::string("initial sampling: ") + hipGetErrorString(e));
   g.n = n;
   g.n_pending = false;
+  g.seg_valid = false;
   c->hp.resize(0);
   return EMC_OK;
 }
@@ -1373,21 +1383,35 @@ static int settle_n(emc_ctx* c) {
   if (!g.n_pending) return EMC_OK;
   HIPCHECK(c, hipEventSynchronize(g.n_ready));
   g.n = (long long)*g.n_host;
+  g.sorted_n = g.n;
   g.n_pending = false;
   return EMC_OK;
 }
 
 // Counting sort of the population by deposit tile into the second buffer, which
 // becomes the current one.  Empty slots are dropped.
+static SortArgs sort_args(emc_ctx* c) {
+  Gpu& g = *c->gpu;
+  SortArgs a{};
+  a.P = g.P; a.Q = g.P2; a.n = g.n;
+  a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
+  a.tcount = g.tcount; a.toff = g.toff; a.tcursor = g.tcursor; a.tstay = g.tstay; a.toff_old = g.toff_old;
+  a.seg_valid = g.seg_valid ? 1 : 0;
+  return a;
+}
+// slots outside the last sort's segments (all of them without segments)
+static long long sort_nappend(emc_ctx* c) {
+  const Gpu& g = *c->gpu;
+  return g.seg_valid ? std::max(0LL, g.n - g.sorted_n) : g.n;
+}
+
 static int run_sort(emc_ctx* c) {
   Gpu& g = *c->gpu;
   int st = settle_n(c);
   if (st) return st;
-  SortArgs a{};
-  a.P = g.P; a.Q = g.P2; a.n = g.n;
-  a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
-  a.tcount = g.tcount; a.toff = g.toff; a.tcursor = g.tcursor;
-  HIPCHECK(c, emc_launch_tile_sort(a, g.stream));
+  SortArgs a = sort_args(c);
+  HIPCHECK(c, emc_launch_tile_sort(a, sort_nappend(c), g.stream));
+  g.seg_valid = true;
   std::swap(g.P, g.P2);
   HIPCHECK(c, hipMemcpyAsync(g.n_host, g.toff + g.ntile, sizeof(unsigned), hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(c, hipEventRecord(g.n_ready, g.stream));
@@ -1699,6 +1723,7 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
     g.dead = dead; g.dead_cap = clone_cap;
     g.slow = slow;
     g.n = total;
+    g.seg_valid = false;
     if (before) *before = (long long)h[0];
     if (after) *after = total;
     return EMC_OK;
@@ -1706,6 +1731,7 @@ static int run_mr(emc_ctx* c, uint64_t step, long long* before, long long* after
   HIPCHECK(c, emc_launch_mr_write(a, g.stream));
   std::swap(g.P, g.P2);
   g.n = total;
+  g.seg_valid = false;
   if (before) *before = (long long)h[0];
   if (after) *after = total;
   return EMC_OK;
@@ -1935,6 +1961,7 @@ int emc_checkpoint_load(emc_ctx* c, const char* path) {
     if (p.bytes) HIPCHECK(c, hipMemcpy(*p.dev, p.h.data(), p.bytes, hipMemcpyHostToDevice));
   g.n = nl;
   g.n_pending = false;
+  g.seg_valid = false;
   c->next_step = next;
   c->win = win;
   c->win_steps = flags[0];
@@ -2673,12 +2700,9 @@ static int batch_one_step(emc_batch* b, emc_fly_tallies* acc, double tsum[5], in
     if (st) return bfail(b, st, c->err);
     st = grow_if_full(c);
     if (st) return bfail(b, st, c->err);
-    SortArgs& a = hs[q];
-    a = SortArgs{};
-    a.P = g.P; a.Q = g.P2; a.n = g.n;
-    a.ntx = g.ntx; a.nty = g.nty; a.ntz = g.ntz; a.ntile = g.ntile;
-    a.tcount = g.tcount; a.toff = g.toff; a.tcursor = g.tcursor;
-    nmax = std::max(nmax, g.n);
+    hs[q] = sort_args(c);
+    nmax = std::max(nmax, sort_nappend(c));
+    g.seg_valid = true;
     std::swap(g.P, g.P2);
     hd[q] = deposit_args(c, 1);
     hd[q].st = stat_args(c, false);
@@ -2688,7 +2712,7 @@ static int batch_one_step(emc_batch* b, emc_fly_tallies* acc, double tsum[5], in
   BHIP(b, hipEventRecord(b->ev[0], s));
   BHIP(b, hipMemcpyAsync(b->d_tab + b->off_sort, b->h_tab + b->off_sort, b->off_fly - b->off_sort,
                          hipMemcpyHostToDevice, s));
-  BHIP(b, emc_launch_tile_sort_b(b->bt, nmax, s));
+  BHIP(b, emc_launch_tile_sort_b(b->bt, b->ntile, nmax, s));
   // the live counts (32-bit toff[ntile] of each device) into d_n
   BHIP(b, emc_launch_copy_b(b->cp_n, K, 1, s));
   BHIP(b, hipMemcpyAsync(b->h_n, b->d_n, K * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -2731,6 +2755,7 @@ static int batch_one_step(emc_batch* b, emc_fly_tallies* acc, double tsum[5], in
     emc_ctx* c = b->c[q];
     Gpu& g = *c->gpu;
     g.n = (long long)(unsigned)b->h_n[q];
+    g.sorted_n = g.n;
     g.n_pending = false;
     hfl[q] = fly_args(c, step[q], 0);
     nmax = std::max(nmax, g.n);

@@ -46,14 +46,24 @@ constexpr int TILE_CELLS = TILE_I * TILE_J * TILE_K;
 constexpr int DEP_NQ = 11;  // node charge + 5 statistics x 2 carrier types
 
 // Counting sort of the particles by tile (drops empty slots).
+// The population P was left by the previous sort in tile segments: slots
+// toff[t] .. toff[t+1]-1 held the particles of tile t, and slots from toff[ntile]
+// on hold particles added since (clones).  A particle still in the tile of its
+// segment keeps its place among the segment's stayers; the others (movers: they
+// changed tile, or were added) go after the stayers of their new tile.  With
+// seg_valid = 0 (the population was written some other way) every particle is
+// a mover.
 struct SortArgs {
   PartDev P;                  // source
   PartDev Q;                  // destination, tile order
   long long n;
   int ntx, nty, ntz, ntile;
-  unsigned* tcount;           // [ntile] zero on entry; left zero by k_tile_scan
-  unsigned* toff;             // [ntile + 1]
+  unsigned* tcount;           // [ntile] movers into each tile; zero on entry, left zero by k_tile_scan
+  unsigned* toff;             // [ntile + 1] segments of the previous sort on entry, of this one after k_tile_scan
   unsigned* tcursor;          // [ntile] zeroed by k_tile_scan
+  unsigned* tstay;            // [ntile] stayers of each tile (k_tile_hist)
+  unsigned* toff_old;         // [ntile + 1] the segments of the previous sort (k_tile_scan; all 0 without)
+  int seg_valid, pad;
 };
 
 
@@ -273,7 +283,7 @@ hipError_t emc_launch_zero_f32(float* p, size_t n, hipStream_t s);
 hipError_t emc_launch_pcg_start_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, hipStream_t s);
 hipError_t emc_launch_pcg_iter_b(const emc::BatchTables& t, const emc::MgArgs& shape, int nblocks, int it,
                                  hipStream_t s);
-hipError_t emc_launch_tile_sort_b(const emc::BatchTables& t, long long nmax, hipStream_t s);
+hipError_t emc_launch_tile_sort_b(const emc::BatchTables& t, int ntile, long long nappend_max, hipStream_t s);
 hipError_t emc_launch_deposit_b(const emc::BatchTables& t, int ntile, size_t nnode, hipStream_t s);
 hipError_t emc_launch_field_b(const emc::BatchTables& t, size_t ncell, hipStream_t s);
 hipError_t emc_launch_fly_b(const emc::BatchTables& t, long long nmax, long long wave_slots, hipStream_t s);
@@ -288,7 +298,9 @@ hipError_t emc_launch_fly_clones(const emc::FlyArgs& a, unsigned long long b, un
 hipError_t emc_launch_merge(const emc::FlyArgs& a, unsigned long long begin, unsigned long long end,
                             const unsigned long long* ndead, unsigned long long* ticket, hipStream_t s);
 hipError_t emc_launch_clones_dev(const emc::FlyArgs& a, hipStream_t s);
-hipError_t emc_launch_tile_sort(const emc::SortArgs& a, hipStream_t s);
+// nappend: an upper bound of the movers outside the segments (slots from the
+// previous sort's total on, or every slot without segments), for the grid
+hipError_t emc_launch_tile_sort(const emc::SortArgs& a, long long nappend, hipStream_t s);
 hipError_t emc_launch_deposit(const emc::DepositArgs& a, hipStream_t s);
 hipError_t emc_launch_rhs(const emc::RhsArgs& a, hipStream_t s);
 hipError_t emc_launch_qc_potential(const emc::QcArgs& a, hipStream_t s);

@@ -332,11 +332,57 @@ __device__ inline unsigned wave_run_add(unsigned* ctr, int key) {
   return base + (unsigned)(lane - hl);
 }
 
+// Blocks 0 .. ntile-1 take the segment of their tile; the blocks after them
+// stride over the slots past the segments (every particle there is a mover).
+__device__ inline void seg_bounds(const SortArgs& a, const unsigned* off, unsigned& lo, unsigned& hi) {
+  const int t = blockIdx.x;
+  lo = hi = 0;
+  if (a.seg_valid) {
+    const unsigned n = (unsigned)a.n;
+    lo = min(off[t], n);
+    hi = min(off[t + 1], n);
+  }
+}
+__device__ inline long long append_begin(const SortArgs& a, const unsigned* off) {
+  return a.seg_valid ? (long long)off[a.ntile] : 0;
+}
+
+// Stayers of each segment (one plain store per tile) and movers per destination
+// tile (an atomic per run of equal tiles in a wave).
 template <class AS>
 __global__ __launch_bounds__(256) void k_tile_hist(AS as_) {
   const SortArgs& a = kargs(as_);
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+  const int lane = threadIdx.x & 63;
+  if ((int)blockIdx.x < a.ntile) {
+    __shared__ unsigned stay_sum;
+    if (threadIdx.x == 0) stay_sum = 0;
+    __syncthreads();
+    unsigned lo, hi;
+    seg_bounds(a, a.toff, lo, hi);
+    unsigned stay = 0;  // lane 0 of each wave
+    for (unsigned base = lo; base < hi; base += blockDim.x) {
+      const unsigned q = base + threadIdx.x;
+      int t = -1;
+      bool st = false;
+      if (q < hi) {
+        const uint32_t pk = a.P.pk[q];
+        if (pk != PK_DEAD) {
+          t = tile_of(a, pk);
+          st = t == (int)blockIdx.x;
+        }
+      }
+      const unsigned long long m = __ballot(st);
+      if (lane == 0) stay += (unsigned)__popcll(m);
+      wave_run_count(a.tcount, st ? -1 : t);
+    }
+    if (lane == 0 && stay) atomicAdd(&stay_sum, stay);
+    __syncthreads();
+    if (threadIdx.x == 0) a.tstay[blockIdx.x] = stay_sum;
+    return;
+  }
+  const long long stride = (long long)(gridDim.x - a.ntile) * blockDim.x;
+  for (long long base = append_begin(a, a.toff) + (long long)(blockIdx.x - a.ntile) * blockDim.x; base < a.n;
+       base += stride) {
     const long long q = base + threadIdx.x;
     int t = -1;
     if (q < a.n) {
@@ -347,8 +393,9 @@ __global__ __launch_bounds__(256) void k_tile_hist(AS as_) {
   }
 }
 
-// One block of 1024 threads: exclusive scan of the tile counts into toff, then the
-// counts and cursors are cleared for the next sort.
+// One block of 1024 threads: the tile counts (stayers + movers) are scanned into
+// toff, whose previous contents (the segments the scatter reads) go to toff_old;
+// the mover counts and cursors are cleared for the next sort.
 template <class AS>
 __global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
   const SortArgs& a = kargs(as_);
@@ -357,8 +404,9 @@ __global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
   const int tid = threadIdx.x;
   const int chunk = (nt + 1023) / 1024;
   const int b0 = tid * chunk, b1 = min(nt, b0 + chunk);
+  const unsigned old_total = a.toff[nt];
   unsigned loc = 0;
-  for (int t = b0; t < b1; ++t) loc += a.tcount[t];
+  for (int t = b0; t < b1; ++t) loc += a.tstay[t] + a.tcount[t];
   // inclusive scan of loc over the block
   const int lane = tid & 63, w = tid >> 6;
   unsigned v = loc;
@@ -372,7 +420,8 @@ __global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
   for (int q = 0; q < w; ++q) woff += wsum[q];
   unsigned run = woff + v - loc;  // exclusive prefix of this thread's chunk
   for (int t = b0; t < b1; ++t) {
-    const unsigned cnt = a.tcount[t];
+    const unsigned cnt = a.tstay[t] + a.tcount[t];
+    a.toff_old[t] = a.seg_valid ? a.toff[t] : 0u;
     a.toff[t] = run;
     run += cnt;
     a.tcount[t] = 0;
@@ -381,24 +430,46 @@ __global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
   if (tid == 1023) {
     unsigned tot = 0;
     for (int q = 0; q < 16; ++q) tot += wsum[q];
+    a.toff_old[nt] = a.seg_valid ? old_total : 0u;
     a.toff[nt] = tot;
   }
 }
 
+// Stayers of tile t land at toff[t] + their rank among the segment's stayers
+// (a block scan: no atomics, the segment's order kept); movers into tile t at
+// toff[t] + tstay[t] + a cursor (one atomic per run of equal tiles in a wave).
+// Each particle is read before its slot is known, so the loads overlap the
+// cursor atomic's round trip.
 template <class AS>
 __global__ __launch_bounds__(256) void k_tile_scatter(AS as_) {
   const SortArgs& a = kargs(as_);
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+  const bool seg = (int)blockIdx.x < a.ntile;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __shared__ unsigned wtot[16];
+  long long base, end, stride;
+  unsigned ranked = 0;  // stayers of the segment placed so far
+  unsigned dst0 = 0;
+  if (seg) {
+    unsigned lo, hi;
+    seg_bounds(a, a.toff_old, lo, hi);
+    base = lo;
+    end = hi;
+    stride = blockDim.x;
+    dst0 = a.toff[blockIdx.x];
+  } else {
+    base = append_begin(a, a.toff_old) + (long long)(blockIdx.x - a.ntile) * blockDim.x;
+    end = a.n;
+    stride = (long long)(gridDim.x - a.ntile) * blockDim.x;
+  }
+  for (; base < end; base += stride) {
     const long long q = base + threadIdx.x;
     int t = -1;
     uint32_t pk = PK_DEAD;
-    if (q < a.n) {
+    if (q < end) {
       pk = a.P.pk[q];
       if (pk != PK_DEAD) t = tile_of(a, pk);
     }
-    // the particle is read before the slot is known, so the loads overlap the
-    // cursor atomic's round trip
+    const bool st = seg && t == (int)blockIdx.x;
     double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0, en = 0, ch = 0;
     int32_t itet = 0, isym = 0;
     uint64_t key = 0;
@@ -406,9 +477,25 @@ __global__ __launch_bounds__(256) void k_tile_scatter(AS as_) {
       x = a.P.x[q]; y = a.P.y[q]; z = a.P.z[q]; kx = a.P.kx[q]; ky = a.P.ky[q]; kz = a.P.kz[q];
       en = a.P.energy[q]; ch = a.P.charge[q]; itet = a.P.itet[q]; isym = a.P.isym[q]; key = a.P.key[q];
     }
-    const unsigned r = wave_run_add(a.tcursor, t);
+    const unsigned r = wave_run_add(a.tcursor, st ? -1 : t);
+    size_t d = 0;
+    if (seg) {
+      // rank of this stayer: stayers of the earlier waves, then of the lanes below
+      const unsigned long long m = __ballot(st);
+      if (lane == 0) wtot[w] = (unsigned)__popcll(m);
+      __syncthreads();
+      unsigned before = 0, total = 0;
+      for (int o = 0; o < nw; ++o) {
+        const unsigned c = wtot[o];
+        if (o < w) before += c;
+        total += c;
+      }
+      __syncthreads();
+      if (st) d = (size_t)dst0 + ranked + before + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+      ranked += total;
+    }
     if (t < 0) continue;
-    const size_t d = (size_t)a.toff[t] + r;
+    if (!st) d = (size_t)a.toff[t] + a.tstay[t] + r;
     a.Q.x[d] = x; a.Q.y[d] = y; a.Q.z[d] = z;
     a.Q.kx[d] = kx; a.Q.ky[d] = ky; a.Q.kz[d] = kz;
     a.Q.energy[d] = en; a.Q.charge[d] = ch;
@@ -1407,12 +1494,17 @@ hipError_t emc_launch_clones_dev(const FlyArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 // sort P into Q by tile; toff[ntile] is the live count afterwards
-hipError_t emc_launch_tile_sort(const SortArgs& a, hipStream_t s) {
-  // 2048 blocks striding over the population: the per-particle work is short, and
-  // one block per 256 particles (12 k blocks at 3 M) costs more in block dispatch
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_hist<SortArgs>, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
+// One block per tile segment, then blocks striding over the movers outside the
+// segments: at most 2048 (the per-particle work is short, and one block per 256
+// particles, 12 k blocks at 3 M, costs more in block dispatch).
+static unsigned sort_grid(int ntile, long long nappend) {
+  return (unsigned)ntile + (unsigned)grid_for(nappend > 0 ? nappend : 1, 256, 2048);
+}
+hipError_t emc_launch_tile_sort(const SortArgs& a, long long nappend, hipStream_t s) {
+  const unsigned g = sort_grid(a.ntile, nappend);
+  hipLaunchKernelGGL(k_tile_hist<SortArgs>, dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_tile_scan<SortArgs>, dim3(1), dim3(1024), 0, s, a);
-  if (a.n > 0) hipLaunchKernelGGL(k_tile_scatter<SortArgs>, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tile_scatter<SortArgs>, dim3(g), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
@@ -1595,12 +1687,12 @@ hipError_t emc_launch_pcg_iter_b(const BatchTables& t, const MgArgs& shape, int 
   else launch_vcycle<MgArgs, const MgArgs*>(shape, (it & 1) ? t.mg_odd : t.mg_even, nblocks, K, s);
   return hipGetLastError();
 }
-hipError_t emc_launch_tile_sort_b(const BatchTables& t, long long nmax, hipStream_t s) {
+hipError_t emc_launch_tile_sort_b(const BatchTables& t, int ntile, long long nappend_max, hipStream_t s) {
   const unsigned K = (unsigned)t.K;
-  if (nmax > 0) hipLaunchKernelGGL(k_tile_hist<const SortArgs*>, dim3(grid_for(nmax, 256, 2048), K), dim3(256), 0, s, t.sort);
+  const unsigned g = sort_grid(ntile, nappend_max);
+  hipLaunchKernelGGL(k_tile_hist<const SortArgs*>, dim3(g, K), dim3(256), 0, s, t.sort);
   hipLaunchKernelGGL(k_tile_scan<const SortArgs*>, dim3(1, K), dim3(1024), 0, s, t.sort);
-  if (nmax > 0)
-    hipLaunchKernelGGL(k_tile_scatter<const SortArgs*>, dim3(grid_for(nmax, 256, 2048), K), dim3(256), 0, s, t.sort);
+  hipLaunchKernelGGL(k_tile_scatter<const SortArgs*>, dim3(g, K), dim3(256), 0, s, t.sort);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit_b(const BatchTables& t, int ntile, size_t nnode, hipStream_t s) {
