@@ -770,17 +770,34 @@ __global__ void k_field(AS as_) {
 
 // ------------------------------------------------------------------ PCG
 // Operator: y = diag x + sum_dir c_dir(n) x(n+dir) + c_dir(n-dir) x(n-dir).
+// Keeps the compiler from moving memory accesses across this point: the loads
+// above are all issued before the arithmetic below waits for the first of them
+// (otherwise the scheduler interleaves load, wait, multiply-add per neighbour).
+__device__ inline void mem_fence_compiler() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 template <class PA>
 __device__ inline double apply_op(const PA& a, const typename PA::real* __restrict__ x, size_t n, int i, int j,
                                   int k) {
+  // every neighbour is loaded from a clamped address and a missing one adds 0.0,
+  // so the loads carry no branches and issue together (the sum is the same, term
+  // for term, as over the present neighbours only)
   const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
-  double v = (double)a.diag[n] * (double)x[n];
-  if (i + 1 < a.pnx) v += (double)a.cx[n] * (double)x[n + sx];
-  if (i > 0) v += (double)a.cx[n - sx] * (double)x[n - sx];
-  if (j + 1 < a.pny) v += (double)a.cy[n] * (double)x[n + sy];
-  if (j > 0) v += (double)a.cy[n - sy] * (double)x[n - sy];
-  if (k + 1 < a.pnz) v += (double)a.cz[n] * (double)x[n + 1];
-  if (k > 0) v += (double)a.cz[n - 1] * (double)x[n - 1];
+  const bool ip = i + 1 < a.pnx, im = i > 0, jp = j + 1 < a.pny, jm = j > 0, kp = k + 1 < a.pnz, km = k > 0;
+  const size_t nim = im ? n - sx : n, njm = jm ? n - sy : n, nkm = km ? n - 1 : n;
+  const double x0 = x[n], x1 = x[ip ? n + sx : n], x2 = x[nim], x3 = x[jp ? n + sy : n], x4 = x[njm],
+               x5 = x[kp ? n + 1 : n], x6 = x[nkm];
+  const double c0 = a.diag[n], c1 = a.cx[n], c2 = a.cx[nim], c3 = a.cy[n], c4 = a.cy[njm], c5 = a.cz[n], c6 = a.cz[nkm];
+  mem_fence_compiler();
+  double v = c0 * x0;
+  v += ip ? c1 * x1 : 0.0;
+  v += im ? c2 * x2 : 0.0;
+  v += jp ? c3 * x3 : 0.0;
+  v += jm ? c4 * x4 : 0.0;
+  v += kp ? c5 * x5 : 0.0;
+  v += km ? c6 * x6 : 0.0;
   return v;
 }
 
@@ -941,27 +958,51 @@ __global__ __launch_bounds__(256) void k_pcg_spmvp(AS as_, int it) {
   }
   const size_t sx = a.pnz, sy = (size_t)a.pnx * a.pnz;
   double pq = 0;
-  // z in the CG's precision, or the FP32 cycle's output
-  auto pass = [&](const auto* __restrict__ z) {
-    auto pv = [&](size_t m) -> double { return first ? (double)z[m] : (double)z[m] + beta * (double)p_in[m]; };
+  // z in the CG's precision, or the FP32 cycle's output.  Every neighbour is
+  // loaded from a clamped address and a missing one adds 0.0, so the loads of a
+  // node issue together (as in apply_op).
+  auto pass = [&](const auto* __restrict__ z, auto first_c) {
+    constexpr bool FIRST = decltype(first_c)::value;
     GRID_LOOP(n, a.n) {
       int i, j, k;
       node_ijk(a, n, i, j, k);
-      const double pn = pv(n);
-      double q = (double)a.diag[n] * pn;
-      if (i + 1 < a.pnx) q += (double)a.cx[n] * pv(n + sx);
-      if (i > 0) q += (double)a.cx[n - sx] * pv(n - sx);
-      if (j + 1 < a.pny) q += (double)a.cy[n] * pv(n + sy);
-      if (j > 0) q += (double)a.cy[n - sy] * pv(n - sy);
-      if (k + 1 < a.pnz) q += (double)a.cz[n] * pv(n + 1);
-      if (k > 0) q += (double)a.cz[n - 1] * pv(n - 1);
-      p_out[n] = (R)pn;
+      const bool ip = i + 1 < a.pnx, im = i > 0, jp = j + 1 < a.pny, jm = j > 0, kp = k + 1 < a.pnz, km = k > 0;
+      const size_t nb[7] = {n, ip ? n + sx : n, im ? n - sx : n, jp ? n + sy : n, jm ? n - sy : n,
+                            kp ? n + 1 : n, km ? n - 1 : n};
+      double zv[7], pv[7];
+#pragma unroll
+      for (int o = 0; o < 7; ++o) zv[o] = z[nb[o]];
+      if (!FIRST) {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) pv[o] = p_in[nb[o]];
+      }
+      const double c0 = a.diag[n], c1 = a.cx[n], c2 = a.cx[nb[2]], c3 = a.cy[n], c4 = a.cy[nb[4]], c5 = a.cz[n],
+                   c6 = a.cz[nb[6]];
+      mem_fence_compiler();
+      double p[7];
+#pragma unroll
+      for (int o = 0; o < 7; ++o) p[o] = FIRST ? zv[o] : zv[o] + beta * pv[o];
+      double q = c0 * p[0];
+      q += ip ? c1 * p[1] : 0.0;
+      q += im ? c2 * p[2] : 0.0;
+      q += jp ? c3 * p[3] : 0.0;
+      q += jm ? c4 * p[4] : 0.0;
+      q += kp ? c5 * p[5] : 0.0;
+      q += km ? c6 * p[6] : 0.0;
+      p_out[n] = (R)p[0];
       a.q[n] = (R)q;
-      pq += pn * q;
+      pq += p[0] * q;
     }
   };
-  if (a.z32) pass(a.z32);
-  else pass((const R*)a.z);
+  using T1 = std::integral_constant<bool, true>;
+  using T0 = std::integral_constant<bool, false>;
+  if (a.z32) {
+    if (first) pass(a.z32, T1{});
+    else pass(a.z32, T0{});
+  } else {
+    if (first) pass((const R*)a.z, T1{});
+    else pass((const R*)a.z, T0{});
+  }
   block_reduce1(pq, a.part_pq);
 }
 
@@ -1003,14 +1044,6 @@ __device__ inline void lvl_ijk(const LV& L, size_t n, int& i, int& j, int& k) {
   const size_t t = n / L.nz;
   i = (int)(t % L.nx);
   j = (int)(t / L.nx);
-}
-
-// Keeps the compiler from moving memory accesses across this point: the loads
-// above are all issued before the arithmetic below waits for the first of them
-// (otherwise the scheduler interleaves load, wait, multiply-add per neighbour).
-__device__ inline void mem_fence_compiler() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_sched_barrier(0);
 }
 
 // (A_l x)(n) with x given by an accessor; level 0 is the 7-point operator in FP64,
