@@ -9,16 +9,17 @@ finfet/ids_vs_step2.png and finfet/heat6999.vtr):
 
                                    reference   here (seed 1)   bound
   contact 1, window 0..999         7.788 pA    7.768 pA        1 %
-  contact 1, window 1000..1999     3.730 pA    3.821 pA        4 %
-  contact 1, steady (12 windows)   3.427 pA    3.511 pA        4.5 %
-  contact 0, window 0..999        -3.443 pA   -3.421 pA        2 %
-  contact 0, windows 3..7         -3.463 pA   -3.575 pA        4.5 %
+  contact 1, window 1000..1999     3.730 pA    3.820 pA        4 %
+  contact 1, steady (12 windows)   3.427 pA    3.520 pA        4.5 %
+  contact 0, window 0..999        -3.443 pA   -3.420 pA        2 %
+  contact 0, windows 3..7         -3.463 pA   -3.581 pA        4.5 %
   heat, y profile correlation      1           0.999           > 0.99
-  heat, total                      1           1.07            15 %
+  heat, total                      1           1.08            15 %
 
-Seeds 1 and 2 and runs without MultipleRefresh give the same picture: the first
-window agrees within 0.3 % (contact 1), the steady drain current is 2.5 % (± 0.3 %)
-above the reference's and the source current 3 % above.  The first window is the
+Seeds 1 and 2, runs without MultipleRefresh and the all-FP64 solve give the same
+picture: the first window agrees within 0.3 % (contact 1), the steady drain
+current is 2.5 to 2.7 % above the reference's and the source current 3 to 3.4 %
+above.  The first window is the
 discharge of the sampled population into the contacts and is set by the initial
 state, the fields and the flight; the steady windows add the contacts' particle
 exchange and MultipleRefresh over thousands of steps.  The bounds hold the
