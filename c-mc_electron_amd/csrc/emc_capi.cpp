@@ -1097,6 +1097,39 @@ int emc_band_locate(emc_ctx* c, int64_t n, const double* k, const double* energy
   return EMC_OK;
 }
 
+int emc_band_final_state(emc_ctx* c, int64_t n, const uint64_t* key, uint64_t step, const double* energy,
+                         const int* iband, const int* isym, double* k, int* itet, int* ndraws, int* selfsc) {
+  if (!c) return EMC_ERR_ARG;
+  if (n < 0 || (n > 0 && (!key || !energy || !iband || !isym || !k || !itet || !ndraws || !selfsc)))
+    return fail(c, EMC_ERR_ARG, "band_final_state: bad arguments");
+  const Band& b = c->bb->band;
+  for (int64_t q = 0; q < n; ++q)
+    if (iband[q] < 0 || iband[q] >= MNB || isym[q] < 0 || isym[q] >= 48 || itet[q] < 0 || itet[q] >= b.nt)
+      return fail(c, EMC_ERR_ARG, "band_final_state: band, symmetry or tetrahedron index out of range");
+  BandDev B{};
+  fill_band_dev_host(c, B);
+  MeshDev M{};
+  fill_mesh_dev_scalars(c, M);
+  FieldsDev F{};
+  TallyDev T{};
+  Flight f(B, M, F, T);
+  for (int64_t q = 0; q < n; ++q) {
+    f.rng.init(key[q], step);
+    f.energy = energy[q];
+    f.iband = iband[q];
+    f.isym = isym[q];
+    f.itet = itet[q];
+    f.kx = f.ky = f.kz = 0.0;
+    f.Flag_SelfScatter = 0;
+    f.get_state();
+    k[3 * q] = f.kx; k[3 * q + 1] = f.ky; k[3 * q + 2] = f.kz;
+    itet[q] = f.itet;
+    ndraws[q] = (int)(f.rng.block * 2u - (f.rng.have ? 1u : 0u));
+    selfsc[q] = f.Flag_SelfScatter;
+  }
+  return EMC_OK;
+}
+
 int emc_mesh_view_get(emc_ctx* c, emc_mesh_view* v) {
   if (!c || !v) return EMC_ERR_ARG;
   const Device& d = c->dev;

@@ -306,6 +306,23 @@ class Engine:
                                              _ptr(it, C.c_int), _ptr(sy, C.c_int)))
         return it, sy
 
+    def band_final_state(self, key, step, energy, iband, isym, itet):
+        """get_state with getK on the host tables for n draws: returns (k [n, 3],
+        itet, ndraws, selfsc)."""
+        key = np.ascontiguousarray(key, np.uint64).reshape(-1)
+        n = len(key)
+        e = np.ascontiguousarray(energy, np.float64).reshape(n)
+        ib = np.ascontiguousarray(iband, np.int32).reshape(n)
+        sy = np.ascontiguousarray(isym, np.int32).reshape(n)
+        it = np.array(itet, np.int32).reshape(n).copy()
+        k = np.zeros((n, 3))
+        nd = np.zeros(n, np.int32)
+        sc = np.zeros(n, np.int32)
+        self._check(self.lib.emc_band_final_state(self.ctx, n, _ptr(key, C.c_uint64), step, _ptr(e, C.c_double),
+                                                  _ptr(ib, C.c_int), _ptr(sy, C.c_int), _ptr(k, C.c_double),
+                                                  _ptr(it, C.c_int), _ptr(nd, C.c_int), _ptr(sc, C.c_int)))
+        return k, it, nd, sc
+
     def mg_levels(self):
         """The Poisson preconditioner's multigrid hierarchy: list of (dims, a27 or
         None, omega) per level; a27 has shape (27, n)."""

@@ -257,6 +257,17 @@ int emc_checkpoint_load(emc_ctx *ctx, const char *path);
 int emc_band_locate(emc_ctx *ctx, int64_t n, const double *k, const double *energy, const int *iband, int *itet,
                     int *isym);
 
+/* get_state with getK (src/ParticleUtils.cpp:48-284) on the host tables, the code
+ * the flight kernels run: n final-state draws, the q-th at energy energy[q] in
+ * band iband[q] with symmetry operation isym[q], its random numbers from the
+ * stream of (key[q], step).  itet[q] is read as the particle's tetrahedron and
+ * receives the chosen one, k[3q .. 3q+2] the wave vector (normalised; 0 on a
+ * self-scattering), ndraws[q] the random numbers used and selfsc[q] 1 where the
+ * draw self-scatters (no list entry, or the 10000-try limit).  No counterpart
+ * in the reference: a seam for tests, like emc_band_locate.  Needs no GPU. */
+int emc_band_final_state(emc_ctx *ctx, int64_t n, const uint64_t *key, uint64_t step, const double *energy,
+                         const int *iband, const int *isym, double *k, int *itet, int *ndraws, int *selfsc);
+
 /* Initial population (host sampling as init_particle_data), uploaded to the GPU
  * when one is attached.  n_out receives the particle count. */
 int emc_particles_init(emc_ctx *ctx, int64_t *n_out);
