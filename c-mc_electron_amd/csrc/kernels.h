@@ -115,7 +115,7 @@ struct DepositArgs {
   size_t nnode, ncell;
   int ntx, nty, ntz, ntile;
   const unsigned* toff;       // [ntile + 1]
-  unsigned long long* blk;    // [ntile][DEP_NQ][TILE_NODES] per-tile node sums
+  unsigned long long* blk;    // [ntile][TILE_NODES][DEP_NQ] per-tile node sums
   unsigned long long *node_q, *node_stat, *cell_q, *cell_n;  // any may be null
   int stat_accumulate;        // 1: node_stat += this step's sums, 0: node_stat = sums
   int pad0;

@@ -623,8 +623,13 @@ __global__ __launch_bounds__(256) void k_deposit_tile(AS as_) {
     }
   }
   if (p1 > p0) {
+    // node-major in memory ([node][quantity]): the gather reads a node's sums
+    // from one 88-byte run
     unsigned long long* b = a.blk + (size_t)t * DEP_NQ * TILE_NODES;
-    for (int e = threadIdx.x; e < DEP_NQ * TILE_NODES; e += blockDim.x) b[e] = sn[e];
+    for (int e = threadIdx.x; e < DEP_NQ * TILE_NODES; e += blockDim.x) {
+      const int ln = e / DEP_NQ, q = e - ln * DEP_NQ;
+      b[e] = sn[q * TILE_NODES + ln];
+    }
   }
 }
 
@@ -652,9 +657,9 @@ __global__ __launch_bounds__(256) void k_deposit_gather(AS as_) {
           const int t = (ty[b] * a.ntx + tx[aa]) * a.ntz + tz[c];
           if (a.toff[t + 1] == a.toff[t]) continue;
           const int ln = (ly[b] * (TILE_I + 1) + lx[aa]) * (TILE_K + 1) + lz[c];
-          const unsigned long long* blk = a.blk + (size_t)t * DEP_NQ * TILE_NODES + ln;
+          const unsigned long long* blk = a.blk + ((size_t)t * TILE_NODES + ln) * DEP_NQ;
 #pragma unroll
-          for (int q = 0; q < DEP_NQ; ++q) s[q] += blk[(size_t)q * TILE_NODES];
+          for (int q = 0; q < DEP_NQ; ++q) s[q] += blk[q];
         }
     if (a.node_q) a.node_q[n] = s[0];
     if (a.rhs.rhs) rhs_node(a.rhs, n, s[0]);
