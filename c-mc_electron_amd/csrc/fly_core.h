@@ -210,7 +210,14 @@ struct Flight {
     const int b0 = par_type == 0 ? B.bandofE : B.bandofH;
     const int nb = par_type == 0 ? B.nbandE : B.nbandH;
     itet = 0;
-    for (;;) {
+    // Every lane of the wave leaves this loop together (lanes that are done wait
+    // without drawing), as in get_state's rejection loop: with lanes leaving on
+    // different iterations and the random-number state live, the GPU build gave
+    // other draws than the host build for most of the population (gfx950), after
+    // changes elsewhere in the flight code.
+    bool more = true;
+    while (EMC_ANY(more)) {
+      if (!more) continue;
       energy = -dm_log(1 - Random() * tmp_const);
       const double ddos = caldossum(energy, par_type);
       double rdos = Random() * B.dosmax[par_type];
@@ -225,7 +232,7 @@ struct Flight {
       isym = ((int)(Random() * 48)) % 48;
       get_state();
       if (Flag_SelfScatter) continue;
-      break;
+      more = false;
     }
   }
   EMC_HD EMC_INL double caldostetmax(double e, int ib) const {
