@@ -343,6 +343,52 @@ def test_flight_parity_large_population():
         e.close()
 
 
+def test_flight_parity_near_band_edge(eng):
+    """Electrons placed on the iso-energy surfaces just above the intervalley phonon
+    energies (20.5, 48.9 and 62.7 meV; their wave vectors and tetrahedra from
+    emc_band_final_state), in a weak field, twenty steps: phonon emission leaves many
+    of them 1-3 meV above the band edge, where get_state's rejection loop takes
+    hundreds to thousands of tries, while the other lanes of the wave take their own
+    paths.  Population and tallies equal the oracle's bit for bit."""
+    eng.particles_init()
+    p = eng.download()
+    n = p.n
+    el = np.nonzero((p.i[:n] >= 0) & (p.ptype[:n] == 0))[0]
+    rng = np.random.default_rng(21)
+    targets = np.array([0.0205, 0.0489, 0.0627])
+    energy = targets[rng.integers(0, len(targets), len(el))]
+    isym = rng.integers(0, 48, len(el)).astype(np.int32)
+    band = np.full(len(el), eng.band.bandof[0], np.int32)
+    k, itet, nd, sc = eng.band_final_state(p.key[el], 12345, energy, band, isym, np.zeros(len(el), np.int32))
+    ok = sc == 0
+    q = el[ok]
+    p.kx[q], p.ky[q], p.kz[q] = k[ok, 0], k[ok, 1], k[ok, 2]
+    p.energy[q] = energy[ok]
+    p.itet[q] = itet[ok]
+    p.isym[q] = isym[ok]
+    assert ok.sum() > 0.9 * len(el)
+    # the final states these emissions reach take long loops
+    kk, _, nd2, _ = eng.band_final_state(p.key[el[:400]], 7, np.full(400, 0.0015), band[:400], isym[:400],
+                                         np.zeros(400, np.int32))
+    assert np.median(nd2) > 100
+    eng.upload(p)
+    f = uniform_fields(eng, ex=0.5, ey=2.0, ez=-0.5)
+    eng.fields_set(f)
+    eng.tallies_reset()
+    po = eng.download()
+    phonons = 0
+    for step in range(20):
+        print(f"[gpu] near-band-edge parity step {step}", flush=True)
+        tg = eng.fly_step(300 + step)
+        pg = eng.download()
+        po, to, _ = oracle_lib.fly_step(eng, po, po.n, f, 300 + step)
+        assert_same_tallies(tg, to, f"near-edge step {step}")
+        assert_same_population(pg, po, f"near-edge step {step}")
+        po = sort_by_key(po)
+        phonons += tg.n_phonon
+    assert phonons > 100
+
+
 def test_multiple_refresh_grows_the_population():
     """A refresh whose output needs more slots than the population was given:
     the buffers grow (new particle buffers, flight lists and work arrays), the
