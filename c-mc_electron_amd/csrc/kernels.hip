@@ -465,18 +465,18 @@ __global__ __launch_bounds__(256) void k_tile_scatter(AS as_) {
     const long long q = base + threadIdx.x;
     int t = -1;
     uint32_t pk = PK_DEAD;
-    if (q < end) {
-      pk = a.P.pk[q];
-      if (pk != PK_DEAD) t = tile_of(a, pk);
-    }
-    const bool st = seg && t == (int)blockIdx.x;
     double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0, en = 0, ch = 0;
     int32_t itet = 0, isym = 0;
     uint64_t key = 0;
-    if (t >= 0) {
+    if (q < end) {
+      // every field loaded with pk (an empty slot's are read and dropped), so the
+      // loads do not wait for pk
+      pk = a.P.pk[q];
       x = a.P.x[q]; y = a.P.y[q]; z = a.P.z[q]; kx = a.P.kx[q]; ky = a.P.ky[q]; kz = a.P.kz[q];
       en = a.P.energy[q]; ch = a.P.charge[q]; itet = a.P.itet[q]; isym = a.P.isym[q]; key = a.P.key[q];
+      if (pk != PK_DEAD) t = tile_of(a, pk);
     }
+    const bool st = seg && t == (int)blockIdx.x;
     const unsigned r = wave_run_add(a.tcursor, st ? -1 : t);
     size_t d = 0;
     if (seg) {
