@@ -439,9 +439,10 @@ __global__ __launch_bounds__(1024) void k_tile_scan(AS as_) {
 // (a block scan: no atomics, the segment's order kept); movers into tile t at
 // toff[t] + tstay[t] + a cursor (one atomic per run of equal tiles in a wave).
 // Each particle is read before its slot is known, so the loads overlap the
-// cursor atomic's round trip.
+// cursor atomic's round trip.  512 threads, so that most segments (a few hundred
+// particles) take one pass of the loop.
 template <class AS>
-__global__ __launch_bounds__(256) void k_tile_scatter(AS as_) {
+__global__ __launch_bounds__(512) void k_tile_scatter(AS as_) {
   const SortArgs& a = kargs(as_);
   const bool seg = (int)blockIdx.x < a.ntile;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1542,7 +1543,7 @@ hipError_t emc_launch_tile_sort(const SortArgs& a, long long nappend, hipStream_
   const unsigned g = sort_grid(a.ntile, nappend);
   hipLaunchKernelGGL(k_tile_hist<SortArgs>, dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_tile_scan<SortArgs>, dim3(1), dim3(1024), 0, s, a);
-  hipLaunchKernelGGL(k_tile_scatter<SortArgs>, dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tile_scatter<SortArgs>, dim3(g), dim3(512), 0, s, a);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit(const DepositArgs& a, hipStream_t s) {
@@ -1730,7 +1731,7 @@ hipError_t emc_launch_tile_sort_b(const BatchTables& t, int ntile, long long nap
   const unsigned g = sort_grid(ntile, nappend_max);
   hipLaunchKernelGGL(k_tile_hist<const SortArgs*>, dim3(g, K), dim3(256), 0, s, t.sort);
   hipLaunchKernelGGL(k_tile_scan<const SortArgs*>, dim3(1, K), dim3(1024), 0, s, t.sort);
-  hipLaunchKernelGGL(k_tile_scatter<const SortArgs*>, dim3(g, K), dim3(256), 0, s, t.sort);
+  hipLaunchKernelGGL(k_tile_scatter<const SortArgs*>, dim3(g, K), dim3(512), 0, s, t.sort);
   return hipGetLastError();
 }
 hipError_t emc_launch_deposit_b(const BatchTables& t, int ntile, size_t nnode, hipStream_t s) {
