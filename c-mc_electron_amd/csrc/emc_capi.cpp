@@ -1098,7 +1098,8 @@ int emc_band_locate(emc_ctx* c, int64_t n, const double* k, const double* energy
 }
 
 int emc_band_final_state(emc_ctx* c, int64_t n, const uint64_t* key, uint64_t step, const double* energy,
-                         const int* iband, const int* isym, double* k, int* itet, int* ndraws, int* selfsc) {
+                         const int* iband, const int* isym, double* k, int* itet, int* ndraws, int* selfsc,
+                         int serial) {
   if (!c) return EMC_ERR_ARG;
   if (n < 0 || (n > 0 && (!key || !energy || !iband || !isym || !k || !itet || !ndraws || !selfsc)))
     return fail(c, EMC_ERR_ARG, "band_final_state: bad arguments");
@@ -1121,7 +1122,8 @@ int emc_band_final_state(emc_ctx* c, int64_t n, const uint64_t* key, uint64_t st
     f.itet = itet[q];
     f.kx = f.ky = f.kz = 0.0;
     f.Flag_SelfScatter = 0;
-    f.get_state();
+    if (serial < 0) f.get_state();
+    else f.get_state_chunked(serial);
     k[3 * q] = f.kx; k[3 * q + 1] = f.ky; k[3 * q + 2] = f.kz;
     itet[q] = f.itet;
     ndraws[q] = (int)(f.rng.block * 2u - (f.rng.have ? 1u : 0u));

@@ -159,6 +159,15 @@ struct Flight {
   }
   EMC_HD EMC_INL double fastsurf(double e, int it) { return fastsurf_r(e, it, B.sr[it]); }
   EMC_HD EMC_INL double fastsurf_r(double e, int it, const SurfRec& s) {
+    double r = fastsurf_signed(e, it, s);
+    if (r < 0.0) {
+      if (T.counters) count(CNT_DROP0 + 6);
+      r = 0.0;
+    }
+    return r;
+  }
+  // FASTSURF before the clamp of a negative value (which fastsurf_r tallies)
+  EMC_HD EMC_INL double fastsurf_signed(double e, int it, const SurfRec& s) {
     const double e0 = s.e[0], e1 = s.e[1], e2 = s.e[2], e3 = s.e[3];
     double r = 0.0;
     if (e > e0 && e < e3) {
@@ -171,10 +180,6 @@ struct Flight {
           r = surf(e, it);
       } else
         r = s.fachigh * (e - e3) * (e - e3);
-    }
-    if (r < 0.0) {
-      if (T.counters) count(CNT_DROP0 + 6);
-      r = 0.0;
     }
     return r;
   }
@@ -477,6 +482,176 @@ struct Flight {
     }
     if (count2 >= 10000) Flag_SelfScatter = 1;
     else { Flag_SelfScatter = 0; getK(); }
+  }
+
+  // get_state() in pieces, for its wave-parallel form in the slow flight kernel
+  // (get_state_wave) and for get_state_chunked(), the same search on one thread.
+  // Tries that run long are taken GS_CHUNK draws at a time: draw n of a particle
+  // depends only on its key, the step and n (Philox is counter based), so every
+  // position of a chunk can be formed at once and evaluated both as the first
+  // draw of a try (the tetrahedron it picks, whether that spans the energy, its
+  // DOS weight) and as the acceptance draw of a try that started one position
+  // earlier; gs_walk() then follows the loop's control flow over those bits.  The
+  // tries, the draws consumed, the tetrahedron and the tallies are get_state()'s.
+  static constexpr int GS_CAP = 10000;  // get_state()'s bound on the tries
+  static constexpr int GS_CHUNK = 128;  // draws per chunk (two per lane of a wave)
+  // get_state() up to its loop: false when no draw is made (self-scattering set)
+  EMC_HD EMC_INL bool gs_begin(double& maxdostet, int& nted, int& base, bool& small) {
+    const double e = energy;
+    if (e <= 0.0) { Flag_SelfScatter = 1; return false; }
+    maxdostet = caldostetmax(e, iband);
+    int itab = (int)(e * B.dlists);
+    small = itab < MWLES;
+    if (!small) {
+      itab = (int)(e * B.dlist);
+      if (itab >= MWLE) itab = MWLE - 1;
+    }
+    nted = small ? B.ntlists[itab * MNB + iband] : B.ntlist[itab * MNB + iband];
+    base = small ? B.ptlists[itab * MNB + iband] : B.ptlist[itab * MNB + iband];
+    if (nted == 0) { Flag_SelfScatter = 1; return false; }
+    return true;
+  }
+  // one try of the loop: true when the search goes on
+  EMC_HD EMC_INL bool gs_try(double e, double maxdostet, int nted, int base, bool small, int& count2) {
+    const int* list = small ? B.tlists : B.tlist;
+    itet = list[(int)(nted * Random()) + base];
+    count2++;
+    const SurfRec& sr = B.sr[itet];
+    if ((count2 < GS_CAP) && !((sr.e[0] < e) && (sr.e[3] > e))) return true;
+    const double dostet = fastsurf_r(e, itet, sr) / sr.vgt;
+    if (((Random() * maxdostet) > dostet) && (count2 < GS_CAP)) return true;
+    return false;
+  }
+  // get_state() after its loop
+  EMC_HD EMC_INL void gs_end(int count2) {
+    if (count2 >= GS_CAP) Flag_SelfScatter = 1;
+    else { Flag_SelfScatter = 0; getK(); }
+  }
+  // draw u taken as the first draw of a try: the tetrahedron, whether it spans
+  // e, its DOS weight when it does and whether FASTSURF was negative there
+  EMC_HD EMC_INL void gs_eval(double e, int nted, int base, bool small, double u, int& it, bool& span, bool& neg,
+                              double& dostet) {
+    const int* list = small ? B.tlists : B.tlist;
+    it = list[(int)(nted * u) + base];
+    const SurfRec& sr = B.sr[it];
+    span = (sr.e[0] < e) && (sr.e[3] > e);
+    neg = false;
+    dostet = 0.0;
+    if (span) {
+      double r = fastsurf_signed(e, it, sr);
+      if (r < 0.0) { neg = true; r = 0.0; }
+      dostet = r / sr.vgt;
+    }
+  }
+  // The loop over one chunk.  Bit q of span / acc / neg (q < 64 in word 0, else
+  // word 1): the try whose first draw is position q picks a tetrahedron spanning
+  // the energy / accepts it with the draw at q + 1 / has a negative FASTSURF.
+  // count2 tries were made before the chunk.  A try that needs the draw after
+  // the chunk is left to the next one.
+  struct GsWalk {
+    int consumed;  // draws used
+    int count2;    // tries made in all
+    int pos;       // first draw of the last try, when done
+    int nneg;      // FASTSURF clamps among the tries
+    bool done;
+  };
+  // (the two words of each set are passed apart, so no array is indexed at run
+  // time and the walk stays in registers)
+  static EMC_HD EMC_INL bool gs_bit(uint64_t m0, uint64_t m1, int q) {
+    return ((q < 64 ? m0 : m1) >> (q & 63)) & 1u;
+  }
+  static EMC_HD EMC_INL GsWalk gs_walk(uint64_t span0, uint64_t span1, uint64_t acc0, uint64_t acc1, uint64_t neg0,
+                                       uint64_t neg1, int count2) {
+    GsWalk w{GS_CHUNK, count2, -1, 0, false};
+    int p = 0, c = count2;
+    while (p < GS_CHUNK) {
+      // tries p, p+1, .. q-1 miss the energy: one draw each
+      int q = GS_CHUNK;
+      if (p < 64) {
+        const uint64_t m = span0 >> p;
+        if (m) q = p + __builtin_ctzll(m);
+        else if (span1) q = 64 + __builtin_ctzll(span1);
+      } else {
+        const uint64_t m = span1 >> (p - 64);
+        if (m) q = p + __builtin_ctzll(m);
+      }
+      if (c + (q - p) >= GS_CAP) {
+        // the last try is among them: it takes its second draw whatever it picked
+        const int t = p + (GS_CAP - 1 - c);
+        if (t + 1 >= GS_CHUNK) { w.consumed = t; w.count2 = GS_CAP - 1; return w; }
+        w.consumed = t + 2; w.count2 = GS_CAP; w.pos = t; w.done = true;
+        return w;
+      }
+      c += q - p;
+      p = q;
+      if (p >= GS_CHUNK) break;
+      if (p + 1 >= GS_CHUNK) { w.consumed = p; w.count2 = c; return w; }
+      c += 1;
+      if (gs_bit(neg0, neg1, p)) w.nneg++;
+      if (c >= GS_CAP || gs_bit(acc0, acc1, p)) { w.consumed = p + 2; w.count2 = c; w.pos = p; w.done = true; return w; }
+      p += 2;
+    }
+    w.consumed = GS_CHUNK;
+    w.count2 = c;
+    return w;
+  }
+  // Draw position q of a chunk that starts at Philox block blk, after the spare
+  // draw sp when have is set; block words as in Rng::next.
+  static EMC_HD EMC_INL double gs_word(const uint32_t w[4], int part) {
+    const uint64_t v = part ? ((((uint64_t)w[3] << 32) | w[2]) >> 12) : ((((uint64_t)w[1] << 32) | w[0]) >> 12);
+    return ((double)v + 0.5) * 2.220446049250313080847263336181640625e-16;
+  }
+  // the stream state after k draws of a chunk: block advanced past every block
+  // whose first word was used; have set when the next draw is a second word
+  // (the caller then sets spare to it, unless have was set and k == 0)
+  static EMC_HD EMC_INL void gs_advance(uint32_t& blk, int& have, int k) {
+    blk += (uint32_t)(have ? k / 2 : (k + 1) / 2);
+    have = have ? !(k & 1) : (k & 1);
+  }
+  EMC_HD double gs_draw(uint32_t blk, int have, double sp, int q) const {
+    const int r = q - have;
+    if (r < 0) return sp;
+    uint32_t w[4] = {blk + (uint32_t)(r >> 1), rng.c1, rng.c2, 0u};
+    philox4x32_10(w, rng.k0, rng.k1);
+    return gs_word(w, r & 1);
+  }
+  // get_state() with the tries after the first `serial` taken in chunks, on one
+  // thread: the restatement of get_state_wave that the host checks
+  EMC_HD void get_state_chunked(int serial) {
+    const double e = energy;
+    double maxdostet = 0;
+    int nted = 0, base = 0;
+    bool small = false;
+    if (!gs_begin(maxdostet, nted, base, small)) return;
+    int count2 = 0;
+    bool again = true;
+    while (again && count2 < serial) again = gs_try(e, maxdostet, nted, base, small, count2);
+    while (again) {
+      uint64_t span[2] = {0, 0}, acc[2] = {0, 0}, neg[2] = {0, 0};
+      int its[GS_CHUNK];
+      double u[GS_CHUNK], dost[GS_CHUNK];
+      for (int q = 0; q < GS_CHUNK; ++q) u[q] = gs_draw(rng.block, rng.have, rng.spare, q);
+      for (int q = 0; q < GS_CHUNK; ++q) {
+        bool sp = false, ng = false;
+        gs_eval(e, nted, base, small, u[q], its[q], sp, ng, dost[q]);
+        if (sp) span[q >> 6] |= 1ull << (q & 63);
+        if (ng) neg[q >> 6] |= 1ull << (q & 63);
+        if (sp && q + 1 < GS_CHUNK && !((u[q + 1] * maxdostet) > dost[q])) acc[q >> 6] |= 1ull << (q & 63);
+      }
+      const GsWalk w = gs_walk(span[0], span[1], acc[0], acc[1], neg[0], neg[1], count2);
+      if (w.nneg && T.counters) tally_add(&T.counters[CNT_DROP0 + 6], w.nneg);
+      count2 = w.count2;
+      if (w.done) { itet = its[w.pos]; again = false; }
+      uint32_t blk = rng.block;
+      int have = rng.have;
+      const double next = gs_draw(blk, have, rng.spare, w.consumed);
+      const bool keep = have && w.consumed == 0;
+      gs_advance(blk, have, w.consumed);
+      if (have && !keep) rng.spare = next;
+      rng.block = blk;
+      rng.have = have;
+    }
+    gs_end(count2);
   }
 
   // TALIND (src/ParticleUtils.cpp:626-748)
@@ -1032,67 +1207,78 @@ struct Flight {
     x += vx * Tf; y += vy * Tf; z += vz * Tf;
   }
 
+  // One iteration of fly()'s event loop, split at the final-state draw so that
+  // the slow kernel can run get_state() with the whole wave (get_state_wave):
+  // event_head() runs up to it (false: the particle stops, fly() leaves its
+  // loop; post says whether a final state is drawn), get_state() follows when
+  // post is set, then event_tail() (false: the particle was caught).
+  EMC_HD EMC_INL bool event_head(Loop& L, int& flag, int& post, Saved& s0) {
+    GetV();
+    L.loop++;
+    n_events++;
+    if (L.loop > FLY_LOOP_CAP) { Flag_Catch = 1; drop_reason = 3; return false; }
+    double Tf, gam;
+    size_t c;
+    flag = next_event(L, Tf, gam, c);
+    if (Tf < MY_ZERO) { Flag_Catch = 1; drop_reason = 0; return false; }
+    advance(L, Tf, gam);
+    if (energy < 0) { Flag_Catch = 1; drop_reason = 1; return false; }
+    post = POST_NONE;
+    switch (flag) {
+      case 1:
+        HitTet();
+        L.getTet = L.getCell = true;
+        break;
+      case 2: {
+        bool diffuse = false;
+        if (HitCell(diffuse)) L.getTet = L.getCell = true;
+        else { Flag_Catch = 1; drop_reason = 4; }
+        if (diffuse) post = prep_diffuse(s0);
+        break;
+      }
+      case 3:
+        post = prep_phonon(s0);
+        break;
+      case 4:
+        if (par_type == PELEC) ElectronImpurityScatter();
+        if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_impurity++; }
+        L.getImp = true;
+        break;
+      case 5:
+        // ParticleSurfaceScatter (src/ParSurfaceScatter.cpp:48-61)
+        if (Random() * SurfScGamma < SurfRoughnessScRate) {
+          idir = M.surface_dir[M.nearest_surf[c]];
+          post = prep_diffuse(s0);
+        } else {
+          Flag_SelfScatter = 0;
+          post = select_prep(Random() * calscattsum(energy, iband), s0);
+        }
+        break;
+      default:
+        break;
+    }
+    return true;
+  }
+  EMC_HD EMC_INL bool event_tail(Loop& L, int flag, int post, const Saved& s0) {
+    if (post != POST_NONE) finish(post, s0);
+    if (flag == 3) {
+      if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_phonon++; }
+      L.getPh = true;
+    } else if (flag == 5) {
+      if (!Flag_SelfScatter) L.getTet = L.getCell = true;
+      L.getSurf = true;
+    }
+    return !Flag_Catch;
+  }
   EMC_HD EMC_INL void fly() {
     Loop L;
     fly_begin(L);
     while (left_time > 0) {
-      GetV();
-      L.loop++;
-      n_events++;
-      if (L.loop > FLY_LOOP_CAP) { Flag_Catch = 1; drop_reason = 3; break; }
-      double Tf, gam;
-      size_t c;
-      const int flag = next_event(L, Tf, gam, c);
-      if (Tf < MY_ZERO) { Flag_Catch = 1; drop_reason = 0; break; }
-      advance(L, Tf, gam);
-      if (energy < 0) { Flag_Catch = 1; drop_reason = 1; break; }
-      int post = POST_NONE;
+      int flag = 0, post = POST_NONE;
       Saved s0{};
-      switch (flag) {
-        case 1:
-          HitTet();
-          L.getTet = L.getCell = true;
-          break;
-        case 2: {
-          bool diffuse = false;
-          if (HitCell(diffuse)) L.getTet = L.getCell = true;
-          else { Flag_Catch = 1; drop_reason = 4; }
-          if (diffuse) post = prep_diffuse(s0);
-          break;
-        }
-        case 3:
-          post = prep_phonon(s0);
-          break;
-        case 4:
-          if (par_type == PELEC) ElectronImpurityScatter();
-          if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_impurity++; }
-          L.getImp = true;
-          break;
-        case 5:
-          // ParticleSurfaceScatter (src/ParSurfaceScatter.cpp:48-61)
-          if (Random() * SurfScGamma < SurfRoughnessScRate) {
-            idir = M.surface_dir[M.nearest_surf[c]];
-            post = prep_diffuse(s0);
-          } else {
-            Flag_SelfScatter = 0;
-            post = select_prep(Random() * calscattsum(energy, iband), s0);
-          }
-          break;
-        default:
-          break;
-      }
-      if (post != POST_NONE) {
-        get_state();
-        finish(post, s0);
-      }
-      if (flag == 3) {
-        if (!Flag_SelfScatter) { L.getTet = L.getCell = true; n_phonon++; }
-        L.getPh = true;
-      } else if (flag == 5) {
-        if (!Flag_SelfScatter) L.getTet = L.getCell = true;
-        L.getSurf = true;
-      }
-      if (Flag_Catch) break;
+      if (!event_head(L, flag, post, s0)) break;
+      if (post != POST_NONE) get_state();
+      if (!event_tail(L, flag, post, s0)) break;
     }
   }
 

@@ -134,6 +134,110 @@ __global__ __launch_bounds__(256, 3) void k_fly_fast(AS as_) {
   if (lane == 0 && ev) atomicAdd(&a.T.counters[CNT_EVENTS], (unsigned long long)ev);
 }
 
+// get_state() for the lanes of a wave that want a final state, with the wave's
+// 64 lanes sharing a search that runs long.  Each lane first makes up to
+// GS_SERIAL tries of its own (the common case ends there).  The lanes still
+// drawing are then served one at a time: the lane's search is broadcast, every
+// lane forms two positions of a chunk of 128 of its draws (lane j: Philox block
+// j of the chunk, its words spread so that lane j holds positions j and j + 64)
+// and evaluates them, and Flight::gs_walk replays the loop over the ballots.
+// The draws, tries, tetrahedron and tallies are get_state()'s
+// (Flight::get_state_chunked restates this on one thread for the host check).
+// All lanes of the wave must call it together.
+constexpr int GS_SERIAL = 32;
+__device__ __forceinline__ void get_state_wave(Flight& f, bool want) {
+  if (!__ballot(want)) return;
+  const int lane = threadIdx.x & 63;
+  double maxdostet = 0;
+  int nted = 0, base = 0;
+  bool small = false;
+  const bool drew = want && f.gs_begin(maxdostet, nted, base, small);
+  bool again = drew;
+  int count2 = 0;
+  while (__ballot(again && count2 < GS_SERIAL)) {
+    if (!(again && count2 < GS_SERIAL)) continue;
+    again = f.gs_try(f.energy, maxdostet, nted, base, small, count2);
+  }
+  for (unsigned long long need = __ballot(again); need; need &= need - 1) {
+    const int L = __builtin_ctzll(need);
+    const double e = __shfl(f.energy, L, 64), umax = __shfl(maxdostet, L, 64);
+    const int unted = __shfl(nted, L, 64), ubase = __shfl(base, L, 64);
+    const bool usmall = __shfl((int)small, L, 64) != 0;
+    const uint32_t k0 = __shfl(f.rng.k0, L, 64), k1 = __shfl(f.rng.k1, L, 64);
+    const uint32_t c1 = __shfl(f.rng.c1, L, 64), c2 = __shfl(f.rng.c2, L, 64);
+    uint32_t blk = __shfl(f.rng.block, L, 64);
+    int have = __shfl(f.rng.have, L, 64);
+    double spare = __shfl(f.rng.spare, L, 64);
+    int c = __shfl(count2, L, 64), fin = 0, nneg = 0;
+    bool done = false;
+    while (!done) {
+      uint32_t w[4] = {blk + (uint32_t)lane, c1, c2, 0u};
+      philox4x32_10(w, k0, k1);
+      const double wa = Flight::gs_word(w, 0), wb = Flight::gs_word(w, 1);
+      // position q: draw q - have of the blocks (word (q - have) & 1 of block
+      // (q - have) / 2, held by that lane), or the spare draw
+      double u[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h - have;
+        const int src = r < 0 ? 0 : r >> 1;
+        const double va = __shfl(wa, src, 64), vb = __shfl(wb, src, 64);
+        u[h] = r < 0 ? spare : ((r & 1) ? vb : va);
+      }
+      int it[2];
+      bool sp[2], ng[2];
+      double dost[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) f.gs_eval(e, unted, ubase, usmall, u[h], it[h], sp[h], ng[h], dost[h]);
+      // the draw after position q: lane + 1's, across the halves at lane 63
+      const double n0a = __shfl(u[0], (lane + 1) & 63, 64), n0b = __shfl(u[1], 0, 64);
+      const double n1 = __shfl(u[1], (lane + 1) & 63, 64);
+      const double nx0 = lane < 63 ? n0a : n0b;
+      const bool acc0 = sp[0] && !((nx0 * umax) > dost[0]);
+      const bool acc1 = sp[1] && lane < 63 && !((n1 * umax) > dost[1]);
+      const Flight::GsWalk wk = Flight::gs_walk(__ballot(sp[0]), __ballot(sp[1]), __ballot(acc0), __ballot(acc1),
+                                                __ballot(ng[0]), __ballot(ng[1]), c);
+      nneg += wk.nneg;
+      c = wk.count2;
+      if (wk.done) {
+        fin = __shfl(wk.pos < 64 ? it[0] : it[1], wk.pos & 63, 64);
+        done = true;
+      }
+      const int k = wk.consumed;
+      const bool keep = have && k == 0;
+      const int oldh = have;
+      Flight::gs_advance(blk, have, k);
+      if (have && !keep) spare = __shfl(wb, (k - 1 - oldh) / 2, 64);
+    }
+    if (lane == L) {
+      f.itet = fin;
+      count2 = c;
+      f.rng.block = blk;
+      f.rng.have = have;
+      f.rng.spare = spare;
+      again = false;
+      if (nneg && f.T.counters) tally_add(&f.T.counters[CNT_DROP0 + 6], nneg);
+    }
+  }
+  if (drew) f.gs_end(count2);
+}
+
+// Flight::fly() with the event loop run until the wave's last lane leaves it, so
+// that get_state_wave has every lane.  Lanes with act false take part without a
+// particle.
+__device__ __forceinline__ void fly_wave(Flight& f, bool act) {
+  Flight::Loop L;
+  if (act) f.fly_begin(L);
+  bool alive = act && f.left_time > 0;
+  while (__ballot(alive)) {
+    int flag = 0, post = Flight::POST_NONE;
+    Flight::Saved s0{};
+    if (alive && !f.event_head(L, flag, post, s0)) alive = false;
+    get_state_wave(f, alive && post != Flight::POST_NONE);
+    if (alive) alive = f.event_tail(L, flag, post, s0) && f.left_time > 0;
+  }
+}
+
 // Full event loop for the particles k_fly_fast listed.  These histories diverge
 // (scattering, zone faces, contacts), and a wave runs its lanes' different paths
 // one after the other, so the list is spread thinly: each wave takes `lpw` entries
@@ -149,28 +253,33 @@ __global__ __launch_bounds__(256) void k_fly_slow(AS as_) {
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-  if (lane < lpw) {
-    for (long long e = wave * lpw + lane; e < ns; e += nwaves * lpw) {
-      const long long q = a.slow[e];
-      const uint32_t pk = a.P.pk[q];
-      Flight f(a.C->B, a.C->M, a.C->F, a.T);
-      load_particle(f, a, q, pk);
-      f.fly();
-      ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
-      if (f.Flag_Catch) {
-        ++ca;
-        finish_drop(a.T, f);
-        a.P.pk[q] = PK_DEAD;
-        const unsigned long long d = atomicAdd(a.ndead, 1ull);
-        if (d < a.dead_cap) a.dead[d] = q;
-        continue;
-      }
-      a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
-      a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
-      a.P.energy[q] = f.energy;
-      a.P.pk[q] = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
-      a.P.itet[q] = f.itet; a.P.isym[q] = f.isym;
+  // every lane of the wave runs the loop (fly_wave needs them all); lanes from
+  // lpw on, and lanes past the list, fly nothing
+  for (long long e = wave * lpw + lane;; e += nwaves * lpw) {
+    const bool act = lane < lpw && e < ns;
+    if (!__ballot(act)) break;
+    long long q = 0;
+    Flight f(a.C->B, a.C->M, a.C->F, a.T);
+    if (act) {
+      q = a.slow[e];
+      load_particle(f, a, q, a.P.pk[q]);
     }
+    fly_wave(f, act);
+    if (!act) continue;
+    ev += f.n_events; ph += f.n_phonon; im += f.n_impurity;
+    if (f.Flag_Catch) {
+      ++ca;
+      finish_drop(a.T, f);
+      a.P.pk[q] = PK_DEAD;
+      const unsigned long long d = atomicAdd(a.ndead, 1ull);
+      if (d < a.dead_cap) a.dead[d] = q;
+      continue;
+    }
+    a.P.x[q] = f.x; a.P.y[q] = f.y; a.P.z[q] = f.z;
+    a.P.kx[q] = f.kx; a.P.ky[q] = f.ky; a.P.kz[q] = f.kz;
+    a.P.energy[q] = f.energy;
+    a.P.pk[q] = pk_make(f.par_type, f.icell, f.jcell, f.kcell);
+    a.P.itet[q] = f.itet; a.P.isym[q] = f.isym;
   }
   flush_counters(a.T, ev, ph, im, ca);
 }

@@ -306,9 +306,10 @@ class Engine:
                                              _ptr(it, C.c_int), _ptr(sy, C.c_int)))
         return it, sy
 
-    def band_final_state(self, key, step, energy, iband, isym, itet):
+    def band_final_state(self, key, step, energy, iband, isym, itet, serial=-1):
         """get_state with getK on the host tables for n draws: returns (k [n, 3],
-        itet, ndraws, selfsc)."""
+        itet, ndraws, selfsc).  serial >= 0: the tries after the first `serial` in
+        chunks of 128 draws, as the slow flight kernel takes them."""
         key = np.ascontiguousarray(key, np.uint64).reshape(-1)
         n = len(key)
         e = np.ascontiguousarray(energy, np.float64).reshape(n)
@@ -320,7 +321,8 @@ class Engine:
         sc = np.zeros(n, np.int32)
         self._check(self.lib.emc_band_final_state(self.ctx, n, _ptr(key, C.c_uint64), step, _ptr(e, C.c_double),
                                                   _ptr(ib, C.c_int), _ptr(sy, C.c_int), _ptr(k, C.c_double),
-                                                  _ptr(it, C.c_int), _ptr(nd, C.c_int), _ptr(sc, C.c_int)))
+                                                  _ptr(it, C.c_int), _ptr(nd, C.c_int), _ptr(sc, C.c_int),
+                                                  int(serial)))
         return k, it, nd, sc
 
     def mg_levels(self):

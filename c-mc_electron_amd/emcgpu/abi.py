@@ -171,7 +171,7 @@ def declare(lib):
         "emc_mesh_view_get": (C.c_int, [ctx, P(MeshView)]),
         "emc_band_locate": (C.c_int, [ctx, C.c_int64, _d, _d, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "emc_band_final_state": (C.c_int, [ctx, C.c_int64, _u64, C.c_uint64, _d, P(C.c_int), P(C.c_int), _d,
-                                           P(C.c_int), P(C.c_int), P(C.c_int)]),
+                                           P(C.c_int), P(C.c_int), P(C.c_int), C.c_int]),
         "emc_multiple_refresh": (C.c_int, [ctx, C.c_uint64, P(C.c_int64), P(C.c_int64)]),
         "emc_checkpoint_save": (C.c_int, [ctx, C.c_char_p]),
         "emc_checkpoint_load": (C.c_int, [ctx, C.c_char_p]),

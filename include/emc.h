@@ -263,10 +263,14 @@ int emc_band_locate(emc_ctx *ctx, int64_t n, const double *k, const double *ener
  * stream of (key[q], step).  itet[q] is read as the particle's tetrahedron and
  * receives the chosen one, k[3q .. 3q+2] the wave vector (normalised; 0 on a
  * self-scattering), ndraws[q] the random numbers used and selfsc[q] 1 where the
- * draw self-scatters (no list entry, or the 10000-try limit).  No counterpart
- * in the reference: a seam for tests, like emc_band_locate.  Needs no GPU. */
+ * draw self-scatters (no list entry, or the 10000-try limit).  serial < 0 runs
+ * the loop of get_state as written; serial >= 0 takes the tries after the first
+ * `serial` in chunks of 128 draws, as the slow flight kernel's wave-parallel form
+ * does (the same results by construction; this is its check).  No counterpart in
+ * the reference: a seam for tests, like emc_band_locate.  Needs no GPU. */
 int emc_band_final_state(emc_ctx *ctx, int64_t n, const uint64_t *key, uint64_t step, const double *energy,
-                         const int *iband, const int *isym, double *k, int *itet, int *ndraws, int *selfsc);
+                         const int *iband, const int *isym, double *k, int *itet, int *ndraws, int *selfsc,
+                         int serial);
 
 /* Initial population (host sampling as init_particle_data), uploaded to the GPU
  * when one is attached.  n_out receives the particle count. */

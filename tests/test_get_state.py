@@ -31,7 +31,10 @@ CASES = [  # (band, energy eV)
 ]
 
 
-def test_final_state_equals_the_oracle(eng):
+# serial -1: get_state's loop as written; 0, 1, 7, 32: the tries after the first
+# `serial` in chunks of 128 draws (the slow flight kernel's wave-parallel form)
+@pytest.mark.parametrize("serial", [-1, 0, 1, 7, 32])
+def test_final_state_equals_the_oracle(eng, serial):
     lib = oracle_lib.load()
     v = eng.band
     rng = np.random.default_rng(11)
@@ -42,7 +45,7 @@ def test_final_state_equals_the_oracle(eng):
         isym = rng.integers(0, 48, n_per).astype(np.int32)
         energy = np.full(n_per, E)
         iband = np.full(n_per, ib, np.int32)
-        k, it, nd, sc = eng.band_final_state(keys, 5, energy, iband, isym, np.zeros(n_per, np.int32))
+        k, it, nd, sc = eng.band_final_state(keys, 5, energy, iband, isym, np.zeros(n_per, np.int32), serial)
         ko = np.zeros(3)
         ito = C.c_int(0)
         ndo = C.c_int(0)
