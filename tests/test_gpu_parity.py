@@ -348,8 +348,11 @@ def test_flight_parity_near_band_edge(eng):
     energies (20.5, 48.9 and 62.7 meV; their wave vectors and tetrahedra from
     emc_band_final_state), in a weak field, twenty steps: phonon emission leaves many
     of them 1-3 meV above the band edge, where get_state's rejection loop takes
-    hundreds to thousands of tries, while the other lanes of the wave take their own
-    paths.  Population and tallies equal the oracle's bit for bit."""
+    hundreds to thousands of tries (at 1.6 meV about one search in five reaches the
+    10 000-try limit), while the other lanes of the wave take their own paths.  The
+    slow kernel takes those searches with the whole wave, 128 draws at a time after
+    4 tries of the particle's own.  Population and tallies equal the oracle's bit
+    for bit."""
     eng.particles_init()
     p = eng.download()
     n = p.n
