@@ -555,44 +555,64 @@ struct Flight {
     int nneg;      // FASTSURF clamps among the tries
     bool done;
   };
-  // (the two words of each set are passed apart, so no array is indexed at run
-  // time and the walk stays in registers)
-  static EMC_HD EMC_INL bool gs_bit(uint64_t m0, uint64_t m1, int q) {
-    return ((q < 64 ? m0 : m1) >> (q & 63)) & 1u;
+  // The two words of each set are passed apart (no array indexed at run time).
+  // The tries are found with word operations instead of one by one: a try that
+  // starts at a spanning position takes the next position as its acceptance
+  // draw, so within a run of spanning positions the tries start at every other
+  // position from the run's first, and a run begins with a try.  Runs are split
+  // by the parity of their first position with a carry (adding the run's first
+  // bit clears the run); the tries of the chunk are then counted with popcounts
+  // and the try the limit falls on is selected.
+  static EMC_HD EMC_INL uint64_t gs_span_starts(uint64_t S, uint64_t first_odd) {
+    const uint64_t E = 0x5555555555555555ull;  // even positions
+    const uint64_t RS = S & ~(S << 1);         // first positions of runs
+    const uint64_t even_first = RS & E & ~first_odd;
+    const uint64_t run_even = S & ~(S + even_first);
+    return (run_even & E) | (S & ~run_even & ~E);
+  }
+  // set bits at positions 0..q of the 128-bit set (m0, m1)
+  static EMC_HD EMC_INL int gs_pop_upto(uint64_t m0, uint64_t m1, int q) {
+    if (q < 64) return __builtin_popcountll(m0 & ((2ull << q) - 1ull));
+    return __builtin_popcountll(m0) + __builtin_popcountll(m1 & ((2ull << (q - 64)) - 1ull));
+  }
+  // position of the k-th set bit (k >= 1), or -1
+  static EMC_HD EMC_INL int gs_select(uint64_t m0, uint64_t m1, int k) {
+    uint64_t m = m0;
+    int off = 0;
+    const int p0 = __builtin_popcountll(m0);
+    if (k > p0) { k -= p0; m = m1; off = 64; }
+    if (k > __builtin_popcountll(m)) return -1;
+    for (int i = 1; i < k; ++i) m &= m - 1ull;
+    return off + __builtin_ctzll(m);
   }
   static EMC_HD EMC_INL GsWalk gs_walk(uint64_t span0, uint64_t span1, uint64_t acc0, uint64_t acc1, uint64_t neg0,
                                        uint64_t neg1, int count2) {
+    // spanning tries (TS) and all tries (T); position 64 is an acceptance draw
+    // when a spanning try starts at 63
+    const uint64_t ts0 = gs_span_starts(span0, 0);
+    const uint64_t a64 = ts0 >> 63;
+    const uint64_t ts1 = gs_span_starts(span1, a64 & span1 & 1ull);
+    const uint64_t t0 = ~(ts0 << 1), t1 = ~((ts1 << 1) | a64);
+    const uint64_t ac0 = acc0 & ts0, ac1 = acc1 & ts1, ng0 = neg0 & ts0, ng1 = neg1 & ts1;
+    const int qa = ac0 ? __builtin_ctzll(ac0) : ac1 ? 64 + __builtin_ctzll(ac1) : -1;  // first accepted
+    const int tc = gs_select(t0, t1, GS_CAP - count2);  // the try the limit falls on
+    const int last = (tc >= 0 && (qa < 0 || tc <= qa)) ? tc : qa;
     GsWalk w{GS_CHUNK, count2, -1, 0, false};
-    int p = 0, c = count2;
-    while (p < GS_CHUNK) {
-      // tries p, p+1, .. q-1 miss the energy: one draw each
-      int q = GS_CHUNK;
-      if (p < 64) {
-        const uint64_t m = span0 >> p;
-        if (m) q = p + __builtin_ctzll(m);
-        else if (span1) q = 64 + __builtin_ctzll(span1);
-      } else {
-        const uint64_t m = span1 >> (p - 64);
-        if (m) q = p + __builtin_ctzll(m);
-      }
-      if (c + (q - p) >= GS_CAP) {
-        // the last try is among them: it takes its second draw whatever it picked
-        const int t = p + (GS_CAP - 1 - c);
-        if (t + 1 >= GS_CHUNK) { w.consumed = t; w.count2 = GS_CAP - 1; return w; }
-        w.consumed = t + 2; w.count2 = GS_CAP; w.pos = t; w.done = true;
-        return w;
-      }
-      c += q - p;
-      p = q;
-      if (p >= GS_CHUNK) break;
-      if (p + 1 >= GS_CHUNK) { w.consumed = p; w.count2 = c; return w; }
-      c += 1;
-      if (gs_bit(neg0, neg1, p)) w.nneg++;
-      if (c >= GS_CAP || gs_bit(acc0, acc1, p)) { w.consumed = p + 2; w.count2 = c; w.pos = p; w.done = true; return w; }
-      p += 2;
+    if (last >= 0 && last + 1 < GS_CHUNK) {
+      w.consumed = last + 2;
+      w.count2 = count2 + gs_pop_upto(t0, t1, last);
+      w.nneg = gs_pop_upto(ng0, ng1, last);
+      w.pos = last;
+      w.done = true;
+    } else if (last >= 0 || (ts1 >> 63)) {
+      // the chunk's last try needs the draw after it: left to the next chunk
+      w.consumed = GS_CHUNK - 1;
+      w.count2 = count2 + gs_pop_upto(t0, t1, GS_CHUNK - 2);
+      w.nneg = gs_pop_upto(ng0, ng1, GS_CHUNK - 2);
+    } else {
+      w.count2 = count2 + __builtin_popcountll(t0) + __builtin_popcountll(t1);
+      w.nneg = __builtin_popcountll(ng0) + __builtin_popcountll(ng1);
     }
-    w.consumed = GS_CHUNK;
-    w.count2 = c;
     return w;
   }
   // Draw position q of a chunk that starts at Philox block blk, after the spare
