@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256, 3) void k_fly_fast(AS as_) {
 // The draws, tries, tetrahedron and tallies are get_state()'s
 // (Flight::get_state_chunked restates this on one thread for the host check).
 // All lanes of the wave must call it together.
-constexpr int GS_SERIAL = 4;
+constexpr int GS_SERIAL = 1;
 __device__ __forceinline__ void get_state_wave(Flight& f, bool want) {
   if (!__ballot(want)) return;
   const int lane = threadIdx.x & 63;

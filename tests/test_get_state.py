@@ -31,8 +31,8 @@ CASES = [  # (band, energy eV)
 ]
 
 
-# serial -1: get_state's loop as written; 0, 1, 4 (the slow flight kernel's
-# value), 7, 32: the tries after the first `serial` in chunks of 128 draws, as the
+# serial -1: get_state's loop as written; 0, 1 (the slow flight kernel's value),
+# 4, 7, 32: the tries after the first `serial` in chunks of 128 draws, as the
 # kernel's wave-parallel form takes them
 @pytest.mark.parametrize("serial", [-1, 0, 1, 4, 7, 32])
 def test_final_state_equals_the_oracle(eng, serial):

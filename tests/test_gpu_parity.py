@@ -351,7 +351,7 @@ def test_flight_parity_near_band_edge(eng):
     hundreds to thousands of tries (at 1.6 meV about one search in five reaches the
     10 000-try limit), while the other lanes of the wave take their own paths.  The
     slow kernel takes those searches with the whole wave, 128 draws at a time after
-    4 tries of the particle's own.  Population and tallies equal the oracle's bit
+    the particle's first try.  Population and tallies equal the oracle's bit
     for bit."""
     eng.particles_init()
     p = eng.download()
